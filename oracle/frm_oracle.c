@@ -1,0 +1,705 @@
+/* CPU oracle for the fractional-solver hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates the reference's CPU algorithms (sevaboh/theses_code) so tests,
+ * the smoke check and the bench's CPU baseline can check the HIP path.  The
+ * product never links or calls this file.  Operation order follows the
+ * reference wherever rounding matters; every block cites the reference lines
+ * it restates.  Build: oracle/Makefile (-O2 -ffp-contract=off, no FMA
+ * contraction, so results reproduce the reference's x86-64 build).
+ *
+ * Parity status of each function is recorded next to it (pinned = checked
+ * against reference-written golden vectors and/or closed forms in tests/).
+ */
+#include "frm_oracle.h"
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ===================================================================== */
+/* L0: Gamma / LogGamma                                                   */
+/* ===================================================================== */
+/* W. J. Cody's rational approximation of Gamma on (1,2) with argument
+ * reduction, A&S 6.1.41 asymptotics for log Gamma above 12; the coefficient
+ * table is the published one the reference uses (mt_minimal.cpp:20-189,
+ * MF_int_series.cpp:25-193).  Pinned: tests compare with math.gamma. */
+static const double cody_p[8] = {
+    -1.71618513886549492533811E+0, 2.47656508055759199108314E+1,
+    -3.79804256470945635097577E+2, 6.29331155312818442661052E+2,
+    8.66966202790413211295064E+2,  -3.14512729688483675254357E+4,
+    -3.61444134186911729807069E+4, 6.64561438202405440627855E+4};
+static const double cody_q[8] = {
+    -3.08402300119738975254353E+1, 3.15350626979604161529144E+2,
+    -1.01515636749021914166146E+3, -3.10777167157231109440444E+3,
+    2.25381184209801510330112E+4,  4.75584627752788110767815E+3,
+    -1.34659959864969306392456E+5, -1.15132259675553483497211E+5};
+
+double orc_log_gamma(double x) {
+    if (x <= 0.0) return orc_log_gamma(x + 1.0) - log(x); /* MF_int_series.cpp:37 */
+    if (x < 12.0) return log(fabs(orc_gamma(x)));
+    static const double c[8] = {1.0 / 12.0,   -1.0 / 360.0,        1.0 / 1260.0, -1.0 / 1680.0,
+                                1.0 / 1188.0, -691.0 / 360360.0, 1.0 / 156.0,  -3617.0 / 122400.0};
+    double z = 1.0 / (x * x);
+    double acc = c[7];
+    for (int k = 6; k >= 0; --k) {
+        acc *= z;
+        acc += c[k];
+    }
+    return (x - 0.5) * log(x) - x + 0.91893853320467274178032973640562 + acc / x;
+}
+
+double orc_gamma(double x) {
+    if (x <= 0.0) return (1.0 / x) * orc_gamma(x + 1.0); /* MF_int_series.cpp:81 */
+    if (x < 0.001) return 1.0 / (x * (1.0 + 0.577215664901532860606512090 * x));
+    if (x < 12.0) {
+        double y = x;
+        int shift = 0;
+        int below_one = (y < 1.0);
+        if (below_one)
+            y += 1.0;
+        else {
+            shift = (int)floor(y) - 1;
+            y -= shift;
+        }
+        double num = 0.0, den = 1.0, z = y - 1;
+        for (int k = 0; k < 8; ++k) {
+            num = (num + cody_p[k]) * z;
+            den = den * z + cody_q[k];
+        }
+        double r = num / den + 1.0;
+        if (below_one)
+            r /= (y - 1.0);
+        else
+            for (int k = 0; k < shift; ++k) r *= y++;
+        return r;
+    }
+    if (x > 171.624) return DBL_MAX * 2.0;
+    return exp(orc_log_gamma(x));
+}
+
+/* ===================================================================== */
+/* 2D LOD solver (fr_d_t_g_2d.cpp)                                        */
+/* ===================================================================== */
+/* g(t) and the derivatives of its inverse, fr_d_t_g_2d.cpp:2823-2870
+ * (func 0: g = t; func 3: g = t^p). */
+static double tg_g(int func, double p, double t) { return func == 3 ? pow(t, p) : t; }
+static double tg_inv_g_der(int func, double p, double t, int n, double d) {
+    if (func == 3) {
+        if (n == 0) return pow(t, 1.0 / p);
+        if (n == 1) return ((1.0 / p) * pow(t, (1.0 / p) - 1)) * (d / pow(t, 1.0 / p));
+        return d * ((1.0 / p) - n + 1) / t;
+    }
+    return (n == 0) ? t : ((n == 1) ? (d / t) : 0);
+}
+
+/* Taylor series of (g(tj)-g(x))^-alpha around g(tj), integrated over
+ * [t0,t1]: _kb_row, fr_d_t_g_2d.cpp:2937-2963.  For g = t the series
+ * ends after its first term (the inverse's second derivative is 0), giving
+ * gtj^(1-a) * ((1-t0/gtj)^(1-a) - (1-t1/gtj)^(1-a)) / (1-a). */
+static double tg_kb_row(int func, double p, double t0, double t1, double gtj, double alpha,
+                        double int_err) {
+    double sum = 0.0, v, v2, v3, v4;
+    int i = 0;
+    v = tg_inv_g_der(func, p, gtj, 1, tg_inv_g_der(func, p, gtj, 0, 1));
+    v *= pow(gtj, 1 - alpha);
+    v3 = pow(1.0 - tg_g(func, p, t1) / gtj, 1 - alpha);
+    v4 = pow(1.0 - tg_g(func, p, t0) / gtj, 1 - alpha);
+    v2 = -(v3 - v4) / (1 - alpha);
+    sum += v2 * v;
+    while (fabs(v * v2) > int_err) {
+        i++;
+        v = tg_inv_g_der(func, p, gtj, i + 1, v);
+        v *= -gtj / i;
+        v3 *= 1.0 - tg_g(func, p, t1) / gtj;
+        v4 *= 1.0 - tg_g(func, p, t0) / gtj;
+        v2 = -(v3 - v4) / (i - alpha + 1);
+        sum += v2 * v;
+    }
+    return sum;
+}
+
+/* kb(s, ja, sa, l) at step n (fr_d_t_g_2d.cpp:3112-3131, int_alg = 1):
+ * Da * integral over [tau(s+sa), tau(s+sa+l)] of (g(tau(n+ja)) - g(x))^-alpha,
+ * zero when the interval lies past the target.  Pinned: golden kb tables and
+ * the g = t closed form. */
+double orc_tg_kb(int func, double p, double alpha, double tau, double int_err, int n, int s,
+                 double ja, double sa, double l) {
+    double Da = 1.0 / orc_gamma(1.0 - alpha);
+    if (tau * (s + sa + l) <= tau * (n + ja))
+        return Da * tg_kb_row(func, p, tau * (s + sa), tau * (s + sa + l),
+                              tg_g(func, p, tau * (n + ja)), alpha, int_err);
+    return 0.0;
+}
+
+struct orc_tg {
+    int sN, M, ld, func;
+    double p, alpha, tau, sigma, C1, S, int_err;
+    double *rowA, *rowB, *colA, *colB, *C;
+    double** old;
+    int nold, cap;
+    int tstep;
+    double *kb; /* [8][cap_kb]: the eight kb variants for s = 0..n */
+    int cap_kb;
+    double *Al, *Bt, *Om;
+};
+
+static double* dup_arr(const double* src, size_t n) {
+    double* d = (double*)malloc(n * sizeof(double));
+    memcpy(d, src, n * sizeof(double));
+    return d;
+}
+
+static void tg_push(orc_tg* t) {
+    size_t nn = (size_t)t->ld * (t->M + 2);
+    if (t->nold == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 64;
+        t->old = (double**)realloc(t->old, t->cap * sizeof(double*));
+    }
+    t->old[t->nold++] = dup_arr(t->C, nn);
+}
+
+orc_tg* orc_tg_create(int sN, int M, int func, double fpow, double alpha, double tau,
+                      double sigma, double C1, double S, double int_err, const double* rowA,
+                      const double* rowB, const double* colA, const double* colB,
+                      const double* C0) {
+    orc_tg* t = (orc_tg*)calloc(1, sizeof(orc_tg));
+    size_t nn = (size_t)(sN + 2) * (M + 2);
+    t->sN = sN;
+    t->M = M;
+    t->ld = sN + 2;
+    t->func = func;
+    t->p = (func == 0) ? 1.0 : fpow;
+    t->alpha = alpha;
+    t->tau = tau;
+    t->sigma = sigma;
+    t->C1 = C1;
+    t->S = S;
+    t->int_err = int_err;
+    t->rowA = dup_arr(rowA, nn);
+    t->rowB = dup_arr(rowB, nn);
+    t->colA = dup_arr(colA, nn);
+    t->colB = dup_arr(colB, nn);
+    t->C = dup_arr(C0, nn);
+    int mx = (sN > M ? sN : M) + 2;
+    t->Al = (double*)calloc(mx, sizeof(double));
+    t->Bt = (double*)calloc(mx, sizeof(double));
+    t->Om = (double*)calloc(mx, sizeof(double));
+    tg_push(t); /* oldC[0] = C_0, as initial() does (fr_d_t_g_2d.cpp:2294-2297) */
+    return t;
+}
+
+void orc_tg_destroy(orc_tg* t) {
+    if (!t) return;
+    for (int k = 0; k < t->nold; ++k) free(t->old[k]);
+    free(t->old);
+    free(t->rowA); free(t->rowB); free(t->colA); free(t->colB); free(t->C);
+    free(t->kb); free(t->Al); free(t->Bt); free(t->Om);
+    free(t);
+}
+
+/* weight variant index: bit0 ja==1, bit1 sa==1/2, bit2 l==1 (fr_d_t_g_2d.cpp:3077-3080) */
+#define TG_KB(t, v, s) ((t)->kb[(size_t)(v) * (t)->cap_kb + (s)])
+
+static void tg_fill_kb(orc_tg* t, int n) {
+    if (t->cap_kb < n + 1) {
+        t->cap_kb = 2 * (n + 1);
+        t->kb = (double*)realloc(t->kb, 8 * (size_t)t->cap_kb * sizeof(double));
+    }
+    for (int s = 0; s <= n; ++s)
+        for (int v = 0; v < 8; ++v)
+            TG_KB(t, v, s) = orc_tg_kb(t->func, t->p, t->alpha, t->tau, t->int_err, n, s,
+                                       (v & 1) ? 1.0 : 0.5, (v & 2) ? 0.5 : 0.0,
+                                       (v & 4) ? 1.0 : 0.5);
+}
+
+/* One sweep along a line (U1 = al1, Om1, bt1, back-substitution:
+ * fr_d_t_g_2d.cpp:2523-2551, 3267-3304, 3306-3492, 3494-3553).
+ * dir 0: row j (stride 1, nodes i = 0..sN+1); dir 1: column i (stride ld). */
+static void tg_line(orc_tg* t, int dir, int line, int n) {
+    const int N = dir == 0 ? t->sN : t->M;
+    const size_t base = dir == 0 ? (size_t)line * t->ld : (size_t)line;
+    const size_t stride = dir == 0 ? 1 : (size_t)t->ld;
+    const double* A = dir == 0 ? t->rowA : t->colA;
+    const double* B = dir == 0 ? t->rowB : t->colB;
+    double* U = t->C;
+    double* Al = t->Al;
+    double* Bt = t->Bt;
+    double* Om = t->Om;
+    const double sg = t->sigma / t->tau;
+    /* current-step weight: rows kb(n,1/2,0,1/2), columns kb(n,1,1/2,1/2) */
+    const double kcur = dir == 0 ? TG_KB(t, 0, n) : TG_KB(t, 3, n);
+#define NODE(i) (base + (size_t)(i) * stride)
+    Al[1] = dir == 0 ? 0 : 1;
+    for (int i = 1; i < N; ++i) {
+        double R = sg * kcur + A[NODE(i)] + B[NODE(i)];
+        Al[i + 1] = B[NODE(i)] / (R - A[NODE(i)] * Al[i]);
+    }
+    for (int i = 1; i < N; ++i) Om[i] = -sg * kcur * U[NODE(i)];
+    for (int i = 1; i < N; ++i) Om[i] += t->S * 0.5;
+    for (int i = 1; i < N; ++i) {
+        size_t id = NODE(i);
+        double ts = 0;
+        if (t->nold >= 2) {
+            if (t->nold >= 3)
+                for (int q = ((t->nold - 3) / 2) * 2; q >= 0; q -= 2) {
+                    /* full-step increments, newest first: kb(s,1/2,0,1) or kb(s,1,0,1) */
+                    double kv = dir == 0 ? TG_KB(t, 4, q / 2) : TG_KB(t, 5, q / 2);
+                    ts += kv * (t->old[q + 2][id] - t->old[q][id]);
+                }
+            if (dir == 1) {
+                /* the column half-step adds kb(n,1,0,1/2)*(C_n - C_{n+1/2}) */
+                double kv = TG_KB(t, 1, (t->nold - 1) / 2);
+                ts += kv * (t->old[t->nold - 2][id] - t->old[t->nold - 1][id]);
+            }
+        }
+        Om[i] += sg * ts;
+    }
+    Bt[1] = dir == 0 ? t->C1 : 0.0;
+    for (int i = 1; i < N; ++i) {
+        double a = A[NODE(i)], b = B[NODE(i)];
+        if (Al[i + 1] == 0.0 && b == 0.0) {
+            double R = sg * kcur + a + b;
+            Bt[i + 1] = (a * Bt[i] - Om[i]) / (R - a * Al[i]);
+        } else
+            Bt[i + 1] = (Al[i + 1] / b) * (a * Bt[i] - Om[i]);
+    }
+    U[NODE(N)] = Bt[N] / (1.0 - Al[N]);
+    for (int i = N - 1; i >= 0; --i) U[NODE(i)] = Al[i + 1] * U[NODE(i + 1)] + Bt[i + 1];
+    U[NODE(N + 1)] = U[NODE(N)];
+#undef NODE
+}
+
+void orc_tg_step(orc_tg* t) {
+    int n = t->tstep;
+    tg_fill_kb(t, n);
+    for (int j = 1; j < t->M; ++j) tg_line(t, 0, j, n); /* rows, target n+1/2 */
+    tg_push(t);                                          /* C_{n+1/2} */
+    for (int i = 1; i < t->sN; ++i) tg_line(t, 1, i, n); /* columns, target n+1 */
+    tg_push(t);                                          /* C_{n+1} */
+    t->tstep++;
+}
+
+void orc_tg_get(const orc_tg* t, double* C) {
+    memcpy(C, t->C, (size_t)t->ld * (t->M + 2) * sizeof(double));
+}
+int orc_tg_tstep(const orc_tg* t) { return t->tstep; }
+
+void orc_tg_coefs_ref(int sN, int M, const double* V2, double dv, double dL, double dLx,
+                      double* rowA, double* rowB, double* colA, double* colB) {
+    size_t nn = (size_t)(sN + 2) * (M + 2);
+    for (size_t q = 0; q < nn; ++q) {
+        double sv = V2[q];
+        rowA[q] = (sv / dL) * ((0.5 * (dv + dv) / dL) + 0.5);
+        rowB[q] = (sv / dL) * ((0.5 * (dv + dv) / dL) - 0.5);
+        colA[q] = (sv / (dLx * dLx)) * 0.5 * (dv + dv);
+        colB[q] = (sv / (dLx * dLx)) * 0.5 * (dv + dv);
+    }
+}
+
+void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx) {
+    /* constructor defaults: L = 22, H = 1/sqrt(5), k = 2 sqrt(5)/(pi 2.5),
+     * Q = k (L/2 - H), Lx = 5, dL = 2 Lx/(2 sN + 1), dLx = Q/M */
+    double k = 2.0 * sqrt(5.0) / (M_PI * 2.5);
+    double H = 1 / sqrt(5.0);
+    double Q = k * (0.5 * 22.0 - H);
+    *dL = 2 * 5.0 / (2.0 * sN + 1.0);
+    *dLx = Q / M;
+}
+
+/* ===================================================================== */
+/* psi weights of mt_psi_caputo.cpp (g(t) = t^beta, func_in_kernel = 3)   */
+/* ===================================================================== */
+#define PSI_EPS 1e-5      /* global_eps, mt_psi_caputo.cpp:247 */
+#define PSI_EPS2 1e-5     /* global_eps2 */
+#define PSI_MAXIT 100000  /* integr_max_niter, mt_psi_caputo.cpp:242 */
+#define PSI_KB2 0.5       /* kb2_coef, mt_psi_caputo.cpp:239 */
+
+typedef struct {
+    double beta, tau;
+    int n_t0;       /* rows allocated */
+    double** v3;    /* v3[t0][i], mt_psi_caputo.cpp:567 row2_precalc[1] */
+    int* len;
+    int* cap;
+} psi_cache;
+
+static double psi_g(double beta, double t) { return pow(t, beta); }
+static double psi_inv_g_der(double beta, double t, int n, double d) { /* :268-293 */
+    if (n == 0) return pow(t, 1.0 / beta);
+    if (n == 1) return ((1.0 / beta) * pow(t, (1.0 / beta) - 1)) * (d / pow(t, 1.0 / beta));
+    return d * ((1.0 / beta) - n + 1) / t;
+}
+
+/* midpoint rule on n panels: __kb, mt_psi_caputo.cpp:435-446 */
+static double psi_kb_mid(double beta, int n, double t0, double t1, double gtj, double alpha) {
+    double ret = 0.0, step = (t1 - t0) / n;
+    for (int i = 0; i < n; i++) ret += step * pow(gtj - psi_g(beta, t0 + (i + 0.5) * step), -alpha);
+    return ret;
+}
+/* adaptive 3/8-rule march toward the singular end: ___kb, :447-473 */
+static double psi_kb_sing(double beta, int n, double t0, double t1, double gtj, double alpha) {
+    double f = pow(gtj - psi_g(beta, t0), -alpha), f1, f2, f3, f4;
+    double sum = 0.0, v, t = t0, step = (t1 - t0) / n;
+    do {
+        f1 = pow(gtj - psi_g(beta, t + 0.5 * step), -alpha);
+        f3 = pow(gtj - psi_g(beta, t + step / 3.0), -alpha);
+        f4 = pow(gtj - psi_g(beta, t + 2.0 * step / 3.0), -alpha);
+        f2 = pow(gtj - psi_g(beta, t + step), -alpha);
+        v = (f + 3 * f3 + 3 * f4 + f2) * step / 8.0;
+        if (!isfinite(v)) break;
+        sum += v;
+        t += step;
+        if ((f1 / f) < (f2 / f1)) step /= (f2 / f1) / (f1 / f);
+        while ((t + step) > t1) step /= 2.0;
+        f = f2;
+    } while ((t1 - t) > PSI_EPS * PSI_EPS * PSI_EPS);
+    return sum;
+}
+/* recursive subdivision: _kb, :476-494 */
+static double psi_kb_adapt(double beta, int n, double t0, double t1, double gtj, int sing,
+                           double alpha) {
+    double v1 = psi_kb_mid(beta, n, t0, t1, gtj, alpha);
+    double v2 = psi_kb_mid(beta, n * 2, t0, t1, gtj, alpha);
+    if ((t1 - t0) < PSI_EPS) {
+        if (sing) return psi_kb_sing(beta, n * 16, t0, t1, gtj, alpha);
+        return 0.5 * (v1 + v2);
+    }
+    if (fabs(v2 - v1) > PSI_EPS) {
+        v1 = psi_kb_adapt(beta, n, t0, 0.5 * (t0 + t1), gtj, sing, alpha);
+        v2 = psi_kb_adapt(beta, n, 0.5 * (t0 + t1), t1, gtj, sing, alpha);
+        return v1 + v2;
+    }
+    return v2;
+}
+
+/* Taylor series around g(tj): _kb_row, mt_psi_caputo.cpp:497-566 (branch
+ * g(t1) <= gtj, the only one calc_int reaches). */
+static double psi_kb_row(double beta, double t0, double t1, double gtj, double alpha) {
+    double sum = 0.0, v, v2, v3, v4, i = 0;
+    int niter = 0;
+    double gt1 = psi_g(beta, t1), gt0 = psi_g(beta, t0);
+    if (gt1 <= gtj) {
+        v = psi_inv_g_der(beta, gtj, 1, psi_inv_g_der(beta, gtj, 0, 1));
+        v *= pow(gtj, 1 - alpha);
+        v3 = pow(fabs(1.0 - gt1 / gtj), 1 - alpha);
+        v4 = pow(fabs(1.0 - gt0 / gtj), 1 - alpha);
+        v2 = -(v3 - v4) / (1 - alpha);
+        sum += v2 * v;
+        while (fabs(v * v2) > PSI_EPS) {
+            i += 1.0;
+            v = psi_inv_g_der(beta, gtj, (int)(i + 1.0), v);
+            v *= -gtj / i;
+            v3 *= fabs(1.0 - gt1 / gtj);
+            v4 *= fabs(1.0 - gt0 / gtj);
+            v2 = -(v3 - v4) / (i - alpha + 1.0);
+            sum += v2 * v;
+            if (++niter > PSI_MAXIT) break;
+            if (!isfinite(sum)) break;
+        }
+        return sum;
+    }
+    /* g(t1) > g(tj) never occurs for calc_int's intervals (t1 <= a) */
+    return NAN;
+}
+
+static double* psi_cache_row(psi_cache* c, int t0, int need) {
+    if (t0 >= c->n_t0) {
+        int nn = 2 * (t0 + 1);
+        c->v3 = (double**)realloc(c->v3, nn * sizeof(double*));
+        c->len = (int*)realloc(c->len, nn * sizeof(int));
+        c->cap = (int*)realloc(c->cap, nn * sizeof(int));
+        for (int k = c->n_t0; k < nn; ++k) { c->v3[k] = NULL; c->len[k] = 0; c->cap[k] = 0; }
+        c->n_t0 = nn;
+    }
+    if (need > c->cap[t0]) {
+        c->cap[t0] = 2 * need + 8;
+        c->v3[t0] = (double*)realloc(c->v3[t0], c->cap[t0] * sizeof(double));
+    }
+    return c->v3[t0];
+}
+
+/* Newton binomial in g(tj) times Taylor series of the inverse's derivative
+ * around g(t1): _kb_row2, mt_psi_caputo.cpp:567-719 (mode 0: g(t1) <= gtj),
+ * with the per-interval cache of the inner integrals v3 (row2_precalc). */
+static double psi_kb_row2(psi_cache* c, double beta, double t0, double t1, double gtj, int it0,
+                          double alpha) {
+    double sum = 0.0, v1 = 1.0, v2, v3, v4, v5, v6, v7, i = 0, m;
+    double gt0 = psi_g(beta, t0), gt1 = psi_g(beta, t1);
+    int niter = 0;
+    v2 = pow(gtj, -alpha) * gt1;
+    v6 = gt0 / gt1;
+    do {
+        if (i != 0.0) {
+            v1 *= -(-alpha - i + 1.0) / i;
+            v2 *= gt1 / gtj;
+            v6 *= gt0 / gt1;
+        }
+        int ii = (int)i;
+        double* row = psi_cache_row(c, it0, ii + 1);
+        if (c->len[it0] > ii) {
+            v3 = row[ii];
+        } else {
+            if (i == 0.0)
+                v3 = (t1 - t0) / gt1;
+            else {
+                v3 = 0.0;
+                m = 0.0;
+                v4 = psi_inv_g_der(beta, gt1, 1, psi_inv_g_der(beta, gt1, 0, 0));
+                v5 = (1.0 / (i + 1.0)) * (1.0 - v6);
+                v3 += v4 * v5;
+                v7 = 1;
+                while (fabs(v4 * v5) > (fabs(v3) * PSI_EPS)) {
+                    m += 1.0;
+                    v4 = psi_inv_g_der(beta, gt1, (int)(m + 1.0), v4);
+                    v4 /= m;
+                    v4 /= ((i + m + 1.0) / (gt1 * m));
+                    if (m == 1.0)
+                        v7 = (1.0 / (gt1)) * (v6 * (gt0 - gt1));
+                    else
+                        v7 *= (gt0 - gt1) * ((m - 1.0) / m) * ((i + m) / (gt1 * (m - 1.0)));
+                    v5 = -(v5 + v7);
+                    v3 += v4 * v5;
+                    if (++niter > PSI_MAXIT) break;
+                    if (!isfinite(v3)) break;
+                }
+            }
+            /* the reference pads the cache to i+1 entries (zeros) then stores */
+            for (int q = c->len[it0]; q < ii; ++q) row[q] = 0;
+            row[ii] = v3;
+            c->len[it0] = ii + 1;
+        }
+        sum += v1 * v2 * v3;
+        i += 1.0;
+        if (++niter > PSI_MAXIT) break;
+        if (!isfinite(sum)) break;
+    } while (fabs(v1 * v2 * v3) > PSI_EPS2);
+    return sum;
+}
+
+/* calc_int(alpha, t0, t1, a, tau, time=1), mt_psi_caputo.cpp:720-740.
+ * Pinned: golden tables from the reference; beta = 1 closed form. */
+static double psi_calc_int(psi_cache* c, double alpha, double beta, int t0, int t1, int a,
+                           double tau) {
+    double v;
+    double gtj = psi_g(beta, a * tau);
+    if ((t1 <= PSI_KB2 * a) && !(beta == 1))
+        v = psi_kb_row2(c, beta, t0 * tau, t1 * tau, gtj, t0, alpha);
+    else
+        v = psi_kb_row(beta, t0 * tau, t1 * tau, gtj, alpha);
+    if (!isfinite(v)) v = psi_kb_adapt(beta, 1, t0 * tau, t1 * tau, gtj, t1 == a, alpha);
+    return v;
+}
+
+static void psi_cache_free(psi_cache* c) {
+    for (int k = 0; k < c->n_t0; ++k) free(c->v3[k]);
+    free(c->v3);
+    free(c->len);
+    free(c->cap);
+    memset(c, 0, sizeof(*c));
+}
+
+void orc_psi_weights(double alpha, double beta, double tau, int A, double* out) {
+    psi_cache c;
+    memset(&c, 0, sizeof(c));
+    for (int aa = 0; aa < A; ++aa) out[aa] = psi_calc_int(&c, alpha, beta, aa, aa + 1, A, tau);
+    psi_cache_free(&c);
+}
+
+/* ===================================================================== */
+/* 1D H_solver (mt_minimal.cpp / mt_psi_caputo.cpp)                       */
+/* ===================================================================== */
+struct orc_mt {
+    int N, psi, form, top, bottom_cond, picard;
+    double L, dL, alpha, beta, tau, Da, Dg, D, k, k0, b, top_val, Ev, Hbottom;
+    double *U, *Un, *Al, *Bt, *Om;
+    double** old;
+    int nold, cap;
+    int tstep;
+    double* kbt; /* kb_time of the current step (psi) */
+    int cap_kbt;
+    psi_cache pc;
+};
+
+/* KK(i) and inv_dw_dh(i) evaluated at state X (mt_minimal.cpp:481-562 for
+ * cv_form 2, k_form 0; the CONST and KEXP forms are this build's physics) */
+static double mt_KK(const orc_mt* m, const double* X, int i) {
+    if (m->form == ORC_MT_CV2) return 1.0 * m->k;
+    if (m->form == ORC_MT_KEXP) return m->k0 * exp(m->b * X[i]);
+    return m->D;
+}
+static double mt_inv(const orc_mt* m, const double* X, int i) {
+    if (m->form != ORC_MT_CV2) return 1.0;
+    double P = -X[i], Ch = 0.0;
+    if (P >= 10) Ch = 0.05;
+    if ((P < 10) && (P >= 0.0)) Ch = 0.5 - P * (0.5 - 0.05) / 10.0;
+    if (Ch != 0.0) return 1.0 / Ch;
+    return 1e20;
+}
+
+orc_mt* orc_mt_create(int N, double L, int psi, double beta, double alpha, double tau, int form,
+                      double D, double k, double k0, double b, int top, double top_val,
+                      double Ev, int bottom_cond, double Hbottom, int picard, const double* U0) {
+    orc_mt* m = (orc_mt*)calloc(1, sizeof(orc_mt));
+    m->N = N;
+    m->L = L;
+    m->dL = L / N;
+    m->psi = psi;
+    m->beta = beta;
+    m->alpha = alpha;
+    m->tau = tau;
+    m->form = form;
+    m->D = D;
+    m->k = k;
+    m->k0 = k0;
+    m->b = b;
+    m->top = top;
+    m->top_val = top_val;
+    m->Ev = Ev;
+    m->bottom_cond = bottom_cond;
+    m->Hbottom = Hbottom;
+    m->picard = picard < 1 ? 1 : picard;
+    /* reset_D (mt_minimal.cpp:744-749; mt_psi_caputo.cpp:1176-1181), gamma = 1 */
+    if (psi)
+        m->Da = pow(tau, -1) / orc_gamma(2.0 - alpha);
+    else
+        m->Da = pow(tau, -alpha) / orc_gamma(2.0 - alpha);
+    m->Dg = pow(m->dL, -2) / orc_gamma(2.0 - 1.0);
+    m->U = dup_arr(U0, N + 2);
+    m->Un = dup_arr(U0, N + 2);
+    m->Al = (double*)calloc(N + 2, sizeof(double));
+    m->Bt = (double*)calloc(N + 2, sizeof(double));
+    m->Om = (double*)calloc(N + 2, sizeof(double));
+    m->tstep = 1;
+    return m;
+}
+
+void orc_mt_destroy(orc_mt* m) {
+    if (!m) return;
+    for (int q = 0; q < m->nold; ++q) free(m->old[q]);
+    free(m->old);
+    free(m->U); free(m->Un); free(m->Al); free(m->Bt); free(m->Om); free(m->kbt);
+    psi_cache_free(&m->pc);
+    free(m);
+}
+
+/* Time-history sum of Om1: Da * sum_{t=1}^{os-2} w_t (old[t+1]-old[t]) with
+ * w_t = kappa(os-t) (mt_minimal.cpp:906-939) or (1-alpha) kb_time[t]
+ * (mt_psi_caputo.cpp:1315-1342), summed in increasing t. */
+static double mt_time_sum(int psi, double alpha, const double* kbt, double* const* old, int os,
+                          int id) {
+    double time_sum = 0;
+    if (os >= 2) {
+        double prev = old[1][id];
+        for (int t = 0; t < os - 1; t++) {
+            double kv, cur = old[1 + t][id];
+            if (psi)
+                kv = kbt[t] * (1 - alpha);
+            else
+                kv = (pow((double)(os - t), 1.0 - alpha) - pow((double)(os - t - 1), 1.0 - alpha));
+            time_sum += kv * (cur - prev);
+            prev = cur;
+        }
+    }
+    return time_sum;
+}
+
+void orc_mt_history(int N, int psi, double beta, double alpha, double tau, int os, int tstep,
+                    const double* snaps, double* out) {
+    double** old = (double**)malloc(sizeof(double*) * (os > 0 ? os : 1));
+    for (int q = 0; q < os; ++q) old[q] = (double*)snaps + (size_t)q * (N + 2);
+    double* kbt = NULL;
+    double Da = psi ? pow(tau, -1) / orc_gamma(2.0 - alpha) : pow(tau, -alpha) / orc_gamma(2.0 - alpha);
+    if (psi) {
+        kbt = (double*)malloc((tstep + 1) * sizeof(double));
+        psi_cache c;
+        memset(&c, 0, sizeof(c));
+        for (int aa = 0; aa <= tstep; ++aa) kbt[aa] = psi_calc_int(&c, alpha, beta, aa, aa + 1, tstep + 1, tau);
+        psi_cache_free(&c);
+    }
+    for (int i = 1; i < N; ++i) out[i - 1] = Da * mt_time_sum(psi, alpha, kbt, old, os, i);
+    free(kbt);
+    free(old);
+}
+
+void orc_mt_step(orc_mt* m) {
+    const int N = m->N;
+    double *U = m->U, *Al = m->Al, *Bt = m->Bt, *Om = m->Om;
+    /* psi: kb_time[aa] = calc_int(alpha, aa, aa+1, tstep+1, tau, 1) for aa <= tstep
+     * (mt_psi_caputo.cpp:1473-1490) */
+    if (m->psi) {
+        if (m->cap_kbt < m->tstep + 1) {
+            m->cap_kbt = 2 * (m->tstep + 1);
+            m->kbt = (double*)realloc(m->kbt, m->cap_kbt * sizeof(double));
+        }
+        for (int aa = 0; aa <= m->tstep; ++aa)
+            m->kbt[aa] = psi_calc_int(&m->pc, m->alpha, m->beta, aa, aa + 1, m->tstep + 1, m->tau);
+    }
+    /* current-step time coefficient (R and Om): Da, or Da*kb_time[tstep]*(1-alpha) */
+    double dcur = m->psi ? m->Da * m->kbt[m->tstep] * (1 - m->alpha) : m->Da;
+    memcpy(m->Un, U, (N + 2) * sizeof(double));
+    /* history part of Om does not change between Picard iterates */
+    double* hist = (double*)malloc((N + 1) * sizeof(double));
+    if (m->alpha != 1.0)
+        for (int i = 1; i < N; ++i) hist[i] = mt_time_sum(m->psi, m->alpha, m->kbt, m->old, m->nold, i);
+    for (int it = 0; it < m->picard; ++it) {
+        const double* X = U; /* coefficient state: U^n on the first pass */
+        /* al1 (mt_minimal.cpp:829-858) */
+        int first_order;
+        if (m->top == ORC_TOP_DIRICHLET) first_order = 0;
+        else if (m->top == ORC_TOP_FLUX) first_order = 1;
+        else first_order = (mt_inv(m, X, 0) != 1e20);
+        Al[1] = first_order ? 1 : 0;
+        for (int i = 1; i < N; i++) {
+            double inv = mt_inv(m, X, i);
+            double A_ = m->Dg * mt_KK(m, X, i - 1) * inv;
+            double B = m->Dg * mt_KK(m, X, i + 1) * inv;
+            double R = inv * (mt_KK(m, X, i + 1) + mt_KK(m, X, i - 1)) * m->Dg + dcur;
+            Al[i + 1] = B / (R - A_ * Al[i]);
+        }
+        /* Om1 (mt_minimal.cpp:860-940), gamma = 1: the space term is zero */
+        for (int i = 1; i < N; ++i) {
+            Om[i] = -dcur * m->Un[i];
+            Om[i] += (mt_KK(m, X, i) - mt_KK(m, X, i - 1)) / m->dL;
+            if (m->alpha != 1.0) Om[i] += m->Da * hist[i];
+        }
+        /* bt1 (mt_minimal.cpp:942-980); Uc() = Gamma(1) dL (Ev - 0) / KK(0) */
+        double Uc = orc_gamma(2.0 - 1.0) * m->dL * (m->Ev - 0.0) / mt_KK(m, X, 0);
+        if (m->top == ORC_TOP_DIRICHLET)
+            Bt[1] = m->top_val;
+        else if (first_order)
+            Bt[1] = -Uc;
+        else
+            Bt[1] = -Uc * mt_KK(m, X, 1) / m->dL;
+        for (int i = 1; i < N; i++) {
+            double inv = mt_inv(m, X, i);
+            double A_ = m->Dg * mt_KK(m, X, i - 1) * inv;
+            double B = m->Dg * mt_KK(m, X, i + 1) * inv;
+            Bt[i + 1] = (Al[i + 1] / B) * (A_ * Bt[i] - Om[i]);
+        }
+        /* U1 bottom boundary and back-substitution (mt_minimal.cpp:987-1024) */
+        double Qb = m->bottom_cond == 0 ? -0.0 : 0;
+        double UNm1 = U[N - 1];
+        if (m->bottom_cond == 1)
+            U[N] = m->Hbottom;
+        else if (fabs(Bt[N] + Qb) > 1e-15) {
+            if (fabs(1.0 - Al[N]) > 1e-15) U[N] = (Bt[N] + Qb) / (1.0 - Al[N]);
+            else U[N] = UNm1 + Qb;
+        } else {
+            if (fabs(1.0 - Al[N]) > 1e-15) U[N] = 0.0;
+            else U[N] = UNm1 + Qb;
+        }
+        for (int i = N - 1; i >= 0; i--) U[i] = Al[i + 1] * U[i + 1] + Bt[i + 1];
+        U[N + 1] = U[N];
+    }
+    free(hist);
+    if (m->alpha != 1.0) {
+        if (m->nold == m->cap) {
+            m->cap = m->cap ? 2 * m->cap : 64;
+            m->old = (double**)realloc(m->old, m->cap * sizeof(double*));
+        }
+        m->old[m->nold++] = dup_arr(U, N + 2);
+    }
+    m->tstep++;
+}
+
+void orc_mt_get(const orc_mt* m, double* U) { memcpy(U, m->U, (m->N + 2) * sizeof(double)); }
+double orc_mt_da(const orc_mt* m) { return m->Da; }

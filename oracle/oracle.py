@@ -1,0 +1,197 @@
+"""ctypes wrapper of the CPU oracle (oracle/frm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product package never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_D = ct.c_double
+_I = ct.c_int
+_P = ct.POINTER(ct.c_double)
+_V = ct.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ct.CDLL(_LIB_PATH)
+        L.orc_gamma.restype = _D
+        L.orc_gamma.argtypes = [_D]
+        L.orc_log_gamma.restype = _D
+        L.orc_log_gamma.argtypes = [_D]
+        L.orc_tg_kb.restype = _D
+        L.orc_tg_kb.argtypes = [_I, _D, _D, _D, _D, _I, _I, _D, _D, _D]
+        L.orc_tg_create.restype = _V
+        L.orc_tg_create.argtypes = [_I, _I, _I, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]
+        L.orc_tg_destroy.argtypes = [_V]
+        L.orc_tg_step.argtypes = [_V]
+        L.orc_tg_get.argtypes = [_V, _P]
+        L.orc_tg_coefs_ref.argtypes = [_I, _I, _P, _D, _D, _D, _P, _P, _P, _P]
+        L.orc_tg_ref_spacing.argtypes = [_I, _I, _P, _P]
+        L.orc_mt_create.restype = _V
+        L.orc_mt_create.argtypes = [_I, _D, _I, _D, _D, _D, _I, _D, _D, _D, _D, _I, _D, _D,
+                                    _I, _D, _I, _P]
+        L.orc_mt_destroy.argtypes = [_V]
+        L.orc_mt_step.argtypes = [_V]
+        L.orc_mt_get.argtypes = [_V, _P]
+        L.orc_mt_da.restype = _D
+        L.orc_mt_da.argtypes = [_V]
+        L.orc_mt_history.argtypes = [_I, _I, _D, _D, _D, _I, _I, _P, _P]
+        L.orc_psi_weights.argtypes = [_D, _D, _D, _I, _P]
+        if hasattr(L, "orc_mittag_leffler"):
+            L.orc_mittag_leffler.restype = _D
+            L.orc_mittag_leffler.argtypes = [_D, _D, _D, _D]
+            L.orc_ab_weights.argtypes = [_D, _D, _I, _I, _D, _I, _P]
+            L.orc_ab_solve.restype = _D
+            L.orc_ab_solve.argtypes = [_D, _D, _I, _I, _I, _I, _D, _I, _P]
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P)
+
+
+def gamma(x: float) -> float:
+    return lib().orc_gamma(x)
+
+
+def tg_kb(func, fpow, alpha, tau, n, s, ja, sa, l, int_err=1e-6) -> float:
+    return lib().orc_tg_kb(func, fpow, alpha, tau, int_err, n, s, ja, sa, l)
+
+
+def tg_kb_table(func, fpow, alpha, tau, n, int_err=1e-6) -> np.ndarray:
+    """The eight kb(s, ja, sa, l) variants for s = 0..n, variant bit0: ja=1,
+    bit1: sa=1/2, bit2: l=1 (fr_d_t_g_2d.cpp:3077-3080)."""
+    out = np.empty((8, n + 1))
+    for v in range(8):
+        ja = 1.0 if v & 1 else 0.5
+        sa = 0.5 if v & 2 else 0.0
+        l = 1.0 if v & 4 else 0.5
+        for s in range(n + 1):
+            out[v, s] = tg_kb(func, fpow, alpha, tau, n, s, ja, sa, l, int_err)
+    return out
+
+
+def tg_ref_spacing(sN: int, M: int):
+    a, b = ct.c_double(), ct.c_double()
+    lib().orc_tg_ref_spacing(sN, M, ct.byref(a), ct.byref(b))
+    return a.value, b.value
+
+
+def tg_coefs_ref(sN, M, V2, dv, dL, dLx):
+    nn = (sN + 2) * (M + 2)
+    V2 = np.ascontiguousarray(V2, dtype=np.float64).reshape(nn)
+    outs = [np.empty(nn) for _ in range(4)]
+    lib().orc_tg_coefs_ref(sN, M, _p(V2), dv, dL, dLx, *[_p(o) for o in outs])
+    return outs
+
+
+class TGSolver:
+    """Oracle of TG_solver::calc_step (fr_d_t_g_2d.cpp:2552), dt_alg = 0."""
+
+    def __init__(self, sN, M, alpha, tau, sigma, C1, S, rowA, rowB, colA, colB, C0,
+                 func=0, fpow=1.0, int_err=1e-6):
+        self.sN, self.M = sN, M
+        nn = (sN + 2) * (M + 2)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(nn)
+                for a in (rowA, rowB, colA, colB, C0)]
+        self._h = lib().orc_tg_create(sN, M, func, fpow, alpha, tau, sigma, C1, S, int_err,
+                                      *[_p(a) for a in arrs])
+
+    def step(self, n=1):
+        for _ in range(n):
+            lib().orc_tg_step(self._h)
+
+    def state(self) -> np.ndarray:
+        out = np.empty((self.M + 2) * (self.sN + 2))
+        lib().orc_tg_get(self._h, _p(out))
+        return out.reshape(self.M + 2, self.sN + 2)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_tg_destroy(self._h)
+            self._h = None
+
+
+FORM_CONST, FORM_CV2, FORM_KEXP = 0, 1, 2
+TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
+
+
+class MTSolver:
+    """Oracle of H_solver::calc_step (mt_minimal.cpp:1032 / mt_psi_caputo.cpp:1435)."""
+
+    def __init__(self, N, L, alpha, tau, U0, psi=0, beta=1.0, form=FORM_CONST, D=0.0, k=0.0,
+                 k0=0.0, b=0.0, top=TOP_AUTO, top_val=0.0, Ev=0.0, bottom_cond=1,
+                 Hbottom=0.0, picard=1):
+        self.N = N
+        U0 = np.ascontiguousarray(U0, dtype=np.float64)
+        assert U0.shape == (N + 2,)
+        self._h = lib().orc_mt_create(N, L, psi, beta, alpha, tau, form, D, k, k0, b, top,
+                                      top_val, Ev, bottom_cond, Hbottom, picard, _p(U0))
+
+    def step(self, n=1):
+        for _ in range(n):
+            lib().orc_mt_step(self._h)
+
+    def state(self) -> np.ndarray:
+        out = np.empty(self.N + 2)
+        lib().orc_mt_get(self._h, _p(out))
+        return out
+
+    @property
+    def Da(self) -> float:
+        return lib().orc_mt_da(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_mt_destroy(self._h)
+            self._h = None
+
+
+def mt_history(N, alpha, tau, snaps, tstep, psi=0, beta=1.0) -> np.ndarray:
+    snaps = np.ascontiguousarray(snaps, dtype=np.float64)
+    os_ = snaps.shape[0]
+    out = np.empty(N - 1)
+    lib().orc_mt_history(N, psi, beta, alpha, tau, os_, tstep, _p(snaps.reshape(-1)), _p(out))
+    return out
+
+
+def psi_weights(alpha, beta, tau, A) -> np.ndarray:
+    out = np.empty(A)
+    lib().orc_psi_weights(alpha, beta, tau, A, _p(out))
+    return out
+
+
+def mittag_leffler(a, b, z, eps) -> float:
+    return lib().orc_mittag_leffler(a, b, z, eps)
+
+
+def ab_weights(a, tau, maxl, maxkiter, eps, D) -> np.ndarray:
+    out = np.empty(D)
+    lib().orc_ab_weights(a, tau, maxl, maxkiter, eps, D, _p(out))
+    return out
+
+
+def ab_solve(a, T, n, m, maxl, maxkiter, eps, F_func=0):
+    C = np.empty(m + 1)
+    err = lib().orc_ab_solve(a, T, n, m, maxl, maxkiter, eps, F_func, _p(C))
+    return err, C

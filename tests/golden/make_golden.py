@@ -1,0 +1,158 @@
+"""Write the golden vectors under tests/golden/ from the reference itself.
+
+Runs the harnesses that oracle/ref/Makefile compiles from the reference's own
+sources (fr_d_t_g_2d.cpp, mt_minimal.cpp, mt_psi_caputo.cpp,
+MF_int_series.cpp) into oracle/_ref/, feeds them seeded inputs and stores
+inputs and outputs as .npz fixtures.  Only the fixtures are committed; this
+script needs /root/reference and runs in the build container only:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(REPO, "oracle", "_ref")
+
+
+def run(binary: str, inp: list | np.ndarray) -> np.ndarray:
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        np.asarray(inp, dtype=np.float64).tofile(fi)
+        r = subprocess.run([os.path.join(REF, binary), fi, fo], capture_output=True, text=True,
+                           env={**os.environ, "OMP_NUM_THREADS": "1"})
+        if r.returncode != 0:
+            raise RuntimeError(f"{binary} failed ({r.returncode}): {r.stderr[-500:]}")
+        return np.fromfile(fo)
+
+
+def save(name: str, meta: dict, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, meta=np.array(json.dumps(meta)), **arrays)
+    print(f"wrote {name}.npz", {k: v.shape for k, v in arrays.items()})
+
+
+# ---------------------------------------------------------------- 2D LOD
+def tg_case(name, seed, func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, fpow=1.0):
+    rng = np.random.default_rng(seed)
+    nn = (sN + 2) * (M + 2)
+    V2 = rng.uniform(0.5, 1.5, nn)
+    C0 = rng.uniform(0.0, 1.0, nn)
+    hdr = [func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, 1, fpow]
+    out = run("ref_tg", np.concatenate([np.array(hdr, float), V2, C0]))
+    nsnap = len([k for k in range(1, nsteps + 1) if k % every == 0 or k == nsteps])
+    snaps = out[: nsnap * nn].reshape(nsnap, M + 2, sN + 2)
+    kb = out[nsnap * nn:].reshape(8, nsteps + 1)
+    meta = dict(source="fr_d_t_g_2d.cpp TG_solver_<func,1,0>::calc_step, dt_alg=0, int_alg=1",
+                func=func, sN=sN, M=M, alpha=alpha, tau=tau, sigma=sigma, dv=dv, C1=C1, S=S,
+                nsteps=nsteps, every=every, fpow=fpow, int_err=1e-6, seed=seed)
+    save(name, meta, V2=V2.reshape(M + 2, sN + 2), C0=C0.reshape(M + 2, sN + 2), snaps=snaps,
+         kb=kb)
+
+
+# ---------------------------------------------------------------- 1D H_solver
+def mt_traj(name, binary, seed, BS, NB, alpha, tau, L, k, Ev, Hb, bc, nsteps, every, U0fn,
+            beta=None):
+    rng = np.random.default_rng(seed)
+    N = BS * NB
+    x = np.arange(N + 2) / N
+    U0 = U0fn(x, rng)
+    U0[N + 1] = U0[N]
+    hdr = [0, BS, NB, alpha, tau, L, k, Ev, Hb, bc, nsteps, every]
+    if beta is not None:
+        hdr.append(beta)
+    out = run(binary, np.concatenate([np.array(hdr, float), U0]))
+    nsnap = len([t for t in range(1, nsteps + 1) if t % every == 0 or t == nsteps])
+    snaps = out[: nsnap * (N + 2)].reshape(nsnap, N + 2)
+    arrays = dict(U0=U0, snaps=snaps)
+    if beta is not None:
+        arrays["kb_time"] = out[nsnap * (N + 2):]
+    meta = dict(source=f"{binary}: H_solver::calc_step, cv_form=2, k_form=0, gamma=1",
+                BS=BS, NB=NB, N=N, alpha=alpha, tau=tau, L=L, k=k, Ev=Ev, Hbottom=Hb,
+                bottom_cond=bc, nsteps=nsteps, every=every, beta=beta, seed=seed)
+    save(name, meta, **arrays)
+
+
+def mt_hist(name, binary, seed, BS, NB, alpha, tau, os_, tstep, beta=None):
+    rng = np.random.default_rng(seed)
+    N = BS * NB
+    snaps = np.cumsum(rng.normal(0, 1, (os_, N + 2)), axis=0)
+    hdr = [1, BS, NB, alpha, tau, os_, tstep]
+    if beta is not None:
+        hdr.append(beta)
+    out = run(binary, np.concatenate([np.array(hdr, float), snaps.reshape(-1)]))
+    meta = dict(source=f"{binary}: H_solver::Om1 time-history sum with U=0", BS=BS, NB=NB,
+                N=N, alpha=alpha, tau=tau, os=os_, tstep=tstep, beta=beta, seed=seed)
+    save(name, meta, snaps=snaps, Da=out[:1], Om=out[1:])
+
+
+def psi_weights(name, cases, As):
+    tabs = {}
+    for q, (alpha, beta, tau) in enumerate(cases):
+        out = run("ref_psi", [2, alpha, beta, tau, len(As)] + list(As))
+        tabs[f"w{q}"] = out
+    meta = dict(source="mt_psi_caputo.cpp calc_int(alpha,a,a+1,A,tau,1), g=t^beta",
+                cases=cases, As=list(As))
+    save(name, meta, **tabs)
+
+
+# ---------------------------------------------------------------- AB
+def mf_all(name):
+    arrays, meta = {}, dict(source="MF_int_series.cpp")
+    wcases = [(0.5, 0.01, 50, 1000, 1e-10, 120), (0.8, 0.01, 50, 1000, 1e-10, 60),
+              (0.3, 0.05, 50, 1000, 1e-10, 40)]
+    for q, c in enumerate(wcases):
+        arrays[f"w{q}"] = run("ref_mf", [0, *c])
+    meta["weights"] = wcases
+    rng = np.random.default_rng(77)
+    hcases = [(0.6, 0.01, 50, 1000, 1e-10, 50)]
+    for q, c in enumerate(hcases):
+        fvs = rng.normal(0, 1, c[-1])
+        arrays[f"fvs{q}"] = fvs
+        arrays[f"h{q}"] = run("ref_mf", [1, *c, *fvs])
+    meta["hist"] = hcases
+    scases = [(0.5, 1.0, 60, 20, 1, 50, 1000, 1e-10, 0), (0.7, 1.0, 40, 16, 1, 50, 1000, 1e-10, 1),
+              (0.3, 0.5, 40, 12, 1, 50, 1000, 1e-10, 2)]
+    arrays["solve_err"] = np.array([run("ref_mf", [2, *c])[0] for c in scases])
+    meta["solve"] = scases
+    ml = []
+    for a in (0.3, 0.5, 0.8, 1.0, 2.0):
+        for b in (1.0, 0.5, 3.0):
+            for z in (-5.0, -1.0, -0.3, 0.0, 0.2, 0.9):
+                ml.append((a, b, z, 1e-12))
+    arrays["ml_in"] = np.array(ml)
+    arrays["ml_out"] = run("ref_mf", [3, len(ml)] + [v for e in ml for v in e])
+    save(name, meta, **arrays)
+
+
+def main():
+    if not os.path.isdir("/root/reference"):
+        sys.exit("the reference sources are needed to write golden vectors")
+    subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+    tg_case("tg_a", 11, 0, 16, 12, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 40, 10)
+    tg_case("tg_b", 12, 0, 9, 20, 0.4, 1e-3, 1.0, 0.2, 0.0, 0.0, 30, 15)
+    tg_case("tg_psi", 13, 3, 10, 8, 0.6, 1e-2, 0.5, 0.1, 0.5, 0.0, 12, 6, fpow=0.7)
+    mt_traj("mt_flux", "ref_mt", 21, 10, 4, 0.8, 0.01, 1.0, 1e-3, 0.0, -20.0, 1, 60, 20,
+            lambda x, r: -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * r.normal(size=x.size))
+    mt_traj("mt_dir_cv2", "ref_mt", 22, 10, 3, 0.6, 0.02, 1.0, 2e-3, -0.5, 0.0, 0, 50, 25,
+            lambda x, r: -12.0 + 10.0 * x + 0.2 * r.normal(size=x.size))
+    mt_hist("mt_hist", "ref_mt", 23, 10, 3, 0.6, 0.01, 25, 26)
+    mt_traj("psi_flux", "ref_psi", 24, 10, 2, 0.75, 0.01, 1.0, 1e-3, 0.0, -20.0, 1, 30, 15,
+            lambda x, r: -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * r.normal(size=x.size),
+            beta=0.8)
+    mt_hist("psi_hist", "ref_psi", 25, 10, 2, 0.65, 0.01, 18, 19, beta=1.3)
+    psi_weights("psi_w", [(0.6, 0.8, 1e-3), (0.9, 1.4, 1e-3), (0.55, 1.0, 1e-2)],
+                [1, 2, 5, 17, 64, 200])
+    mf_all("mf")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,130 @@
+"""The CPU oracle against the reference's own golden vectors and closed forms.
+
+The golden vectors (tests/golden/*.npz) were written by the reference's CPU
+classes, compiled from its sources (tests/golden/make_golden.py).  The oracle
+must reproduce them to round-off; the closed-form checks pin what a plausible
+restatement mistake (a dropped term, a shifted index) would break.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def _snap_steps(m):
+    return [k for k in range(1, m["nsteps"] + 1) if k % m["every"] == 0 or k == m["nsteps"]]
+
+
+@pytest.mark.parametrize("name", ["tg_a", "tg_b", "tg_psi"])
+def test_tg_matches_reference(golden, name):
+    m, z = golden(name)
+    dL, dLx = O.tg_ref_spacing(m["sN"], m["M"])
+    coefs = O.tg_coefs_ref(m["sN"], m["M"], z["V2"], m["dv"], dL, dLx)
+    s = O.TGSolver(m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], m["C1"], m["S"], *coefs,
+                   z["C0"], func=m["func"], fpow=m["fpow"], int_err=m["int_err"])
+    t = 0
+    for q, k in enumerate(_snap_steps(m)):
+        s.step(k - t)
+        t = k
+        ref = z["snaps"][q]
+        assert np.max(np.abs(s.state() - ref)) <= 1e-13 * np.max(np.abs(ref))
+    kb = O.tg_kb_table(m["func"], m["fpow"], m["alpha"], m["tau"], m["nsteps"], m["int_err"])
+    np.testing.assert_allclose(kb, z["kb"], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("name,psi", [("mt_flux", 0), ("mt_dir_cv2", 0), ("psi_flux", 1)])
+def test_mt_matches_reference(golden, name, psi):
+    m, z = golden(name)
+    s = O.MTSolver(m["N"], m["L"], m["alpha"], m["tau"], z["U0"], psi=psi,
+                   beta=m["beta"] or 1.0, form=O.FORM_CV2, k=m["k"], Ev=m["Ev"],
+                   bottom_cond=m["bottom_cond"], Hbottom=m["Hbottom"])
+    t = 0
+    for q, k in enumerate(_snap_steps(m)):
+        s.step(k - t)
+        t = k
+        ref = z["snaps"][q]
+        assert np.max(np.abs(s.state() - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name,psi", [("mt_hist", 0), ("psi_hist", 1)])
+def test_mt_history_matches_reference(golden, name, psi):
+    m, z = golden(name)
+    out = O.mt_history(m["N"], m["alpha"], m["tau"], z["snaps"], m["tstep"], psi=psi,
+                       beta=m["beta"] or 1.0)
+    np.testing.assert_allclose(out, z["Om"], rtol=1e-13, atol=1e-13 * np.max(np.abs(z["Om"])))
+
+
+def test_psi_weights_match_reference(golden):
+    m, z = golden("psi_w")
+    for q, (a, b, tau) in enumerate(m["cases"]):
+        w, off = z[f"w{q}"], 0
+        for A in m["As"]:
+            np.testing.assert_allclose(O.psi_weights(a, b, tau, A), w[off:off + A], rtol=1e-14)
+            off += A
+
+
+def test_gamma_against_math():
+    # Cody's rational form is good to a few ulp below 12; above 12 the A&S
+    # series is exponentiated, which costs ~1e-14 relative by x = 40.
+    for x in [0.05, 0.3, 0.5, 0.999, 1.0, 1.2, 1.5, 2.0, 3.7, 11.9]:
+        assert O.gamma(x) == pytest.approx(math.gamma(x), rel=2e-15)
+    for x in [12.5, 40.0]:
+        assert O.gamma(x) == pytest.approx(math.gamma(x), rel=5e-14)
+
+
+def test_tg_kb_closed_form_caputo():
+    # g = t: kb(s,ja,sa,l) = Da tau^(1-a)/(1-a) [(n+ja-s-sa)^(1-a) - (n+ja-s-sa-l)^(1-a)]
+    alpha, tau, n = 0.7, 1e-3, 37
+    Da = 1.0 / math.gamma(1 - alpha)
+    for s in [0, 5, 36, 37]:
+        for ja, sa, l in [(0.5, 0, 1), (1, 0, 1), (0.5, 0, 0.5), (1, 0.5, 0.5), (1, 0, 0.5)]:
+            d = n + ja - s - sa
+            want = 0.0 if d - l < 0 else Da * tau ** (1 - alpha) / (1 - alpha) * (
+                d ** (1 - alpha) - (d - l) ** (1 - alpha))
+            got = O.tg_kb(0, 1.0, alpha, tau, n, s, ja, sa, l)
+            assert got == pytest.approx(want, rel=1e-11, abs=1e-300)
+
+
+def test_psi_weights_identity_kernel_closed_form():
+    # beta = 1: calc_int(a, t, t+1, A) = tau^(1-a)/(1-a) [(A-t)^(1-a) - (A-t-1)^(1-a)]
+    alpha, tau, A = 0.6, 1e-2, 40
+    w = O.psi_weights(alpha, 1.0, tau, A)
+    t = np.arange(A)
+    want = tau ** (1 - alpha) / (1 - alpha) * ((A - t) ** (1 - alpha) - (A - t - 1.0) ** (1 - alpha))
+    np.testing.assert_allclose(w, want, rtol=1e-11)
+
+
+def test_psi_weights_quadrature_beta():
+    # beta != 1 against a fine midpoint-free Gauss-Legendre quadrature after
+    # the substitution that removes the endpoint singularity; the reference's
+    # series stops at absolute term size 1e-5, so agreement is to ~1e-4 rel.
+    alpha, beta, tau, A = 0.6, 1.4, 1e-2, 30
+    w = O.psi_weights(alpha, beta, tau, A)
+    G = (A * tau) ** beta
+    xg, wg = np.polynomial.legendre.leggauss(200)
+    for t in [0, 3, 10, 20, 29]:
+        a0, a1 = t * tau, (t + 1) * tau
+        # u = (G - s^beta)^(1-alpha) is smooth; integrate in s on [a0, a1]
+        if t < A - 1:
+            s = 0.5 * (a1 - a0) * xg + 0.5 * (a1 + a0)
+            q = 0.5 * (a1 - a0) * np.sum(wg * (G - s ** beta) ** (-alpha))
+        else:
+            # last interval: substitute s = a1 - r^(1/(1-alpha))
+            p = 1.0 / (1 - alpha)
+            rmax = (a1 - a0) ** (1 - alpha)
+            r = 0.5 * rmax * (xg + 1)
+            s = a1 - r ** p
+            q = 0.5 * rmax * np.sum(wg * p * r ** (p - 1) * (G - s ** beta) ** (-alpha))
+        assert w[t] == pytest.approx(q, rel=2e-3)
+
+
+def test_mt_kappa_weights_telescope():
+    # with every increment equal to 1 the L1 history sum telescopes:
+    # sum_{t=1}^{os-2} kappa(os-t) = (os-1)^(1-a) - 1
+    N, alpha, tau, os_ = 20, 0.7, 0.01, 15
+    snaps = np.arange(os_, dtype=float)[:, None] * np.ones((1, N + 2))
+    out = O.mt_history(N, alpha, tau, snaps, os_ + 1)
+    Da = tau ** (-alpha) / math.gamma(2 - alpha)
+    np.testing.assert_allclose(out, Da * ((os_ - 1) ** (1 - alpha) - 1.0), rtol=1e-13)
