@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: node-steps/s of the 2D Caputo LOD solver, plain FP64, one MI355X.
+
+Workload: 512 x 512 interior unknowns (sN = M = 513), alpha = 0.7,
+tau = 1e-3, full FP64 history, convection-diffusion with a seeded
+divergence-free velocity (max|v| = 0.1), D = 1e-3, 8 Gaussian blobs.  A step
+is one LOD time step (memory term + row sweep + column sweep + history push)
+starting from t = 0, so --steps 5000 is the whole configured run and the
+per-step history depth grows from 0 to 4999.
+
+    python bench.py [--gpus 1] [--steps 5000] [--warmup 20]
+
+Prints one JSON line.  `roofline` describes the memory-term kernel (the
+dominant one): algorithmic bytes 8*depth*nodes (history) + 16*nodes (both
+targets) per launch over its HIP-event-timed duration on the solver stream.
+`cpu_baseline` times the oracle (a port of the reference's CPU path) on a
+bounded window of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "node-steps/s at ≤1e-6 rel err vs FP64: chosen form vs plain FP64, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="oracle steps timed for the CPU baseline (0 skips it)")
+    ap.add_argument("--parity-steps", type=int, default=12)
+    return ap.parse_args()
+
+
+def cpu_baseline(case, cfg, depth, nsteps):
+    from oracle import oracle as O
+
+    sN, M, coefs, C0 = case
+    o = O.TGSolver(sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], 0.0, 0.0, *coefs, C0)
+    o.prefill(depth, seed=12345)
+    o.step(1)  # first step at this depth also touches every snapshot once
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    dt = time.perf_counter() - t0
+    nodes = (sN - 1) * (M - 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    del o
+    return {
+        "value": nodes * nsteps / dt,
+        "unit": "node-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{nsteps} LOD steps of the {sN - 1}x{M - 1} workload at history depth "
+                   f"{depth + 1}..{depth + nsteps} (synthetic prefilled history; per-step cost is "
+                   f"linear in depth, so depth T/2 is the run average); oracle/frm_oracle.c, "
+                   f"OpenMP over the nodes of each line as fr_d_t_g_2d.cpp:3341"),
+        "seconds": dt,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the history sums shard by node with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from theses_code_amd import _native as N
+    from theses_code_amd.solvers import TGSolver
+    from theses_code_amd.workloads import C1_WORKLOAD, lod2d_case
+
+    cfg = C1_WORKLOAD
+    case = lod2d_case(cfg["n_interior"], cfg["seed"], vmax=cfg["vmax"], D=cfg["D"])
+    sN, M, coefs, C0 = case
+    nodes = (sN - 1) * (M - 1)
+    K, W = args.steps, args.warmup
+    ctx = N.Context(0)
+    cap = max(K, W, args.parity_steps, 1)
+    s = TGSolver(ctx, sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], coefs, C0, capacity=cap,
+                 C1=0.0, S=0.0)
+
+    # parity against the oracle on the first steps of the same problem
+    parity = None
+    if args.parity_steps > 0:
+        from oracle import oracle as O
+        o = O.TGSolver(sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], 0.0, 0.0, *coefs, C0)
+        o.step(args.parity_steps)
+        s.calc_step(args.parity_steps)
+        ref = o.state()
+        got = s.b_C
+        d = got - ref
+        parity = {"vs": "oracle (CPU port of fr_d_t_g_2d.cpp, pinned to reference goldens)",
+                  "steps": args.parity_steps,
+                  "max_rel_err": float(np.max(np.abs(d)) / np.max(np.abs(ref))),
+                  "rms_rel_err": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref)))}
+        del o
+
+    # warmup, then reset to t = 0
+    s.set_state(C0)
+    if W:
+        s.calc_step(W)
+        ctx.sync()
+    s.set_state(C0)
+
+    ctx.timing_reset()
+    ctx.timing_enable(1 << N.K_MEMTERM)
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.calc_step(K)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.timing_enable(0)
+    launches, ms, nbytes = ctx.timing(N.K_MEMTERM)
+    final = s.b_C
+    dt = t1 - t0
+    value = nodes * K / dt
+
+    achieved = (nbytes / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "memterm_lod2d_512_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": "k_memterm_f64<2>", "launches": launches,
+                "avg_launch_ms": ms / launches if launches else None,
+                "bytes_per_launch_avg": nbytes / launches if launches else None}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        cpu = cpu_baseline(case, cfg, depth=max(K // 2, 1), nsteps=args.cpu_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": dt * 1e3 / max(K, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "form": "plain FP64 GPU path (drop-in)",
+                   "grid_interior": f"{sN - 1}x{M - 1}", "alpha": cfg["alpha"],
+                   "tau": cfg["tau"], "history": "full, FP64", "D": cfg["D"],
+                   "vmax": cfg["vmax"], "nodes": nodes, "seed": cfg["seed"],
+                   "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "final_field_checksum": float(np.sum(final)),
+    }
+    print(json.dumps(line))
+    s.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

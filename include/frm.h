@@ -1,0 +1,157 @@
+/* frm.h -- C ABI of the MI355X-native fractional-memory hot path.
+ *
+ * The reference (sevaboh/theses_code) keeps this path inside C++ solver
+ * classes: TG_solver::calc_step (fr_d_t_g_2d.cpp:2552), H_solver::calc_step
+ * (mt_minimal.cpp:1032, mt_psi_caputo.cpp:1435) and the step loop of
+ * solve_and_check (MF_int_series.cpp:909-954), with an OpenCL/CUDA runtime
+ * wrapper (opencl_cuda_base.cpp) underneath.  This header is the boundary
+ * those classes call instead: plain C, plain pointers and sizes, int status
+ * (0 = FRM_OK, negative = error), no exceptions across it.
+ *
+ * Threading: one frm_ctx owns one HIP stream; calls on one context are not
+ * thread-safe, calls on different contexts are.  Device buffers passed in are
+ * owned by the caller; objects created here (history stores, solvers) own
+ * their device memory until destroyed.
+ */
+#ifndef FRM_H
+#define FRM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    FRM_OK = 0,
+    FRM_E_INVALID = -1, /* bad argument or shape */
+    FRM_E_HIP = -2,     /* HIP runtime error (message in frm_last_error) */
+    FRM_E_NOMEM = -3,   /* device allocation failed */
+    FRM_E_RANGE = -4,   /* history capacity or size limit exceeded */
+    FRM_E_STATE = -5    /* call not valid in the object's current state */
+};
+
+/* memory-term precision forms */
+enum {
+    FRM_PREC_F64 = 0,      /* plain FP64 history and FP64 accumulation */
+    FRM_PREC_F32C = 1,     /* FP32 history, compensated (Kahan) FP32 sums */
+    FRM_PREC_MFMA_BF16 = 2 /* blocked Toeplitz on MFMA, split hi+lo BF16, FP32 acc */
+};
+
+/* kernel-class ids for frm_timing_get */
+enum {
+    FRM_K_MEMTERM = 0, /* history sum (the dominant kernel) */
+    FRM_K_ROWS = 1,    /* row half-step sweep */
+    FRM_K_COLS = 2,    /* column half-step sweep (+ history push) */
+    FRM_K_SWEEP1D = 3, /* batched 1D sweep */
+    FRM_K_OTHER = 4,
+    FRM_K_COUNT = 5
+};
+
+typedef struct frm_ctx frm_ctx;
+typedef struct frm_hist frm_hist;
+typedef struct frm_lod2d frm_lod2d;
+typedef struct frm_mt1d frm_mt1d;
+
+/* ------------------------------------------------- runtime (G1 row) ----
+ * Replaces OpenCL_program / OpenCL_commandqueue / OpenCL_buffer of
+ * opencl_cuda_base.cpp:23-455: device selection, one in-order stream,
+ * buffers and transfers.  Unlike the reference (a Finish() after every
+ * launch, fr_d_t_g_2d.cpp:4046-4048) work stays asynchronous until
+ * frm_sync or a host read-back. */
+int frm_ctx_create(int device, frm_ctx** out);
+int frm_ctx_destroy(frm_ctx* ctx);
+int frm_sync(frm_ctx* ctx);
+const char* frm_strerror(int status);
+const char* frm_last_error(const frm_ctx* ctx);
+int frm_malloc(frm_ctx* ctx, size_t bytes, void** dptr);
+int frm_free(frm_ctx* ctx, void* dptr);
+int frm_upload(frm_ctx* ctx, void* dptr, const void* hptr, size_t bytes);
+int frm_download(frm_ctx* ctx, void* hptr, const void* dptr, size_t bytes);
+/* Per-kernel-class timing with HIP events recorded on the context's stream
+ * around every launch of the classes in `mask` (bit FRM_K_*); 0 disables. */
+int frm_timing_enable(frm_ctx* ctx, uint32_t mask);
+int frm_timing_reset(frm_ctx* ctx);
+/* Synchronises, then returns launches, total milliseconds and the algorithmic
+ * bytes those launches were charged with. */
+int frm_timing_get(frm_ctx* ctx, int kclass, int64_t* launches, double* total_ms,
+                   double* bytes);
+
+/* ------------------------------------------ history store (H1-H3, H5) ----
+ * Device-resident list of per-node increments Delta_s (s = 0..depth-1) of
+ * n_nodes values.  Replaces the per-step heap snapshots oldH.push_back
+ * (mt_minimal.cpp:1025-1030), oldC.push_back (fr_d_t_g_2d.cpp:2617-2622,
+ * 2682-2691) and the save1/save2 kernels (fr_d_t_g_2d.cpp:1947-1962):
+ * the sums only ever read differences of consecutive snapshots, so the
+ * store keeps the differences, one read per term. */
+int frm_hist_create(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
+                    frm_hist** out);
+int frm_hist_destroy(frm_hist* h);
+/* Appends Delta = d_new - d_old (device pointers, n_nodes values each). */
+int frm_hist_push(frm_ctx* ctx, frm_hist* h, const double* d_new, const double* d_old);
+int frm_hist_depth(const frm_hist* h, int32_t* depth);
+int frm_hist_clear(frm_hist* h);
+
+/* Memory term over the store (H1/H2/H3/H5 history loops):
+ *   out[t*n_nodes + i] (+)= scale * sum_{s=0}^{depth-1} w[t*ldw + (depth-1-s)] * Delta_s[i]
+ * for targets t < ntargets (<= 2), i.e. w is indexed by lag-1 with w[0] the
+ * newest increment's weight.  d_w and d_out are device pointers (FP64). */
+typedef struct frm_memterm_desc {
+    int32_t ntargets;   /* 1 or 2 (the two LOD half-step targets share one pass) */
+    int32_t ldw;        /* leading dimension of d_w (>= depth) */
+    const double* d_w;  /* lag-indexed FP64 weights */
+    double scale;
+    int32_t accumulate; /* 0: out = ..., 1: out += ... */
+} frm_memterm_desc;
+int frm_memterm(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double* d_out);
+
+/* ------------------------------------------------ batched tridiagonal ----
+ * a[k] x[k-1] + b[k] x[k] + c[k] x[k+1] = d[k], k = 0..n-1 (a[0], c[n-1]
+ * ignored), nsys independent systems; element k of system s at
+ * s*stride_sys + k*stride_elem.  Replaces the al1/bt1/back-substitution
+ * triple (mt_minimal.cpp:829-858, 942-1024) and the OpenCL Al/Bt/U kernels
+ * (fr_d_t_g_2d.cpp:1604-1658, 1896-1922).  Thomas (progonka) recurrences,
+ * one wavefront per system, chunk seeds from wavefront scans. */
+typedef struct frm_tridiag_desc {
+    int32_t nsys, n;
+    int64_t stride_sys, stride_elem;
+} frm_tridiag_desc;
+int frm_tridiag(frm_ctx* ctx, const frm_tridiag_desc* desc, const double* a, const double* b,
+                const double* c, const double* d, double* x);
+
+/* --------------------------------------- 2D LOD solver (TG_solver) ------
+ * fr_d_t_g_2d.cpp TG_solver_<func=0 (g(t)=t), fixed_d, kform=0> with
+ * dt_alg = 0 (full history).  Grid nodes i = 0..sN+1, j = 0..M+1 at
+ * idx(i,j) = i + j*(sN+2) (fr_d_t_g_2d.cpp:2003).  Row systems j = 1..M-1
+ * (Dirichlet C1 at i = 0, zero gradient at i = sN), column systems
+ * i = 1..sN-1 (zero gradient at both ends).  The sweep coefficients are the
+ * lower (A_) and upper (B) off-diagonals per node and direction
+ * (fr_d_t_g_2d.cpp:3242-3265); the diagonal is (sigma/tau)*kb + A_ + B. */
+typedef struct frm_lod2d_desc {
+    int32_t sN, M;
+    int32_t capacity;  /* max number of steps (history depth) */
+    int32_t precision; /* FRM_PREC_* for the memory term */
+    double alpha, tau, sigma, C1, S;
+    const double* rowA; /* host arrays of (sN+2)*(M+2) values */
+    const double* rowB;
+    const double* colA;
+    const double* colB;
+} frm_lod2d_desc;
+int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
+int frm_lod2d_destroy(frm_lod2d* s);
+/* Initial field (host, (sN+2)*(M+2)); resets the step counter and history,
+ * as TG_solver::initial() pushes C_0 (fr_d_t_g_2d.cpp:2294-2297). */
+int frm_lod2d_set_state(frm_lod2d* s, const double* h_C);
+/* nsteps calls of calc_step(), queued on the context's stream. */
+int frm_lod2d_step(frm_lod2d* s, int32_t nsteps);
+int frm_lod2d_get_state(frm_lod2d* s, double* h_C);
+int frm_lod2d_tstep(const frm_lod2d* s, int32_t* tstep);
+/* Device pointer of the state (reference layout), for callers that keep
+ * the field on the GPU. */
+int frm_lod2d_state_ptr(frm_lod2d* s, double** d_C);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FRM_H */

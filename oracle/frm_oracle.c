@@ -236,6 +236,7 @@ static void tg_line(orc_tg* t, int dir, int line, int n) {
     }
     for (int i = 1; i < N; ++i) Om[i] = -sg * kcur * U[NODE(i)];
     for (int i = 1; i < N; ++i) Om[i] += t->S * 0.5;
+#pragma omp parallel for if (t->nold > 64)
     for (int i = 1; i < N; ++i) {
         size_t id = NODE(i);
         double ts = 0;
@@ -277,6 +278,22 @@ void orc_tg_step(orc_tg* t) {
     for (int i = 1; i < t->sN; ++i) tg_line(t, 1, i, n); /* columns, target n+1 */
     tg_push(t);                                          /* C_{n+1} */
     t->tstep++;
+}
+
+/* CPU-baseline support: append 2*depth synthetic snapshots (a seeded random
+ * walk around the current field, as if depth steps had run) and set tstep, so
+ * that a few steps can be timed at history depth `depth`. */
+void orc_tg_prefill(orc_tg* t, int depth, uint64_t seed) {
+    size_t nn = (size_t)t->ld * (t->M + 2);
+    uint64_t x = seed ? seed : 0x9E3779B97F4A7C15ull;
+    for (int k = 0; k < 2 * depth; ++k) {
+        for (size_t q = 0; q < nn; ++q) {
+            x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+            t->C[q] += 1e-4 * ((double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+        }
+        tg_push(t);
+    }
+    t->tstep += depth;
 }
 
 void orc_tg_get(const orc_tg* t, double* C) {

@@ -42,6 +42,9 @@ void orc_tg_destroy(orc_tg* t);
 /* One calc_step() (fr_d_t_g_2d.cpp:2552-2718), dt_alg = 0. */
 void orc_tg_step(orc_tg* t);
 void orc_tg_get(const orc_tg* t, double* C);
+/* Append 2*depth seeded synthetic snapshots and advance tstep by depth
+ * (used to time steps at a given history depth). */
+void orc_tg_prefill(orc_tg* t, int depth, uint64_t seed);
 int orc_tg_tstep(const orc_tg* t);
 /* Reference coefficient form (fr_d_t_g_2d.cpp:3242-3265, fixed_d = 1):
  * rows A_=(V2/dL)(dv/dL+1/2), B=(V2/dL)(dv/dL-1/2); columns V2*dv/dLx^2. */

@@ -43,6 +43,7 @@ def lib():
         L.orc_tg_destroy.argtypes = [_V]
         L.orc_tg_step.argtypes = [_V]
         L.orc_tg_get.argtypes = [_V, _P]
+        L.orc_tg_prefill.argtypes = [_V, _I, ct.c_uint64]
         L.orc_tg_coefs_ref.argtypes = [_I, _I, _P, _D, _D, _D, _P, _P, _P, _P]
         L.orc_tg_ref_spacing.argtypes = [_I, _I, _P, _P]
         L.orc_mt_create.restype = _V
@@ -120,6 +121,9 @@ class TGSolver:
     def step(self, n=1):
         for _ in range(n):
             lib().orc_tg_step(self._h)
+
+    def prefill(self, depth: int, seed: int = 1):
+        lib().orc_tg_prefill(self._h, depth, seed)
 
     def state(self) -> np.ndarray:
         out = np.empty((self.M + 2) * (self.sN + 2))

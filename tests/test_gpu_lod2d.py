@@ -1,0 +1,102 @@
+"""2D LOD solver (TG_solver::calc_step) on the GPU against the reference's
+golden vectors and the oracle; full-size invariants.
+
+Tolerance: the plain FP64 path must agree with the reference's CPU
+implementation to 1e-12 norm-wise relative error, max|u - u_ref| / max|u_ref|.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from theses_code_amd import _native as N
+from theses_code_amd import solvers as S
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def relerr(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("name", ["tg_a", "tg_b"])
+def test_lod2d_matches_reference_golden(ctx, golden, name):
+    m, z = golden(name)
+    dL, dLx = S.tg_ref_spacing(m["sN"], m["M"])
+    coefs = S.coefs_ref(z["V2"], m["dv"], dL, dLx)
+    s = S.TGSolver(ctx, m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], coefs, z["C0"],
+                   capacity=m["nsteps"], C1=m["C1"], S=m["S"])
+    ks = [k for k in range(1, m["nsteps"] + 1) if k % m["every"] == 0 or k == m["nsteps"]]
+    t = 0
+    for q, k in enumerate(ks):
+        s.calc_step(k - t)
+        t = k
+        assert relerr(s.b_C, z["snaps"][q]) <= TOL, (name, k)
+    assert s.tstep == m["nsteps"]
+    s.close()
+
+
+def _adv_case(rng, sN, M):
+    nn = (sN + 2) * (M + 2)
+    vx = rng.uniform(-0.1, 0.1, nn)
+    vy = rng.uniform(-0.1, 0.1, nn)
+    h = 1.0 / (sN + 1)
+    coefs = S.coefs_advection(vx, vy, 1e-3, h)
+    C0 = rng.uniform(0, 1, nn)
+    return coefs, C0
+
+
+@pytest.mark.parametrize("sN,M,alpha,steps", [(33, 20, 0.7, 150), (100, 70, 0.45, 260),
+                                              (65, 129, 0.9, 120)])
+def test_lod2d_matches_oracle_advection(ctx, sN, M, alpha, steps):
+    rng = np.random.default_rng(sN * M)
+    coefs, C0 = _adv_case(rng, sN, M)
+    tau, sigma, C1, Ssrc = 1e-3, 1.0, 0.0, 0.05
+    o = O.TGSolver(sN, M, alpha, tau, sigma, C1, Ssrc, *coefs, C0)
+    g = S.TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=steps, C1=C1, S=Ssrc)
+    for chunk in (1, steps // 2 - 1, steps - steps // 2):
+        o.step(chunk)
+        g.calc_step(chunk)
+        assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    g.close()
+
+
+def test_lod2d_capacity_error(ctx):
+    rng = np.random.default_rng(1)
+    coefs, C0 = _adv_case(rng, 10, 10)
+    g = S.TGSolver(ctx, 10, 10, 0.5, 1e-3, 1.0, coefs, C0, capacity=3)
+    g.calc_step(3)
+    with pytest.raises(N.FrmError):
+        g.calc_step(1)
+    g.close()
+
+
+def test_lod2d_full_size_linearity_and_constant(ctx):
+    # 512 x 512 unknowns (sN = M = 513): the scheme is linear in (C0, C1, S),
+    # so sol(a X + b Y) = a sol(X) + b sol(Y) with C1 = S = 0; a constant field
+    # with C1 equal to it is a steady state (zero-gradient elsewhere).
+    sN = M = 513
+    rng = np.random.default_rng(7)
+    nn = (sN + 2) * (M + 2)
+    h = 1.0 / (sN + 1)
+    coefs = S.coefs_advection(rng.uniform(-0.1, 0.1, nn), rng.uniform(-0.1, 0.1, nn), 1e-3, h)
+    X, Y = rng.uniform(0, 1, nn), rng.uniform(-1, 1, nn)
+    steps = 40
+    out = []
+    for C0 in (X, Y, 2.0 * X - 3.0 * Y):
+        g = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0, S=0.0)
+        g.calc_step(steps)
+        out.append(g.b_C)
+        g.close()
+    assert relerr(out[2], 2.0 * out[0] - 3.0 * out[1]) <= 1e-12
+    g = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, np.full(nn, 0.37), capacity=steps, C1=0.37)
+    g.calc_step(steps)
+    np.testing.assert_allclose(g.b_C, 0.37, rtol=1e-13)
+    g.close()

@@ -1,0 +1,172 @@
+"""ctypes binding of libfrm.so (include/frm.h).
+
+The HIP library is the only implementation: if libfrm.so is missing or a call
+fails, this module raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfrm.so")
+
+FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
+PREC_F64, PREC_F32C, PREC_MFMA_BF16 = 0, 1, 2
+K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER = 0, 1, 2, 3, 4
+
+_D, _I, _I64, _U32 = ct.c_double, ct.c_int, ct.c_int64, ct.c_uint32
+_P, _V, _S = ct.POINTER(ct.c_double), ct.c_void_p, ct.c_size_t
+
+
+class FrmError(RuntimeError):
+    pass
+
+
+class MemtermDesc(ct.Structure):
+    _fields_ = [("ntargets", ct.c_int32), ("ldw", ct.c_int32), ("d_w", _V), ("scale", _D),
+                ("accumulate", ct.c_int32)]
+
+
+class TridiagDesc(ct.Structure):
+    _fields_ = [("nsys", ct.c_int32), ("n", ct.c_int32), ("stride_sys", ct.c_int64),
+                ("stride_elem", ct.c_int64)]
+
+
+class Lod2dDesc(ct.Structure):
+    _fields_ = [("sN", ct.c_int32), ("M", ct.c_int32), ("capacity", ct.c_int32),
+                ("precision", ct.c_int32), ("alpha", _D), ("tau", _D), ("sigma", _D),
+                ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P)]
+
+
+# name -> (restype, argtypes); every symbol include/frm.h declares
+SIGNATURES = {
+    "frm_ctx_create": (_I, [_I, ct.POINTER(_V)]),
+    "frm_ctx_destroy": (_I, [_V]),
+    "frm_sync": (_I, [_V]),
+    "frm_strerror": (ct.c_char_p, [_I]),
+    "frm_last_error": (ct.c_char_p, [_V]),
+    "frm_malloc": (_I, [_V, _S, ct.POINTER(_V)]),
+    "frm_free": (_I, [_V, _V]),
+    "frm_upload": (_I, [_V, _V, _V, _S]),
+    "frm_download": (_I, [_V, _V, _V, _S]),
+    "frm_timing_enable": (_I, [_V, _U32]),
+    "frm_timing_reset": (_I, [_V]),
+    "frm_timing_get": (_I, [_V, _I, ct.POINTER(_I64), _P, _P]),
+    "frm_hist_create": (_I, [_V, _I64, ct.c_int32, ct.c_int32, ct.POINTER(_V)]),
+    "frm_hist_destroy": (_I, [_V]),
+    "frm_hist_push": (_I, [_V, _V, _V, _V]),
+    "frm_hist_depth": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_hist_clear": (_I, [_V]),
+    "frm_memterm": (_I, [_V, _V, ct.POINTER(MemtermDesc), _V]),
+    "frm_tridiag": (_I, [_V, ct.POINTER(TridiagDesc), _V, _V, _V, _V, _V]),
+    "frm_lod2d_create": (_I, [_V, ct.POINTER(Lod2dDesc), ct.POINTER(_V)]),
+    "frm_lod2d_destroy": (_I, [_V]),
+    "frm_lod2d_set_state": (_I, [_V, _P]),
+    "frm_lod2d_step": (_I, [_V, ct.c_int32]),
+    "frm_lod2d_get_state": (_I, [_V, _P]),
+    "frm_lod2d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libfrm.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FrmError(f"{LIB_PATH} is missing: build it with `python -m theses_code_amd.build` "
+                           "(the HIP library is required; there is no CPU fallback)")
+        L = ct.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, ctx=None):
+    if rc != FRM_OK:
+        msg = lib().frm_last_error(ctx)
+        raise FrmError(f"{lib().frm_strerror(rc).decode()} ({rc}): {msg.decode() if msg else ''}")
+
+
+def _dp(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise ValueError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_P)
+
+
+class Context:
+    """One device and one HIP stream (frm_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _V()
+        _check(lib().frm_ctx_create(device, ct.byref(h)))
+        self.h = h
+
+    def sync(self):
+        _check(lib().frm_sync(self.h), self.h)
+
+    def malloc(self, nbytes: int) -> int:
+        p = _V()
+        _check(lib().frm_malloc(self.h, nbytes, ct.byref(p)), self.h)
+        return p.value
+
+    def free(self, ptr: int):
+        _check(lib().frm_free(self.h, ptr), self.h)
+
+    def upload(self, dptr: int, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        _check(lib().frm_upload(self.h, dptr, arr.ctypes.data, arr.nbytes), self.h)
+
+    def download(self, dptr: int, arr: np.ndarray):
+        _check(lib().frm_download(self.h, arr.ctypes.data, dptr, arr.nbytes), self.h)
+        return arr
+
+    def timing_enable(self, mask: int):
+        _check(lib().frm_timing_enable(self.h, mask), self.h)
+
+    def timing_reset(self):
+        _check(lib().frm_timing_reset(self.h), self.h)
+
+    def timing(self, kclass: int):
+        n, ms, by = _I64(), _D(), _D()
+        _check(lib().frm_timing_get(self.h, kclass, ct.byref(n), ct.byref(ms), ct.byref(by)), self.h)
+        return n.value, ms.value, by.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().frm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class DeviceArray:
+    """A float64 device buffer owned by a Context (for tests of the low-level ABI)."""
+
+    def __init__(self, ctx: Context, n: int):
+        self.ctx, self.n = ctx, int(n)
+        self.ptr = ctx.malloc(max(1, self.n) * 8)
+
+    @classmethod
+    def from_host(cls, ctx: Context, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(ctx, a.size)
+        ctx.upload(d.ptr, a)
+        return d
+
+    def to_host(self) -> np.ndarray:
+        return self.ctx.download(self.ptr, np.empty(self.n))
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = 0

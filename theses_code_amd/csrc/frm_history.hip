@@ -1,0 +1,219 @@
+// History store and memory-term kernels.
+//
+// The reference keeps one heap snapshot per (half-)step and, per node, loops
+// over all of them: mt_minimal.cpp:906-939 (H1), mt_psi_caputo.cpp:1315-1342
+// (H2), fr_d_t_g_2d.cpp:3340-3437 (H3, CPU) and the OpenCL Om kernel at
+// fr_d_t_g_2d.cpp:1704-1840 (one work-item per node, 32-wide LDS weight
+// tiles).  Here the store holds the increments Delta_s = C_{s+1} - C_s
+// snapshot-major (slot s holds all nodes, so a wavefront reading one slot
+// reads 64 consecutive nodes), and the sum for every node is split along
+// time into chunks: each workgroup streams one chunk of slots for a tile of
+// nodes, writes one FP64 partial per target, and the consumer adds the
+// partials in chunk order (deterministic).  The weights are uniform across
+// a wavefront, so they come through the scalar cache.
+#include <algorithm>
+
+#include "frm_history.hpp"
+
+namespace frm {
+
+// ---------------------------------------------------------------- kernels
+// part[(c*NT + t)*ld + i] = sum_{s in chunk c} w[t*ldw + depth-1-s] * Delta_s[i]
+template <int NT>
+__global__ __launch_bounds__(256) void k_memterm_f64(const double* __restrict__ hist, int64_t ld,
+                                                     int depth, const double* __restrict__ w,
+                                                     int ldw, int chunk,
+                                                     double* __restrict__ part) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (i >= ld) return;
+    const int c = blockIdx.y;
+    const int s0 = c * chunk;
+    const int s1 = min(depth, s0 + chunk);
+    double a0[NT], a1[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a0[t] = a1[t] = 0.0;
+    const double* p = hist + (int64_t)s0 * ld + i;
+    // lag-1 index of slot s is depth-1-s; walk the weights backwards
+    const double* wp = w + (depth - 1 - s0);
+    int s = s0;
+    for (; s + 4 <= s1; s += 4) {
+        double2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const double2*>(p + (int64_t)u * ld);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double wt = wp[t * ldw - u];
+                a0[t] = fma(wt, x[u].x, a0[t]);
+                a1[t] = fma(wt, x[u].y, a1[t]);
+            }
+        }
+        p += 4 * ld;
+        wp -= 4;
+    }
+    for (; s < s1; ++s) {
+        const double2 x = *reinterpret_cast<const double2*>(p);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const double wt = wp[t * ldw];
+            a0[t] = fma(wt, x.x, a0[t]);
+            a1[t] = fma(wt, x.y, a1[t]);
+        }
+        p += ld;
+        wp -= 1;
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        double2 r;
+        r.x = a0[t];
+        r.y = a1[t];
+        *reinterpret_cast<double2*>(part + ((int64_t)c * NT + t) * ld + i) = r;
+    }
+}
+
+// out[t*n + i] = scale * sum_c part[(c*NT+t)*ld + i] (+ out)
+__global__ void k_memterm_reduce(const double* __restrict__ part, int64_t ld, int nchunks,
+                                 int nt, int64_t n, double scale, int accumulate,
+                                 double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int c = 0; c < nchunks; ++c) acc += part[((int64_t)c * nt + t) * ld + i];
+    double v = scale * acc;
+    if (accumulate) v += out[(int64_t)t * n + i];
+    out[(int64_t)t * n + i] = v;
+}
+
+__global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+                      double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+// ---------------------------------------------------------------- host side
+int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
+               frm_hist** out) {
+    FRM_REQUIRE(ctx, ctx && out && n_nodes > 0 && capacity > 0, "frm_hist_create: bad arguments");
+    FRM_REQUIRE(ctx, precision == FRM_PREC_F64, "frm_hist_create: precision %d not supported",
+                precision);
+    frm_hist* h = new frm_hist();
+    h->ctx = ctx;
+    h->n_nodes = n_nodes;
+    h->ld = (n_nodes + kLdAlign - 1) / kLdAlign * kLdAlign;
+    h->capacity = capacity;
+    h->precision = precision;
+    h->chunk = kDefaultChunk;
+    h->max_chunks = (capacity + h->chunk - 1) / h->chunk;
+    size_t bytes = (size_t)capacity * h->ld * sizeof(double);
+    hipError_t e = hipMalloc(&h->data, bytes);
+    if (e != hipSuccess) {
+        delete h;
+        return check_hip(ctx, e, "hipMalloc(history store)");
+    }
+    e = hipMalloc(&h->part, (size_t)h->max_chunks * 2 * h->ld * sizeof(double));
+    if (e != hipSuccess) {
+        hipFree(h->data);
+        delete h;
+        return check_hip(ctx, e, "hipMalloc(partials)");
+    }
+    // zero once so padded lanes read zeros, not stale memory
+    e = hipMemsetAsync(h->data, 0, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipFree(h->data);
+        hipFree(h->part);
+        delete h;
+        return check_hip(ctx, e, "hipMemset(history store)");
+    }
+    *out = h;
+    return FRM_OK;
+}
+
+int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
+                     int* nchunks_out) {
+    const int depth = h->depth;
+    const int nchunks = (depth + h->chunk - 1) / h->chunk;
+    *nchunks_out = nchunks;
+    if (nchunks == 0) return FRM_OK;
+    FRM_REQUIRE(ctx, nchunks <= h->max_chunks, "memterm: %d chunks exceed workspace", nchunks);
+    const int threads = 256;
+    dim3 grid((unsigned)(h->ld / 2 + threads - 1) / threads, (unsigned)nchunks);
+    LaunchTimer timer(ctx, FRM_K_MEMTERM,
+                      (double)depth * h->n_nodes * sizeof(double) +
+                          (double)ntargets * h->n_nodes * sizeof(double));
+    if (ntargets == 1)
+        hipLaunchKernelGGL(k_memterm_f64<1>, grid, dim3(threads), 0, ctx->stream,
+                           (const double*)h->data, h->ld, depth, d_w, ldw, h->chunk, h->part);
+    else
+        hipLaunchKernelGGL(k_memterm_f64<2>, grid, dim3(threads), 0, ctx->stream,
+                           (const double*)h->data, h->ld, depth, d_w, ldw, h->chunk, h->part);
+    FRM_HIP(ctx, hipGetLastError());
+    return FRM_OK;
+}
+
+}  // namespace frm
+
+using namespace frm;
+
+extern "C" {
+
+int frm_hist_create(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
+                    frm_hist** out) {
+    return hist_alloc(ctx, n_nodes, capacity, precision, out);
+}
+
+int frm_hist_destroy(frm_hist* h) {
+    if (!h) return FRM_OK;
+    hipStreamSynchronize(h->ctx->stream);
+    hipFree(h->data);
+    hipFree(h->part);
+    delete h;
+    return FRM_OK;
+}
+
+int frm_hist_depth(const frm_hist* h, int32_t* depth) {
+    if (!h || !depth) return set_error(nullptr, FRM_E_INVALID, "frm_hist_depth: null argument");
+    *depth = h->depth;
+    return FRM_OK;
+}
+
+int frm_hist_clear(frm_hist* h) {
+    if (!h) return set_error(nullptr, FRM_E_INVALID, "frm_hist_clear: null argument");
+    h->depth = 0;
+    return FRM_OK;
+}
+
+int frm_hist_push(frm_ctx* ctx, frm_hist* h, const double* d_new, const double* d_old) {
+    FRM_REQUIRE(ctx, ctx && h && d_new && d_old, "frm_hist_push: null argument");
+    if (h->depth >= h->capacity)
+        return set_error(ctx, FRM_E_RANGE, "frm_hist_push: capacity %d reached", h->capacity);
+    double* slot = (double*)h->data + (int64_t)h->depth * h->ld;
+    const int threads = 256;
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)((h->n_nodes + threads - 1) / threads)),
+                       dim3(threads), 0, ctx->stream, d_new, d_old, h->n_nodes, slot);
+    FRM_HIP(ctx, hipGetLastError());
+    h->depth += 1;
+    return FRM_OK;
+}
+
+int frm_memterm(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double* d_out) {
+    FRM_REQUIRE(ctx, ctx && h && desc && d_out, "frm_memterm: null argument");
+    FRM_REQUIRE(ctx, desc->ntargets == 1 || desc->ntargets == 2, "frm_memterm: ntargets %d",
+                desc->ntargets);
+    FRM_REQUIRE(ctx, h->depth == 0 || (desc->d_w && desc->ldw >= h->depth),
+                "frm_memterm: weights shorter than the history depth");
+    int nchunks = 0;
+    int rc = memterm_partials(ctx, h, desc->ntargets, desc->d_w, desc->ldw, &nchunks);
+    if (rc) return rc;
+    const int threads = 256;
+    dim3 grid((unsigned)((h->n_nodes + threads - 1) / threads), (unsigned)desc->ntargets);
+    hipLaunchKernelGGL(k_memterm_reduce, grid, dim3(threads), 0, ctx->stream,
+                       (const double*)h->part, h->ld, nchunks, desc->ntargets, h->n_nodes,
+                       desc->scale, desc->accumulate, d_out);
+    FRM_HIP(ctx, hipGetLastError());
+    return FRM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// History store shared between the memory-term and solver translation units.
+#pragma once
+#include "frm_internal.hpp"
+
+struct frm_hist {
+    frm_ctx* ctx = nullptr;
+    int64_t n_nodes = 0;
+    int64_t ld = 0;  // slot stride in elements (multiple of kLdAlign)
+    int32_t capacity = 0;
+    int32_t depth = 0;
+    int32_t precision = FRM_PREC_F64;
+    int32_t chunk = 0;       // slots per split-K chunk
+    int32_t max_chunks = 0;  // partial-sum workspace size
+    void* data = nullptr;    // capacity * ld elements
+    double* part = nullptr;  // max_chunks * 2 * ld partial sums
+};
+
+namespace frm {
+
+constexpr int64_t kLdAlign = 512;   // 256 threads x 2 nodes per thread
+constexpr int32_t kDefaultChunk = 256;
+
+int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
+               frm_hist** out);
+// Launches the split-K partial sums for the current depth into h->part;
+// returns the number of chunks written (0 when the store is empty).
+int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
+                     int* nchunks);
+
+}  // namespace frm

@@ -1,0 +1,441 @@
+// 2D locally-one-dimensional fractional solver (the hot path of
+// TG_solver::calc_step, fr_d_t_g_2d.cpp:2552-2718, dt_alg = 0, g(t) = t).
+//
+// Per step n the reference, for every row j, forms Om (3306-3492) with the
+// full history sum over s = n-1..0, builds Al/Bt (3267-3304, 3494-3553) and
+// back-substitutes (2548-2550); then the same for every column with the
+// extra half-step term kb(n,1,0,1/2)*(C_n - C_{n+1/2}) (3430-3436).  Its
+// OpenCL version launches Om, Al, Bt and U separately, with Al/Bt/U on one
+// work-item per system and a Finish() after each launch.
+//
+// Here one step is three launches on one stream:
+//   1. memterm: both half-step history sums in one pass over the increments
+//      (rows use kb(s,1/2,0,1), columns kb(s,1,0,1); both depend only on the
+//      lag n-s for g(t) = t), split along time into FP64 partials;
+//   2. rows: one wavefront per row -- RHS assembly, Thomas forward and back
+//      substitution (chunked over lanes, wavefront-scan seeds), boundary
+//      nodes, and a copy of C_n for step 3;
+//   3. cols: the same per column, plus the push of Delta_n = C_{n+1} - C_n
+//      into the history store.
+// With g(t) = t and a time-independent operator the tridiagonal matrices do
+// not change between steps, so their Thomas alpha coefficients are computed
+// once on the host in FP64, with the reference's formula and order.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "frm_history.hpp"
+#include "frm_wavescan.hpp"
+
+struct frm_lod2d {
+    frm_ctx* ctx = nullptr;
+    int sN = 0, M = 0, ld = 0;
+    int NX = 0, NY = 0;  // interior nodes per row / per column
+    int capacity = 0;
+    int tstep = 0;
+    double alpha = 0, tau = 0, sigma = 0, C1 = 0, S = 0;
+    double sg = 0;                  // sigma / tau
+    double kr0 = 0, kc0 = 0, kch = 0;  // current-step weights
+    frm_hist* hist = nullptr;
+    double* C = nullptr;       // state, reference layout
+    double* Cprev = nullptr;   // C_n at interior nodes, compact node order
+    double* w = nullptr;       // [2][capacity] lag-indexed history weights
+    double* rcoef = nullptr;   // rows:    [3][M-1][NX]  a, f, al
+    double* ccoef = nullptr;   // columns: [3][sN-1][NY] a, f, al
+    int cm_rows = 0, cm_cols = 0;
+};
+
+namespace frm {
+
+// ------------------------------------------------------------- weights
+// kb(s, ja, sa, l) for g(t) = t depends on d = n + ja - s - sa only:
+//   Da tau^(1-a)/(1-a) * (d^(1-a) - (d-l)^(1-a)),   Da = 1/Gamma(1-a)
+// (the closed form of _kb_row, fr_d_t_g_2d.cpp:2937-2963).  The difference
+// of powers is evaluated as -d^(1-a) expm1((1-a) log1p(-l/d)), which keeps
+// full relative precision at large lags where the direct difference cancels.
+static double caputo_kb(double alpha, double tau, double d, double l) {
+    const double c = tau == 0 ? 0 : std::pow(tau, 1.0 - alpha) / (1.0 - alpha) / std::tgamma(1.0 - alpha);
+    if (d <= 0) return 0.0;
+    if (l >= d) return c * std::pow(d, 1.0 - alpha);
+    return -c * std::pow(d, 1.0 - alpha) * std::expm1((1.0 - alpha) * std::log1p(-l / d));
+}
+
+// ------------------------------------------------------------- kernels
+struct RowArgs {
+    double* C;
+    double* Cprev;
+    const double* coef;  // [3][lines][n]
+    const double* part;  // partial sums [nch][2][ldh], target 0
+    int64_t ldh;
+    int nch;
+    int ld, lines, n, NX;
+    double negsk, S_half, sg, b0, al1;
+};
+
+template <int CM>
+__global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
+    const int lane = threadIdx.x & 63;
+    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (line >= p.lines) return;  // whole wavefront exits together
+    const int j = line + 1;
+    const int n = p.n;
+    const int e0 = lane * CM;
+    const double* ca = p.coef + (int64_t)line * n;
+    const double* cf = ca + (int64_t)p.lines * n;
+    const double* cl = cf + (int64_t)p.lines * n;
+    double a[CM], f[CM], al[CM], v[CM];
+    double* Crow = p.C + (int64_t)j * p.ld;
+    const int64_t qbase = (int64_t)(j - 1) * p.NX;
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        if (e < n) {
+            a[k] = ca[e];
+            f[k] = cf[e];
+            al[k] = cl[e];
+            const double u = Crow[e + 1];
+            p.Cprev[qbase + e] = u;
+            double ts = 0.0;
+            for (int c = 0; c < p.nch; ++c) ts += p.part[(int64_t)c * 2 * p.ldh + qbase + e];
+            double om = p.negsk * u;
+            om += p.S_half;
+            om += p.sg * ts;
+            v[k] = om;
+        } else {
+            a[k] = f[k] = al[k] = v[k] = 0.0;
+        }
+    }
+    const double un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+#pragma unroll
+    for (int k = 0; k < CM; ++k)
+        if (e0 + k < n) Crow[e0 + k + 1] = v[k];
+    if (lane == 0) Crow[0] = p.al1 * v[0] + p.b0;
+    if (lane == 63) {
+        Crow[n + 1] = un;
+        Crow[n + 2] = un;
+    }
+}
+
+struct ColArgs {
+    double* C;
+    const double* Cprev;
+    double* slot;        // history slot for Delta_n (compact node order)
+    const double* coef;  // [3][lines][n]
+    const double* part;  // target 1 partials at part + ldh
+    int64_t ldh;
+    int nch;
+    int ld, lines, n, NX;
+    double negsk, S_half, sg, kch, b0, al1;
+};
+
+template <int CM>
+__global__ __launch_bounds__(256) void k_lod_cols(ColArgs p) {
+    const int lane = threadIdx.x & 63;
+    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (line >= p.lines) return;
+    const int i = line + 1;
+    const int n = p.n;
+    const int e0 = lane * CM;
+    const double* ca = p.coef + (int64_t)line * n;
+    const double* cf = ca + (int64_t)p.lines * n;
+    const double* cl = cf + (int64_t)p.lines * n;
+    double a[CM], f[CM], al[CM], v[CM], cp[CM];
+    double* Ccol = p.C + i;
+    const int64_t ld = p.ld;
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        if (e < n) {
+            a[k] = ca[e];
+            f[k] = cf[e];
+            al[k] = cl[e];
+            const int64_t q = (int64_t)e * p.NX + (i - 1);  // node (i, j = e+1)
+            const double u = Ccol[(int64_t)(e + 1) * ld];
+            cp[k] = p.Cprev[q];
+            double ts = 0.0;
+            for (int c = 0; c < p.nch; ++c) ts += p.part[((int64_t)c * 2 + 1) * p.ldh + q];
+            ts += p.kch * (cp[k] - u);
+            double om = p.negsk * u;
+            om += p.S_half;
+            om += p.sg * ts;
+            v[k] = om;
+        } else {
+            a[k] = f[k] = al[k] = v[k] = cp[k] = 0.0;
+        }
+    }
+    const double un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        if (e < n) {
+            Ccol[(int64_t)(e + 1) * ld] = v[k];
+            p.slot[(int64_t)e * p.NX + (i - 1)] = v[k] - cp[k];
+        }
+    }
+    if (lane == 0) Ccol[0] = p.al1 * v[0] + p.b0;
+    if (lane == 63) {
+        Ccol[(int64_t)(n + 1) * ld] = un;
+        Ccol[(int64_t)(n + 2) * ld] = un;
+    }
+}
+
+// ------------------------------------------------------------- host helpers
+// Thomas alpha coefficients of one line in the reference's order
+// (al1: fr_d_t_g_2d.cpp:3267-3304, R: 3257-3265); f = Al[i+1]/B(i), or the
+// zero-pivot form 1/(R - A Al[i]) (3519-3522).
+static void line_coefs(int n, const double* A, const double* B, int64_t stride, double sgk,
+                       double al_first, double* oa, double* of, double* oal) {
+    double Al = al_first;
+    for (int e = 0; e < n; ++e) {
+        const double a = A[e * stride], b = B[e * stride];
+        const double R = sgk + a + b;
+        const double Aln = b / (R - a * Al);
+        oa[e] = a;
+        if (Aln == 0.0 && b == 0.0)
+            of[e] = 1.0 / (R - a * Al);
+        else
+            of[e] = Aln / b;
+        oal[e] = Aln;
+        Al = Aln;
+    }
+}
+
+static int pick_cm(int n) {
+    if (n <= 64 * 4) return 4;
+    if (n <= 64 * 8) return 8;
+    if (n <= 64 * 16) return 16;
+    if (n <= 64 * 32) return 32;
+    return 0;
+}
+
+template <typename Args>
+static hipError_t launch_rows(int cm, dim3 grid, hipStream_t st, const Args& a) {
+    switch (cm) {
+        case 4: hipLaunchKernelGGL(k_lod_rows<4>, grid, dim3(256), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(k_lod_rows<8>, grid, dim3(256), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(k_lod_rows<16>, grid, dim3(256), 0, st, a); break;
+        case 32: hipLaunchKernelGGL(k_lod_rows<32>, grid, dim3(256), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename Args>
+static hipError_t launch_cols(int cm, dim3 grid, hipStream_t st, const Args& a) {
+    switch (cm) {
+        case 4: hipLaunchKernelGGL(k_lod_cols<4>, grid, dim3(256), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(k_lod_cols<8>, grid, dim3(256), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(k_lod_cols<16>, grid, dim3(256), 0, st, a); break;
+        case 32: hipLaunchKernelGGL(k_lod_cols<32>, grid, dim3(256), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static void lod_free(frm_lod2d* s) {
+    if (!s) return;
+    if (s->hist) frm_hist_destroy(s->hist);
+    hipFree(s->C);
+    hipFree(s->Cprev);
+    hipFree(s->w);
+    hipFree(s->rcoef);
+    hipFree(s->ccoef);
+    delete s;
+}
+
+}  // namespace frm
+
+using namespace frm;
+
+extern "C" {
+
+int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
+    FRM_REQUIRE(ctx, ctx && d && out, "frm_lod2d_create: null argument");
+    *out = nullptr;
+    FRM_REQUIRE(ctx, d->sN >= 2 && d->M >= 2, "frm_lod2d_create: grid %dx%d too small", d->sN,
+                d->M);
+    FRM_REQUIRE(ctx, d->capacity >= 1, "frm_lod2d_create: capacity %d", d->capacity);
+    FRM_REQUIRE(ctx, d->alpha > 0 && d->alpha < 1 && d->tau > 0,
+                "frm_lod2d_create: need 0 < alpha < 1 and tau > 0");
+    FRM_REQUIRE(ctx, d->rowA && d->rowB && d->colA && d->colB,
+                "frm_lod2d_create: coefficient arrays missing");
+    FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64, "frm_lod2d_create: precision %d not built",
+                d->precision);
+    const int cmr = pick_cm(d->sN - 1), cmc = pick_cm(d->M - 1);
+    FRM_REQUIRE(ctx, cmr && cmc, "frm_lod2d_create: lines longer than %d unknowns", 64 * 32);
+    FRM_HIP(ctx, hipSetDevice(ctx->device));
+
+    frm_lod2d* s = new frm_lod2d();
+    s->ctx = ctx;
+    s->sN = d->sN;
+    s->M = d->M;
+    s->ld = d->sN + 2;
+    s->NX = d->sN - 1;
+    s->NY = d->M - 1;
+    s->capacity = d->capacity;
+    s->alpha = d->alpha;
+    s->tau = d->tau;
+    s->sigma = d->sigma;
+    s->C1 = d->C1;
+    s->S = d->S;
+    s->sg = d->sigma / d->tau;
+    s->cm_rows = cmr;
+    s->cm_cols = cmc;
+    // current-step weights: kb(n,1/2,0,1/2), kb(n,1,1/2,1/2), kb(n,1,0,1/2)
+    s->kr0 = caputo_kb(d->alpha, d->tau, 0.5, 0.5);
+    s->kc0 = caputo_kb(d->alpha, d->tau, 0.5, 0.5);
+    s->kch = caputo_kb(d->alpha, d->tau, 1.0, 0.5);
+
+    const int64_t nodes = (int64_t)s->NX * s->NY;
+    const size_t nn = (size_t)s->ld * (d->M + 2);
+    int rc = hist_alloc(ctx, nodes, d->capacity, d->precision, &s->hist);
+    if (rc) {
+        lod_free(s);
+        return rc;
+    }
+    hipError_t e = hipMalloc(&s->C, nn * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&s->Cprev, nodes * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&s->w, (size_t)2 * d->capacity * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&s->rcoef, (size_t)3 * (d->M - 1) * s->NX * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&s->ccoef, (size_t)3 * (d->sN - 1) * s->NY * sizeof(double));
+    if (e != hipSuccess) {
+        lod_free(s);
+        return check_hip(ctx, e, "hipMalloc(lod2d)");
+    }
+    // lag-indexed history weights: rows kb(s,1/2,0,1) at lag k = n-s is
+    // d = k+1/2, l = 1; columns kb(s,1,0,1) is d = k+1, l = 1
+    std::vector<double> w(2 * (size_t)d->capacity);
+    for (int k = 1; k <= d->capacity; ++k) {
+        w[k - 1] = caputo_kb(d->alpha, d->tau, k + 0.5, 1.0);
+        w[d->capacity + k - 1] = caputo_kb(d->alpha, d->tau, k + 1.0, 1.0);
+    }
+    // line coefficients
+    std::vector<double> rc3((size_t)3 * (d->M - 1) * s->NX), cc3((size_t)3 * (d->sN - 1) * s->NY);
+    const int64_t lrows = d->M - 1, lcols = d->sN - 1;
+    for (int j = 1; j < d->M; ++j) {
+        const int64_t base = (int64_t)j * s->ld + 1;
+        double* oa = rc3.data() + (int64_t)(j - 1) * s->NX;
+        line_coefs(s->NX, d->rowA + base, d->rowB + base, 1, s->sg * s->kr0, 0.0, oa,
+                   oa + lrows * s->NX, oa + 2 * lrows * s->NX);
+    }
+    for (int i = 1; i < d->sN; ++i) {
+        const int64_t base = (int64_t)1 * s->ld + i;
+        double* oa = cc3.data() + (int64_t)(i - 1) * s->NY;
+        line_coefs(s->NY, d->colA + base, d->colB + base, s->ld, s->sg * s->kc0, 1.0, oa,
+                   oa + lcols * s->NY, oa + 2 * lcols * s->NY);
+    }
+    e = hipMemcpy(s->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(s->rcoef, rc3.data(), rc3.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(s->ccoef, cc3.data(), cc3.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        lod_free(s);
+        return check_hip(ctx, e, "hipMemcpy(lod2d setup)");
+    }
+    *out = s;
+    return FRM_OK;
+}
+
+int frm_lod2d_destroy(frm_lod2d* s) {
+    if (!s) return FRM_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    lod_free(s);
+    return FRM_OK;
+}
+
+int frm_lod2d_set_state(frm_lod2d* s, const double* h_C) {
+    if (!s || !h_C) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_lod2d_set_state: null");
+    frm_ctx* ctx = s->ctx;
+    FRM_HIP(ctx, hipMemcpyAsync(s->C, h_C, (size_t)s->ld * (s->M + 2) * sizeof(double),
+                                hipMemcpyHostToDevice, ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->tstep = 0;
+    s->hist->depth = 0;
+    return FRM_OK;
+}
+
+int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
+    if (!s) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_step: null solver");
+    frm_ctx* ctx = s->ctx;
+    FRM_REQUIRE(ctx, nsteps >= 0, "frm_lod2d_step: nsteps %d", nsteps);
+    if (s->hist->depth + nsteps > s->capacity)
+        return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
+                         s->hist->depth + nsteps, s->capacity);
+    const double S_half = s->S * 0.5;
+    const int wpb = 256 / kWave;
+    for (int it = 0; it < nsteps; ++it) {
+        const int n = s->tstep;
+        int nch = 0;
+        int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);
+        if (rc) return rc;
+        {
+            RowArgs a;
+            a.C = s->C;
+            a.Cprev = s->Cprev;
+            a.coef = s->rcoef;
+            a.part = s->hist->part;
+            a.ldh = s->hist->ld;
+            a.nch = nch;
+            a.ld = s->ld;
+            a.lines = s->M - 1;
+            a.n = s->NX;
+            a.NX = s->NX;
+            a.negsk = -s->sg * s->kr0;
+            a.S_half = S_half;
+            a.sg = s->sg;
+            a.b0 = s->C1;
+            a.al1 = 0.0;
+            LaunchTimer t(ctx, FRM_K_ROWS, 0.0);
+            FRM_HIP(ctx, launch_rows(s->cm_rows, dim3((a.lines + wpb - 1) / wpb), ctx->stream, a));
+        }
+        {
+            ColArgs a;
+            a.C = s->C;
+            a.Cprev = s->Cprev;
+            a.slot = (double*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld;
+            a.coef = s->ccoef;
+            a.part = s->hist->part;
+            a.ldh = s->hist->ld;
+            a.nch = nch;
+            a.ld = s->ld;
+            a.lines = s->sN - 1;
+            a.n = s->NY;
+            a.NX = s->NX;
+            a.negsk = -s->sg * s->kc0;
+            a.S_half = S_half;
+            a.sg = s->sg;
+            a.kch = s->kch;
+            a.b0 = 0.0;
+            a.al1 = 1.0;
+            LaunchTimer t(ctx, FRM_K_COLS, 0.0);
+            FRM_HIP(ctx, launch_cols(s->cm_cols, dim3((a.lines + wpb - 1) / wpb), ctx->stream, a));
+        }
+        s->hist->depth += 1;
+        s->tstep = n + 1;
+    }
+    return FRM_OK;
+}
+
+int frm_lod2d_get_state(frm_lod2d* s, double* h_C) {
+    if (!s || !h_C) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_lod2d_get_state: null");
+    frm_ctx* ctx = s->ctx;
+    FRM_HIP(ctx, hipMemcpyAsync(h_C, s->C, (size_t)s->ld * (s->M + 2) * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return FRM_OK;
+}
+
+int frm_lod2d_tstep(const frm_lod2d* s, int32_t* tstep) {
+    if (!s || !tstep) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_tstep: null");
+    *tstep = s->tstep;
+    return FRM_OK;
+}
+
+int frm_lod2d_state_ptr(frm_lod2d* s, double** d_C) {
+    if (!s || !d_C) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_state_ptr: null");
+    *d_C = s->C;
+    return FRM_OK;
+}
+
+}  // extern "C"

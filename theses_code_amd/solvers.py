@@ -1,0 +1,92 @@
+"""Host-side mirrors of the reference's solver classes over the HIP library.
+
+TGSolver mirrors TG_solver (fr_d_t_g_2d.cpp:2037-2771): it keeps the field in
+the reference layout (b_C[(sN+2)*(M+2)], idx(i,j) = i + j*(sN+2)), counts
+tstep, and calc_step() advances one LOD step -- on the GPU, through
+frm_lod2d_step.  Coefficients are given as the per-node off-diagonals of the
+row and column sweeps, built by one of the coefficient forms below.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _native as N
+
+
+def tg_ref_spacing(sN: int, M: int):
+    """Grid spacings TG_solver's constructor derives (fr_d_t_g_2d.cpp:2337-2388):
+    dL = 2 Lx/(2 sN + 1) with Lx = 5, dLx = Q/M with Q = k (L/2 - H),
+    k = 2 sqrt(5)/(2.5 pi), L = 22, H = 1/sqrt(5)."""
+    k = 2.0 * np.sqrt(5.0) / (np.pi * 2.5)
+    H = 1 / np.sqrt(5.0)
+    Q = k * (0.5 * 22.0 - H)
+    return 2 * 5.0 / (2.0 * sN + 1.0), Q / M
+
+
+def coefs_ref(V2: np.ndarray, dv: float, dL: float, dLx: float):
+    """Sweep coefficients of TG_solver_<func,fixed_d=1,kform=0>
+    (fr_d_t_g_2d.cpp:3242-3265): rows A_=(V2/dL)(dv/dL+1/2), B=(V2/dL)(dv/dL-1/2);
+    columns A_=B=V2 dv/dLx^2 (V2 = squared speed per node)."""
+    V2 = np.asarray(V2, dtype=np.float64)
+    rowA = (V2 / dL) * ((0.5 * (dv + dv) / dL) + 0.5)
+    rowB = (V2 / dL) * ((0.5 * (dv + dv) / dL) - 0.5)
+    colA = (V2 / (dLx * dLx)) * 0.5 * (dv + dv)
+    return rowA, rowB, colA, colA.copy()
+
+
+def coefs_advection(vx: np.ndarray, vy: np.ndarray, D: float, h: float):
+    """Central-difference advection-diffusion sweep coefficients on a uniform
+    grid: row A_ = D/h^2 + vx/(2h), B = D/h^2 - vx/(2h); columns with vy.
+    The LOD split treats x-terms in the row half-step and y-terms in the
+    column half-step, exactly as the reference splits its operator."""
+    vx = np.asarray(vx, dtype=np.float64)
+    vy = np.asarray(vy, dtype=np.float64)
+    d = D / (h * h)
+    return d + vx / (2 * h), d - vx / (2 * h), d + vy / (2 * h), d - vy / (2 * h)
+
+
+class TGSolver:
+    """TG_solver::calc_step on the MI355X (dt_alg = 0, g(t) = t)."""
+
+    def __init__(self, ctx: N.Context, sN: int, M: int, alpha: float, tau: float, sigma: float,
+                 coefs, C0: np.ndarray, capacity: int, C1: float = 1.0, S: float = 0.0,
+                 precision: int = N.PREC_F64):
+        self.ctx, self.sN, self.M = ctx, sN, M
+        nn = (sN + 2) * (M + 2)
+        self._coefs = [np.ascontiguousarray(c, dtype=np.float64).reshape(nn) for c in coefs]
+        d = N.Lod2dDesc(sN, M, capacity, precision, alpha, tau, sigma, C1, S,
+                        *[N._dp(c) for c in self._coefs])
+        h = N._V()
+        N._check(N.lib().frm_lod2d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
+        self.h = h
+        self.set_state(C0)
+
+    def set_state(self, C: np.ndarray):
+        C = np.ascontiguousarray(C, dtype=np.float64).reshape(-1)
+        assert C.size == (self.sN + 2) * (self.M + 2)
+        N._check(N.lib().frm_lod2d_set_state(self.h, N._dp(C)), self.ctx.h)
+
+    def calc_step(self, nsteps: int = 1):
+        N._check(N.lib().frm_lod2d_step(self.h, nsteps), self.ctx.h)
+
+    @property
+    def tstep(self) -> int:
+        t = ct.c_int32()
+        N._check(N.lib().frm_lod2d_tstep(self.h, ct.byref(t)), self.ctx.h)
+        return t.value
+
+    @property
+    def b_C(self) -> np.ndarray:
+        out = np.empty((self.M + 2) * (self.sN + 2))
+        N._check(N.lib().frm_lod2d_get_state(self.h, N._dp(out)), self.ctx.h)
+        return out.reshape(self.M + 2, self.sN + 2)
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().frm_lod2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,55 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+2D Caputo convection-diffusion on [0,1]^2 (the fr_d_t_g_2d LOD form): a grid
+of sN = M = n+1 intervals gives n x n interior unknowns per half-step (n row
+systems of n unknowns, then n column systems), nodes i = 0..sN+1 at x = i*h,
+h = 1/sN.  The velocity is the curl of a Gaussian random stream function
+(FFT-filtered white noise, Gaussian spectrum, correlation length 0.1) scaled
+to max|v|; the initial field is a sum of 8 Gaussian blobs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .solvers import coefs_advection
+
+SEED_BASE = 0x5EED0000
+
+
+def gaussian_random_field(n: int, corr: float, h: float, rng) -> np.ndarray:
+    """Periodic Gaussian random field on an n x n grid of spacing h."""
+    noise = rng.normal(size=(n, n))
+    k = np.fft.fftfreq(n, d=h) * 2 * np.pi
+    kx, ky = np.meshgrid(k, k, indexing="xy")
+    spec = np.exp(-0.25 * (kx ** 2 + ky ** 2) * corr ** 2)
+    f = np.real(np.fft.ifft2(np.fft.fft2(noise) * spec))
+    return f / np.std(f)
+
+
+def lod2d_case(n_interior: int, seed: int, vmax: float = 0.1, D: float = 1e-3,
+               blobs: int = 8, blob_sigma: float = 0.05):
+    """Returns (sN, M, coefs, C0) for an n_interior^2 LOD problem."""
+    rng = np.random.default_rng(seed)
+    sN = M = n_interior + 1
+    h = 1.0 / sN
+    ng = sN + 2  # nodes 0..sN+1 in both directions, layout [j][i]
+    psi = gaussian_random_field(ng, 0.1, h, rng)
+    # v = curl psi = (d psi/dy, -d psi/dx), central differences (periodic)
+    vx = (np.roll(psi, -1, axis=0) - np.roll(psi, 1, axis=0)) / (2 * h)
+    vy = -(np.roll(psi, -1, axis=1) - np.roll(psi, 1, axis=1)) / (2 * h)
+    s = vmax / np.max(np.hypot(vx, vy))
+    vx, vy = vx * s, vy * s
+    x = np.arange(ng) * h
+    X, Y = np.meshgrid(x, x, indexing="xy")  # X[j, i] = x_i, Y[j, i] = y_j
+    C0 = np.zeros((ng, ng))
+    amps = rng.uniform(0, 1, blobs)
+    ctr = rng.uniform(0, 1, (blobs, 2))
+    for a, (cx, cy) in zip(amps, ctr):
+        C0 += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * blob_sigma ** 2))
+    coefs = coefs_advection(vx, vy, D, h)
+    return sN, M, coefs, C0
+
+
+# the 2D Caputo convection-diffusion workload quoted by the headline metric
+C1_WORKLOAD = dict(name="lod2d_caputo_512x512_fp64", n_interior=512, alpha=0.7, tau=1e-3,
+                   steps=5000, sigma=1.0, D=1e-3, vmax=0.1, seed=SEED_BASE + 1)
