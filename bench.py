@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=4,
                     help="oracle steps timed for the CPU baseline (0 skips it)")
     ap.add_argument("--parity-steps", type=int, default=12)
+    ap.add_argument("--no-copy-probe", action="store_true",
+                    help="skip the STREAM-copy bandwidth probe")
     return ap.parse_args()
 
 
@@ -139,6 +141,8 @@ def main():
                 "avg_launch_ms": ms / launches if launches else None,
                 "bytes_per_launch_avg": nbytes / launches if launches else None}
 
+    if not args.no_copy_probe:
+        roofline["measured_copy_GBs"] = ctx.copy_bandwidth()
     cpu = None
     if args.cpu_steps > 0:
         cpu = cpu_baseline(case, cfg, depth=max(K // 2, 1), nsteps=args.cpu_steps)

@@ -1,0 +1,24 @@
+/* frm_diag.h -- diagnostics of libfrm with no counterpart in the reference.
+ *
+ * frm_diag_copy runs a plain device-to-device copy kernel (one 16-byte load
+ * and store per lane) `reps` times on the context's stream and reports the
+ * average kernel time from HIP events: the measured achievable HBM
+ * bandwidth the memory-term roofline is compared against, and the known-size
+ * kernel that scales rocprof FETCH_SIZE / WRITE_SIZE counters. */
+#ifndef FRM_DIAG_H
+#define FRM_DIAG_H
+#include <stddef.h>
+
+#include "frm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int frm_diag_copy(frm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, int reps,
+                  double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -69,6 +69,7 @@ SIGNATURES = {
     "frm_lod2d_get_state": (_I, [_V, _P]),
     "frm_lod2d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
+    "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
 }
 
 _lib = None
@@ -139,6 +140,17 @@ class Context:
         n, ms, by = _I64(), _D(), _D()
         _check(lib().frm_timing_get(self.h, kclass, ct.byref(n), ct.byref(ms), ct.byref(by)), self.h)
         return n.value, ms.value, by.value
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 20) -> float:
+        """Measured device copy bandwidth in GB/s (read + write bytes)."""
+        src, dst = self.malloc(nbytes), self.malloc(nbytes)
+        ms = _D()
+        try:
+            _check(lib().frm_diag_copy(self.h, dst, src, nbytes, reps, ct.byref(ms)), self.h)
+        finally:
+            self.free(src)
+            self.free(dst)
+        return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if getattr(self, "h", None):

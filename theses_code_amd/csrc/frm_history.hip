@@ -6,11 +6,12 @@
 // fr_d_t_g_2d.cpp:1704-1840 (one work-item per node, 32-wide LDS weight
 // tiles).  Here the store holds the increments Delta_s = C_{s+1} - C_s
 // snapshot-major (slot s holds all nodes, so a wavefront reading one slot
-// reads 64 consecutive nodes), and the sum for every node is split along
-// time into chunks: each workgroup streams one chunk of slots for a tile of
-// nodes, writes one FP64 partial per target, and the consumer adds the
-// partials in chunk order (deterministic).  The weights are uniform across
-// a wavefront, so they come through the scalar cache.
+// reads 64 consecutive nodes), and each lane sums its nodes over all slots
+// for one or two weight vectors (the two LOD half-step targets share the
+// pass).  When there are too few nodes to fill the GPU the slots are split
+// into time chunks with one FP64 partial each, which the consumer adds in
+// chunk order (deterministic).  The weights are uniform across a wavefront,
+// so they come through the scalar cache.
 #include <algorithm>
 
 #include "frm_history.hpp"
@@ -19,11 +20,23 @@ namespace frm {
 
 // ---------------------------------------------------------------- kernels
 // part[(c*NT + t)*ld + i] = sum_{s in chunk c} w[t*ldw + depth-1-s] * Delta_s[i]
+//
+// Each lane owns two adjacent nodes and streams their increments slot after
+// slot with 16-byte non-temporal loads (the history is read once per step
+// and never reused before it is evicted, so it should not displace the
+// state, coefficients and partials from the caches), four slots in flight.
+// Measured on MI355X: one pass per node column over all slots, with every
+// wavefront walking the same slot region at the same time, keeps DRAM rows
+// open and sustains ~7.0 TB/s; splitting the slots into chunks that
+// different workgroups read concurrently scatters the accesses and drops to
+// 5.1-5.5 TB/s.  The split is therefore used only when the node count alone
+// cannot fill the GPU.
+typedef double frm_dv2 __attribute__((ext_vector_type(2)));
+
 template <int NT>
-__global__ __launch_bounds__(256) void k_memterm_f64(const double* __restrict__ hist, int64_t ld,
-                                                     int depth, const double* __restrict__ w,
-                                                     int ldw, int chunk,
-                                                     double* __restrict__ part) {
+__global__ __launch_bounds__(kMemtermThreads) void k_memterm_f64(
+    const double* __restrict__ hist, int64_t ld, int depth, const double* __restrict__ w, int ldw,
+    int chunk, double* __restrict__ part) {
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
     if (i >= ld) return;
     const int c = blockIdx.y;
@@ -32,14 +45,15 @@ __global__ __launch_bounds__(256) void k_memterm_f64(const double* __restrict__ 
     double a0[NT], a1[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) a0[t] = a1[t] = 0.0;
-    const double* p = hist + (int64_t)s0 * ld + i;
+    const int64_t ldv = ld / 2;
+    const frm_dv2* p = reinterpret_cast<const frm_dv2*>(hist + (int64_t)s0 * ld + i);
     // lag-1 index of slot s is depth-1-s; walk the weights backwards
     const double* wp = w + (depth - 1 - s0);
     int s = s0;
     for (; s + 4 <= s1; s += 4) {
-        double2 x[4];
+        frm_dv2 x[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const double2*>(p + (int64_t)u * ld);
+        for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(p + (int64_t)u * ldv);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
 #pragma unroll
@@ -49,26 +63,25 @@ __global__ __launch_bounds__(256) void k_memterm_f64(const double* __restrict__ 
                 a1[t] = fma(wt, x[u].y, a1[t]);
             }
         }
-        p += 4 * ld;
+        p += 4 * ldv;
         wp -= 4;
     }
     for (; s < s1; ++s) {
-        const double2 x = *reinterpret_cast<const double2*>(p);
+        const frm_dv2 x = __builtin_nontemporal_load(p);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const double wt = wp[t * ldw];
             a0[t] = fma(wt, x.x, a0[t]);
             a1[t] = fma(wt, x.y, a1[t]);
         }
-        p += ld;
+        p += ldv;
         wp -= 1;
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        double2 r;
-        r.x = a0[t];
-        r.y = a1[t];
-        *reinterpret_cast<double2*>(part + ((int64_t)c * NT + t) * ld + i) = r;
+        double* o = part + ((int64_t)c * NT + t) * ld + i;
+        o[0] = a0[t];
+        o[1] = a1[t];
     }
 }
 
@@ -104,8 +117,7 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
     h->ld = (n_nodes + kLdAlign - 1) / kLdAlign * kLdAlign;
     h->capacity = capacity;
     h->precision = precision;
-    h->chunk = kDefaultChunk;
-    h->max_chunks = (capacity + h->chunk - 1) / h->chunk;
+    h->max_chunks = kMaxChunks;
     size_t bytes = (size_t)capacity * h->ld * sizeof(double);
     hipError_t e = hipMalloc(&h->data, bytes);
     if (e != hipSuccess) {
@@ -131,24 +143,35 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
     return FRM_OK;
 }
 
+// Number of time chunks for a launch: one, unless the nodes alone give fewer
+// than kMemtermWaves wavefronts; chunks never get shorter than kMinChunk.
+static int pick_chunks(const frm_hist* h, int depth) {
+    if (depth <= 0) return 0;
+    const int64_t waves = h->ld / (2 * kWave);
+    int nch = 1;
+    if (waves < kMemtermWaves) nch = (int)((kMemtermWaves + waves - 1) / waves);
+    nch = std::min<int>(nch, (depth + kMinChunk - 1) / kMinChunk);
+    return std::max(1, std::min(nch, h->max_chunks));
+}
+
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
                      int* nchunks_out) {
     const int depth = h->depth;
-    const int nchunks = (depth + h->chunk - 1) / h->chunk;
+    const int nchunks = pick_chunks(h, depth);
     *nchunks_out = nchunks;
     if (nchunks == 0) return FRM_OK;
-    FRM_REQUIRE(ctx, nchunks <= h->max_chunks, "memterm: %d chunks exceed workspace", nchunks);
-    const int threads = 256;
-    dim3 grid((unsigned)(h->ld / 2 + threads - 1) / threads, (unsigned)nchunks);
+    const int chunk = (depth + nchunks - 1) / nchunks;
+    const int threads = kMemtermThreads;
+    dim3 grid((unsigned)(h->ld / 2 / threads), (unsigned)nchunks);
     LaunchTimer timer(ctx, FRM_K_MEMTERM,
                       (double)depth * h->n_nodes * sizeof(double) +
                           (double)ntargets * h->n_nodes * sizeof(double));
     if (ntargets == 1)
         hipLaunchKernelGGL(k_memterm_f64<1>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, d_w, ldw, h->chunk, h->part);
+                           (const double*)h->data, h->ld, depth, d_w, ldw, chunk, h->part);
     else
         hipLaunchKernelGGL(k_memterm_f64<2>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, d_w, ldw, h->chunk, h->part);
+                           (const double*)h->data, h->ld, depth, d_w, ldw, chunk, h->part);
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
 }

@@ -9,16 +9,18 @@ struct frm_hist {
     int32_t capacity = 0;
     int32_t depth = 0;
     int32_t precision = FRM_PREC_F64;
-    int32_t chunk = 0;       // slots per split-K chunk
-    int32_t max_chunks = 0;  // partial-sum workspace size
+    int32_t max_chunks = 0;  // partial-sum workspace size (time chunks)
     void* data = nullptr;    // capacity * ld elements
     double* part = nullptr;  // max_chunks * 2 * ld partial sums
 };
 
 namespace frm {
 
-constexpr int64_t kLdAlign = 512;   // 256 threads x 2 nodes per thread
-constexpr int32_t kDefaultChunk = 256;
+constexpr int kMemtermThreads = 512;
+constexpr int64_t kLdAlign = 2 * kMemtermThreads;  // whole workgroups of 2 nodes per lane
+constexpr int64_t kMemtermWaves = 2048;  // wavefronts that keep HBM busy on MI355X
+constexpr int32_t kMinChunk = 64;        // shortest time chunk worth a workgroup
+constexpr int32_t kMaxChunks = 64;
 
 int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
                frm_hist** out);
