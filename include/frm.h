@@ -151,6 +151,56 @@ int frm_lod2d_tstep(const frm_lod2d* s, int32_t* tstep);
  * the field on the GPU. */
 int frm_lod2d_state_ptr(frm_lod2d* s, double** d_C);
 
+/* ------------------------- batched 1D H_solver (mt_minimal / mt_psi_caputo)
+ * A batch of independent 1D problems stepped together, each one
+ * H_solver::calc_step (mt_minimal.cpp:1032-1038, mt_psi_caputo.cpp:1435-1497)
+ * with gamma = 1: nodes i = 0..N, ghost N+1, dL = L/N,
+ *   A_(i) U_{i-1} - R(i) U_i + B(i) U_{i+1} = Om_i,  i = 1..N-1,
+ *   A_ = Dg KK(i-1)/C(i), B = Dg KK(i+1)/C(i), R = (KK(i+1)+KK(i-1)) Dg/C(i) + d0,
+ *   Om = -d0 U^n + (KK(i)-KK(i-1))/dL + Da * history   (mt_minimal.cpp:731-743, 860-940)
+ * with d0 = Da (kind L1: Da = tau^-alpha/Gamma(2-alpha)) or
+ * Da (1-alpha) kb_time[tstep] (kind PSI: Da = tau^-1/Gamma(2-alpha)).
+ * The history sum keeps the reference's index convention: the first two
+ * snapshot increments never enter (mt_minimal.cpp:917-921).  Coefficient
+ * forms: CONST (KK = D, C = 1), CV2 (KK = k, 1/C piecewise in U as cv_form 2,
+ * mt_minimal.cpp:499-503), KEXP (KK = k0 exp(b U), C = 1).  `picard` > 1
+ * re-solves the step with coefficients from the latest iterate (1 = the
+ * reference's lagged coefficients).  Problem p's nodes are packed at offset
+ * sum_{q<p} (N_q + 2) of the state array. */
+enum { FRM_MT_L1 = 0, FRM_MT_PSI = 1 };
+enum { FRM_FORM_CONST = 0, FRM_FORM_CV2 = 1, FRM_FORM_KEXP = 2 };
+enum { FRM_TOP_AUTO = 0, FRM_TOP_DIRICHLET = 1, FRM_TOP_FLUX = 2 };
+
+typedef struct frm_mt1d_problem {
+    int32_t N;           /* intervals (nodes 0..N), 2 <= N <= 1025 */
+    int32_t form;        /* FRM_FORM_* */
+    int32_t top;         /* FRM_TOP_*: AUTO is the reference's rule (al1/bt1) */
+    int32_t bottom_cond; /* 1: U_N = Hbottom; 0: flux rule of U1 (mt_minimal.cpp:987-1020) */
+    double L, alpha, beta; /* beta: psi(t) = t^beta for kind PSI */
+    double D, k, k0, b;
+    double top_val;      /* Dirichlet value for FRM_TOP_DIRICHLET */
+    double Ev;           /* top flux term of Uc() (evaporation minus inflow) */
+    double Hbottom;
+} frm_mt1d_problem;
+
+typedef struct frm_mt1d_desc {
+    int32_t nprob;
+    int32_t kind;      /* FRM_MT_L1 or FRM_MT_PSI */
+    int32_t capacity;  /* max steps */
+    int32_t precision; /* FRM_PREC_F64 or FRM_PREC_F32C */
+    int32_t picard;    /* Picard iterations per step, >= 1 */
+    double tau;
+    const frm_mt1d_problem* probs; /* host array of nprob */
+} frm_mt1d_desc;
+
+int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* desc, frm_mt1d** out);
+int frm_mt1d_destroy(frm_mt1d* s);
+/* Packed initial states (host, sum of N_p + 2 values); resets tstep to 1. */
+int frm_mt1d_set_state(frm_mt1d* s, const double* h_U);
+int frm_mt1d_step(frm_mt1d* s, int32_t nsteps);
+int frm_mt1d_get_state(frm_mt1d* s, double* h_U);
+int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
+
 #ifdef __cplusplus
 }
 #endif

@@ -533,6 +533,8 @@ struct orc_mt {
     int tstep;
     double* kbt; /* kb_time of the current step (psi) */
     int cap_kbt;
+    double* kap; /* kappa(m) cache for the L1 weights */
+    int cap_kap;
     psi_cache pc;
 };
 
@@ -593,7 +595,7 @@ void orc_mt_destroy(orc_mt* m) {
     if (!m) return;
     for (int q = 0; q < m->nold; ++q) free(m->old[q]);
     free(m->old);
-    free(m->U); free(m->Un); free(m->Al); free(m->Bt); free(m->Om); free(m->kbt);
+    free(m->U); free(m->Un); free(m->Al); free(m->Bt); free(m->Om); free(m->kbt); free(m->kap);
     psi_cache_free(&m->pc);
     free(m);
 }
@@ -601,8 +603,10 @@ void orc_mt_destroy(orc_mt* m) {
 /* Time-history sum of Om1: Da * sum_{t=1}^{os-2} w_t (old[t+1]-old[t]) with
  * w_t = kappa(os-t) (mt_minimal.cpp:906-939) or (1-alpha) kb_time[t]
  * (mt_psi_caputo.cpp:1315-1342), summed in increasing t. */
-static double mt_time_sum(int psi, double alpha, const double* kbt, double* const* old, int os,
-                          int id) {
+/* kappa cache: kap[m] = m^(1-a) - (m-1)^(1-a), as the static precalc of
+ * Om1 (mt_minimal.cpp:862-890, 926-931); NULL computes each term. */
+static double mt_time_sum(int psi, double alpha, const double* kbt, const double* kap,
+                          double* const* old, int os, int id) {
     double time_sum = 0;
     if (os >= 2) {
         double prev = old[1][id];
@@ -610,6 +614,8 @@ static double mt_time_sum(int psi, double alpha, const double* kbt, double* cons
             double kv, cur = old[1 + t][id];
             if (psi)
                 kv = kbt[t] * (1 - alpha);
+            else if (kap)
+                kv = kap[os - t];
             else
                 kv = (pow((double)(os - t), 1.0 - alpha) - pow((double)(os - t - 1), 1.0 - alpha));
             time_sum += kv * (cur - prev);
@@ -632,7 +638,7 @@ void orc_mt_history(int N, int psi, double beta, double alpha, double tau, int o
         for (int aa = 0; aa <= tstep; ++aa) kbt[aa] = psi_calc_int(&c, alpha, beta, aa, aa + 1, tstep + 1, tau);
         psi_cache_free(&c);
     }
-    for (int i = 1; i < N; ++i) out[i - 1] = Da * mt_time_sum(psi, alpha, kbt, old, os, i);
+    for (int i = 1; i < N; ++i) out[i - 1] = Da * mt_time_sum(psi, alpha, kbt, NULL, old, os, i);
     free(kbt);
     free(old);
 }
@@ -655,8 +661,17 @@ void orc_mt_step(orc_mt* m) {
     memcpy(m->Un, U, (N + 2) * sizeof(double));
     /* history part of Om does not change between Picard iterates */
     double* hist = (double*)malloc((N + 1) * sizeof(double));
+    if (!m->psi && m->cap_kap < m->nold + 1) {
+        int c0 = m->cap_kap;
+        m->cap_kap = 2 * (m->nold + 1);
+        m->kap = (double*)realloc(m->kap, m->cap_kap * sizeof(double));
+        for (int q = c0; q < m->cap_kap; ++q)
+            m->kap[q] = q >= 1 ? (pow((double)q, 1.0 - m->alpha) - pow((double)(q - 1), 1.0 - m->alpha)) : 0.0;
+    }
     if (m->alpha != 1.0)
-        for (int i = 1; i < N; ++i) hist[i] = mt_time_sum(m->psi, m->alpha, m->kbt, m->old, m->nold, i);
+#pragma omp parallel for if (N * m->nold > 200000)
+        for (int i = 1; i < N; ++i)
+            hist[i] = mt_time_sum(m->psi, m->alpha, m->kbt, m->kap, m->old, m->nold, i);
     for (int it = 0; it < m->picard; ++it) {
         const double* X = U; /* coefficient state: U^n on the first pass */
         /* al1 (mt_minimal.cpp:829-858) */

@@ -41,6 +41,22 @@ class Lod2dDesc(ct.Structure):
                 ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P)]
 
 
+class MT1DProblem(ct.Structure):
+    _fields_ = [("N", ct.c_int32), ("form", ct.c_int32), ("top", ct.c_int32),
+                ("bottom_cond", ct.c_int32), ("L", _D), ("alpha", _D), ("beta", _D), ("D", _D),
+                ("k", _D), ("k0", _D), ("b", _D), ("top_val", _D), ("Ev", _D), ("Hbottom", _D)]
+
+
+class MT1DDesc(ct.Structure):
+    _fields_ = [("nprob", ct.c_int32), ("kind", ct.c_int32), ("capacity", ct.c_int32),
+                ("precision", ct.c_int32), ("picard", ct.c_int32), ("tau", _D),
+                ("probs", ct.POINTER(MT1DProblem))]
+
+
+MT_L1, MT_PSI = 0, 1
+FORM_CONST, FORM_CV2, FORM_KEXP = 0, 1, 2
+TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
+
 # name -> (restype, argtypes); every symbol include/frm.h declares
 SIGNATURES = {
     "frm_ctx_create": (_I, [_I, ct.POINTER(_V)]),
@@ -70,6 +86,12 @@ SIGNATURES = {
     "frm_lod2d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
     "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
+    "frm_mt1d_create": (_I, [_V, ct.POINTER(MT1DDesc), ct.POINTER(_V)]),
+    "frm_mt1d_destroy": (_I, [_V]),
+    "frm_mt1d_set_state": (_I, [_V, _P]),
+    "frm_mt1d_step": (_I, [_V, ct.c_int32]),
+    "frm_mt1d_get_state": (_I, [_V, _P]),
+    "frm_mt1d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
 }
 
 _lib = None

@@ -1,0 +1,689 @@
+// Batched 1D time-fractional moisture solver: the hot path of
+// H_solver::calc_step of mt_minimal.cpp (Caputo L1 weights) and
+// mt_psi_caputo.cpp (psi-Caputo weights, psi(t) = t^beta), gamma = 1.
+//
+// The reference steps one problem at a time on the CPU: coefficient calls
+// per node (al1 829-858, bt1 942-980), the history loop of Om1 (906-939 /
+// 1315-1342) over heap snapshots, the progonka back-substitution (1021-1022)
+// and, for psi, a full rebuild of kb_time with series integrals every step
+// (mt_psi_caputo.cpp:1481-1489).  Here a whole batch of problems of uneven
+// size advances together, three launches per step:
+//   1. (psi only) weights: one lane per (problem, interval), FP64 series;
+//   2. memory term: per node, over the stored increments; the problems are
+//      padded to 128-node groups so that each wavefront belongs to one
+//      problem and reads its weights through the scalar cache;
+//   3. sweep: one wavefront per problem -- coefficients from the current
+//      iterate (held in LDS), Thomas alpha/beta/back-substitution with
+//      wavefront-scan chunk seeds, boundary rules, Picard iterations, and
+//      the push of the new increment.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "frm_history.hpp"
+#include "frm_wavescan.hpp"
+
+namespace frm {
+
+constexpr int kGroup = 128;       // nodes per memory-term wavefront
+constexpr int kMtCM = 16;         // unknowns per lane in the sweep (N - 1 <= 1024)
+constexpr int kMtMaxN = 64 * kMtCM + 1;
+
+struct MtProbDev {
+    int N, form, top, bottom_cond;
+    int64_t uoff, hoff;
+    double dL, Dg, Da, alpha, beta, D, k, k0, b, top_val, Ev, Hbottom;
+};
+
+// ---------------------------------------------------------------- physics
+// KK(i) and inv_dw_dh(i) of H_solver at state value x (mt_minimal.cpp:481-562
+// for cv_form 2 and k_form 0; CONST and KEXP are this build's forms).
+__device__ inline double mt_KK(const MtProbDev& P, double x) {
+    if (P.form == FRM_FORM_CV2) return 1.0 * P.k;
+    if (P.form == FRM_FORM_KEXP) return P.k0 * exp(P.b * x);
+    return P.D;
+}
+__device__ inline double mt_inv(const MtProbDev& P, double x) {
+    if (P.form != FRM_FORM_CV2) return 1.0;
+    const double Pr = -x;
+    double Ch = 0.0;
+    if (Pr >= 10) Ch = 0.05;
+    if ((Pr < 10) && (Pr >= 0.0)) Ch = 0.5 - Pr * (0.5 - 0.05) / 10.0;
+    return Ch != 0.0 ? 1.0 / Ch : 1e20;
+}
+
+// ---------------------------------------------------------------- memory term
+// part[c*ldh + j] = sum over slots of chunk c of w[prob(j)*ldw + depth-1-s] * Delta_s[j]
+// One wavefront = one 128-node group = one problem (gprob), so the weight
+// address is wavefront-uniform.
+typedef double mt_dv2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f64(
+    const double* __restrict__ hist, int64_t ldh, int depth, const double* __restrict__ w,
+    int ldw, const int* __restrict__ gprob, int chunk, double* __restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = t * 2;
+    if (j >= ldh) return;
+    const int p = gprob[t >> 6];
+    const int c = blockIdx.y;
+    const int s0 = c * chunk;
+    const int s1 = min(depth, s0 + chunk);
+    double a0 = 0.0, a1 = 0.0;
+    if (p >= 0) {
+        const int64_t ldv = ldh / 2;
+        const mt_dv2* q = reinterpret_cast<const mt_dv2*>(hist + (int64_t)s0 * ldh + j);
+        const double* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
+        int s = s0;
+        for (; s + 4 <= s1; s += 4) {
+            mt_dv2 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a0 = fma(wp[-u], x[u].x, a0);
+                a1 = fma(wp[-u], x[u].y, a1);
+            }
+            q += 4 * ldv;
+            wp -= 4;
+        }
+        for (; s < s1; ++s) {
+            const mt_dv2 x = __builtin_nontemporal_load(q);
+            a0 = fma(wp[0], x.x, a0);
+            a1 = fma(wp[0], x.y, a1);
+            q += ldv;
+            wp -= 1;
+        }
+    }
+    part[(int64_t)c * ldh + j] = a0;
+    part[(int64_t)c * ldh + j + 1] = a1;
+}
+
+// FP32 increments, FP32 weights, Kahan-compensated FP32 sums; the chunk's
+// compensated sum is returned in FP64.
+typedef float mt_fv2 __attribute__((ext_vector_type(2)));
+
+__device__ inline void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f32c(
+    const float* __restrict__ hist, int64_t ldh, int depth, const float* __restrict__ w, int ldw,
+    const int* __restrict__ gprob, int chunk, double* __restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = t * 2;
+    if (j >= ldh) return;
+    const int p = gprob[t >> 6];
+    const int c = blockIdx.y;
+    const int s0 = c * chunk;
+    const int s1 = min(depth, s0 + chunk);
+    float s0x = 0.f, c0x = 0.f, s1x = 0.f, c1x = 0.f;
+    if (p >= 0) {
+        const int64_t ldv = ldh / 2;
+        const mt_fv2* q = reinterpret_cast<const mt_fv2*>(hist + (int64_t)s0 * ldh + j);
+        const float* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
+        int s = s0;
+        for (; s + 4 <= s1; s += 4) {
+            mt_fv2 x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                kahan_add(s0x, c0x, wp[-u] * x[u].x);
+                kahan_add(s1x, c1x, wp[-u] * x[u].y);
+            }
+            q += 4 * ldv;
+            wp -= 4;
+        }
+        for (; s < s1; ++s) {
+            const mt_fv2 x = __builtin_nontemporal_load(q);
+            kahan_add(s0x, c0x, wp[0] * x.x);
+            kahan_add(s1x, c1x, wp[0] * x.y);
+            q += ldv;
+            wp -= 1;
+        }
+    }
+    part[(int64_t)c * ldh + j] = (double)s0x - (double)c0x;
+    part[(int64_t)c * ldh + j + 1] = (double)s1x - (double)c1x;
+}
+
+// ---------------------------------------------------------------- psi weights
+// calc_int(alpha, a, a+1, A, tau, 1) of mt_psi_caputo.cpp:720-740 for
+// g(t) = t^beta: Taylor series around g(tj) (_kb_row, 497-566) or, for
+// intervals in the first half (t1 <= A/2), Newton binomial in g(tj) times the
+// Taylor series of the inverse's derivative around g(t1) (_kb_row2,
+// 567-719); both stop at term size global_eps = 1e-5 as the reference does.
+// The reference caches the inner integrals of _kb_row2 per interval; they
+// depend only on the interval, so recomputing them gives the same values.
+constexpr double kPsiEps = 1e-5;
+constexpr int kPsiMaxIt = 100000;
+
+__device__ inline double psi_inv_der(double beta, double t, int n, double d) {
+    if (n == 0) return pow(t, 1.0 / beta);
+    if (n == 1) return ((1.0 / beta) * pow(t, (1.0 / beta) - 1)) * (d / pow(t, 1.0 / beta));
+    return d * ((1.0 / beta) - n + 1) / t;
+}
+
+__device__ double psi_kb_row(double beta, double t0, double t1, double gtj, double alpha) {
+    const double gt1 = pow(t1, beta), gt0 = pow(t0, beta);
+    double sum = 0.0, i = 0.0;
+    int niter = 0;
+    double v = psi_inv_der(beta, gtj, 1, psi_inv_der(beta, gtj, 0, 1));
+    v *= pow(gtj, 1 - alpha);
+    double v3 = pow(fabs(1.0 - gt1 / gtj), 1 - alpha);
+    double v4 = pow(fabs(1.0 - gt0 / gtj), 1 - alpha);
+    double v2 = -(v3 - v4) / (1 - alpha);
+    sum += v2 * v;
+    while (fabs(v * v2) > kPsiEps) {
+        i += 1.0;
+        v = psi_inv_der(beta, gtj, (int)(i + 1.0), v);
+        v *= -gtj / i;
+        v3 *= fabs(1.0 - gt1 / gtj);
+        v4 *= fabs(1.0 - gt0 / gtj);
+        v2 = -(v3 - v4) / (i - alpha + 1.0);
+        sum += v2 * v;
+        if (++niter > kPsiMaxIt) break;
+        if (!isfinite(sum)) break;
+    }
+    return sum;
+}
+
+__device__ double psi_kb_row2(double beta, double t0, double t1, double gtj, double alpha) {
+    const double gt0 = pow(t0, beta), gt1 = pow(t1, beta);
+    double sum = 0.0, v1 = 1.0, v2, v3, v4, v5, v6, v7, i = 0.0, m;
+    int niter = 0;
+    v2 = pow(gtj, -alpha) * gt1;
+    v6 = gt0 / gt1;
+    do {
+        if (i != 0.0) {
+            v1 *= -(-alpha - i + 1.0) / i;
+            v2 *= gt1 / gtj;
+            v6 *= gt0 / gt1;
+        }
+        if (i == 0.0) {
+            v3 = (t1 - t0) / gt1;
+        } else {
+            v3 = 0.0;
+            m = 0.0;
+            v4 = psi_inv_der(beta, gt1, 1, psi_inv_der(beta, gt1, 0, 0));
+            v5 = (1.0 / (i + 1.0)) * (1.0 - v6);
+            v3 += v4 * v5;
+            v7 = 1;
+            while (fabs(v4 * v5) > (fabs(v3) * kPsiEps)) {
+                m += 1.0;
+                v4 = psi_inv_der(beta, gt1, (int)(m + 1.0), v4);
+                v4 /= m;
+                v4 /= ((i + m + 1.0) / (gt1 * m));
+                if (m == 1.0)
+                    v7 = (1.0 / (gt1)) * (v6 * (gt0 - gt1));
+                else
+                    v7 *= (gt0 - gt1) * ((m - 1.0) / m) * ((i + m) / (gt1 * (m - 1.0)));
+                v5 = -(v5 + v7);
+                v3 += v4 * v5;
+                if (++niter > kPsiMaxIt) break;
+                if (!isfinite(v3)) break;
+            }
+        }
+        sum += v1 * v2 * v3;
+        i += 1.0;
+        if (++niter > kPsiMaxIt) break;
+        if (!isfinite(sum)) break;
+    } while (fabs(v1 * v2 * v3) > kPsiEps);
+    return sum;
+}
+
+__device__ double psi_calc_int(double alpha, double beta, int a, int A, double tau) {
+    const double gtj = pow(A * tau, beta);
+    if ((a + 1 <= 0.5 * A) && !(beta == 1))
+        return psi_kb_row2(beta, a * tau, (a + 1) * tau, gtj, alpha);
+    return psi_kb_row(beta, a * tau, (a + 1) * tau, gtj, alpha);
+}
+
+// Step k (= tstep): w[p*ldw + l] = (1-alpha) kb_time[depth-l] for l < depth
+// (depth = k-3 history terms) and d0[p] = Da (1-alpha) kb_time[k].
+// Thread x of row p handles interval a = x+1 (x < depth) or a = k (x = depth).
+__global__ void k_psi_weights(const MtProbDev* __restrict__ pr, int nprob, int k, int depth,
+                              double tau, double* __restrict__ w, int ldw, float* __restrict__ w32,
+                              double* __restrict__ d0, int* __restrict__ bad) {
+    const int p = blockIdx.y;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprob || x > depth) return;
+    const MtProbDev P = pr[p];
+    const int a = (x < depth) ? x + 1 : k;
+    const double v = psi_calc_int(P.alpha, P.beta, a, k + 1, tau);
+    if (!isfinite(v)) atomicOr(bad, 1);
+    if (x < depth) {
+        const double wl = v * (1 - P.alpha);
+        const int l = depth - a;
+        w[(int64_t)p * ldw + l] = wl;
+        if (w32) w32[(int64_t)p * ldw + l] = (float)wl;
+    } else {
+        d0[p] = P.Da * v * (1 - P.alpha);
+    }
+}
+
+// ---------------------------------------------------------------- sweep
+struct MtSweepArgs {
+    const MtProbDev* pr;
+    int nprob;
+    double* U;
+    const double* part;
+    int nch;
+    int64_t ldh;
+    const double* d0;
+    void* slot;  // history slot for the new increment, or null
+    int slot_f32;
+    int picard;
+};
+
+template <int CM>
+__global__ __launch_bounds__(256) void k_mt_sweep(MtSweepArgs a) {
+    __shared__ double xs_all[4][64 * CM + 3];  // nodes 0..N+1, N - 1 <= 64*CM
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + wv;
+    if (p >= a.nprob) return;
+    double* xs = xs_all[wv];
+    const MtProbDev P = a.pr[p];
+    const int N = P.N, n = N - 1, e0 = lane * CM;
+    double* Ug = a.U + P.uoff;
+    for (int q = lane; q < N + 2; q += 64) xs[q] = Ug[q];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double u0[CM], hs[CM];
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        u0[k] = 0.0;
+        hs[k] = 0.0;
+        if (e < n) {
+            u0[k] = xs[e + 1];
+            double h = 0.0;
+            for (int c = 0; c < a.nch; ++c) h += a.part[(int64_t)c * a.ldh + P.hoff + e];
+            hs[k] = h;
+        }
+    }
+    const double d0 = a.d0[p];
+    for (int it = 0; it < a.picard; ++it) {
+        double A[CM], B[CM], R[CM], om[CM];
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            const int e = e0 + k;
+            if (e < n) {
+                const int i = e + 1;
+                const double inv = mt_inv(P, xs[i]);
+                const double Km = mt_KK(P, xs[i - 1]), K0 = mt_KK(P, xs[i]), Kp = mt_KK(P, xs[i + 1]);
+                A[k] = P.Dg * Km * inv;
+                B[k] = P.Dg * Kp * inv;
+                R[k] = inv * (Kp + Km) * P.Dg + d0;
+                double o = -d0 * u0[k];
+                o += (K0 - Km) / P.dL;
+                o += P.Da * hs[k];
+                om[k] = o;
+            } else {
+                A[k] = 0.0;
+                B[k] = 1.0;
+                R[k] = 1.0;
+                om[k] = 0.0;
+            }
+        }
+        // top boundary: al1 / bt1 (mt_minimal.cpp:831-834, 944-947)
+        const double x0 = xs[0];
+        int first_order;
+        if (P.top == FRM_TOP_DIRICHLET) first_order = 0;
+        else if (P.top == FRM_TOP_FLUX) first_order = 1;
+        else first_order = (mt_inv(P, x0) != 1e20);
+        const double al_first = first_order ? 1.0 : 0.0;
+        const double Uc = 1.0 * P.dL * (P.Ev - 0.0) / mt_KK(P, x0);
+        double b0;
+        if (P.top == FRM_TOP_DIRICHLET) b0 = P.top_val;
+        else if (first_order) b0 = -Uc;
+        else b0 = -Uc * mt_KK(P, xs[1]) / P.dL;
+        // alpha: Al[i+1] = B/(R - A Al[i]), a Moebius map of Al
+        Mob m{1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+            if (e0 + k < n) m = mob_mul(Mob{0.0, B[k], -A[k], R[k]}, m);
+        const Mob pre = scan_mob_up(m, lane);
+        double al = (pre.m00 * al_first + pre.m01) / (pre.m10 * al_first + pre.m11);
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            if (e0 + k < n) {
+                al = B[k] / (R[k] - A[k] * al);
+                R[k] = al;  // R now holds Al[i+1]
+            }
+        }
+        // beta: Bt[i+1] = (Al[i+1]/B)(A Bt[i] - Om)
+        double Pp = 1.0, Qq = 0.0;
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            if (e0 + k < n) {
+                const double pk = R[k] * A[k] / B[k], qk = -R[k] * om[k] / B[k];
+                Qq = pk * Qq + qk;
+                Pp = pk * Pp;
+            }
+        }
+        scan_affine_up(Pp, Qq, lane);
+        double bt = Pp * b0 + Qq;
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            if (e0 + k < n) {
+                bt = (R[k] / B[k]) * (A[k] * bt - om[k]);
+                om[k] = bt;  // om now holds Bt[i+1]
+            }
+        }
+        // bottom boundary (mt_minimal.cpp:987-1020), gamma = 1 so Qb = -0
+        const int last = (n - 1) / CM;
+        double UN_l = 0.0;
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            if (e0 + k == n - 1) {
+                const double BtN = om[k], AlN = R[k], Qb = -0.0;
+                if (P.bottom_cond == 1)
+                    UN_l = P.Hbottom;
+                else if (fabs(BtN + Qb) > 1e-15)
+                    UN_l = (fabs(1.0 - AlN) > 1e-15) ? (BtN + Qb) / (1.0 - AlN) : xs[N - 1] + Qb;
+                else
+                    UN_l = (fabs(1.0 - AlN) > 1e-15) ? 0.0 : xs[N - 1] + Qb;
+            }
+        }
+        const double UN = __shfl(UN_l, last);
+        // back substitution U_i = Al[i+1] U_{i+1} + Bt[i+1]
+        Pp = 1.0;
+        Qq = 0.0;
+#pragma unroll
+        for (int k = CM - 1; k >= 0; --k) {
+            if (e0 + k < n) {
+                Qq = R[k] * Qq + om[k];
+                Pp = R[k] * Pp;
+            }
+        }
+        scan_affine_down(Pp, Qq, lane);
+        double y = Pp * UN + Qq;
+        double v[CM];
+#pragma unroll
+        for (int k = CM - 1; k >= 0; --k) {
+            v[k] = 0.0;
+            if (e0 + k < n) {
+                y = R[k] * y + om[k];
+                v[k] = y;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+            if (e0 + k < n) xs[e0 + k + 1] = v[k];
+        if (lane == 0) {
+            xs[0] = al_first * v[0] + b0;
+            xs[N] = UN;
+            xs[N + 1] = UN;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    for (int q = lane; q < N + 2; q += 64) Ug[q] = xs[q];
+    if (a.slot) {
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            const int e = e0 + k;
+            if (e < n) {
+                const double d = xs[e + 1] - u0[k];
+                if (a.slot_f32)
+                    static_cast<float*>(a.slot)[P.hoff + e] = (float)d;
+                else
+                    static_cast<double*>(a.slot)[P.hoff + e] = d;
+            }
+        }
+    }
+}
+
+}  // namespace frm
+
+using namespace frm;
+
+struct frm_mt1d {
+    frm_ctx* ctx = nullptr;
+    int nprob = 0, kind = 0, capacity = 0, precision = 0, picard = 1;
+    double tau = 0;
+    int tstep = 1;
+    int depth = 0;
+    int slots = 0;
+    int64_t nU = 0, ldh = 0, ngroups = 0;
+    int maxN = 0;
+    std::vector<MtProbDev> hprobs;
+    MtProbDev* probs = nullptr;
+    int* gprob = nullptr;
+    double* U = nullptr;
+    void* hist = nullptr;     // slots * ldh (FP64 or FP32)
+    double* part = nullptr;   // kMaxChunks * ldh
+    double* w = nullptr;      // nprob * ldw FP64 weights (lag-indexed)
+    float* w32 = nullptr;     // FP32 copy for FRM_PREC_F32C
+    double* d0 = nullptr;     // per-problem current-step coefficient
+    int* bad = nullptr;       // set when a psi weight is not finite
+    int ldw = 0;
+};
+
+static void mt_free(frm_mt1d* s) {
+    if (!s) return;
+    hipFree(s->probs);
+    hipFree(s->gprob);
+    hipFree(s->U);
+    hipFree(s->hist);
+    hipFree(s->part);
+    hipFree(s->w);
+    hipFree(s->w32);
+    hipFree(s->d0);
+    hipFree(s->bad);
+    delete s;
+}
+
+extern "C" {
+
+int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
+    FRM_REQUIRE(ctx, ctx && d && out && d->probs, "frm_mt1d_create: null argument");
+    *out = nullptr;
+    FRM_REQUIRE(ctx, d->nprob >= 1 && d->capacity >= 1 && d->picard >= 1 && d->tau > 0,
+                "frm_mt1d_create: bad batch descriptor");
+    FRM_REQUIRE(ctx, d->kind == FRM_MT_L1 || d->kind == FRM_MT_PSI, "frm_mt1d_create: kind %d",
+                d->kind);
+    FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_F32C,
+                "frm_mt1d_create: precision %d", d->precision);
+    FRM_HIP(ctx, hipSetDevice(ctx->device));
+    frm_mt1d* s = new frm_mt1d();
+    s->ctx = ctx;
+    s->nprob = d->nprob;
+    s->kind = d->kind;
+    s->capacity = d->capacity;
+    s->precision = d->precision;
+    s->picard = d->picard;
+    s->tau = d->tau;
+    s->slots = std::max(1, d->capacity - 2);  // increments enter from step 3 on
+    s->ldw = s->slots;
+    std::vector<int> gp;
+    int64_t uoff = 0, hoff = 0;
+    for (int p = 0; p < d->nprob; ++p) {
+        const frm_mt1d_problem& q = d->probs[p];
+        if (q.N < 2 || q.N > kMtMaxN || !(q.alpha > 0 && q.alpha < 1) || !(q.L > 0) ||
+            q.form < 0 || q.form > 2 || q.top < 0 || q.top > 2) {
+            mt_free(s);
+            return set_error(ctx, FRM_E_INVALID, "frm_mt1d_create: problem %d invalid (N=%d)", p,
+                             q.N);
+        }
+        MtProbDev P{};
+        P.N = q.N;
+        P.form = q.form;
+        P.top = q.top;
+        P.bottom_cond = q.bottom_cond;
+        P.uoff = uoff;
+        P.hoff = hoff;
+        P.dL = q.L / q.N;
+        P.Dg = std::pow(P.dL, -2) / 1.0;  // Gamma(2 - gamma) = Gamma(1) = 1
+        P.alpha = q.alpha;
+        P.beta = q.beta;
+        P.Da = (d->kind == FRM_MT_PSI ? std::pow(d->tau, -1) : std::pow(d->tau, -q.alpha)) /
+               std::tgamma(2.0 - q.alpha);
+        P.D = q.D;
+        P.k = q.k;
+        P.k0 = q.k0;
+        P.b = q.b;
+        P.top_val = q.top_val;
+        P.Ev = q.Ev;
+        P.Hbottom = q.Hbottom;
+        s->hprobs.push_back(P);
+        const int ng = (q.N - 1 + kGroup - 1) / kGroup;
+        for (int g = 0; g < ng; ++g) gp.push_back(p);
+        hoff += (int64_t)ng * kGroup;
+        uoff += q.N + 2;
+        s->maxN = std::max(s->maxN, q.N);
+    }
+    s->nU = uoff;
+    // whole memory-term workgroups: 64-node groups of kMemtermThreads/64 waves
+    const int64_t wg_nodes = 2 * kMemtermThreads;
+    s->ldh = (hoff + wg_nodes - 1) / wg_nodes * wg_nodes;
+    s->ngroups = s->ldh / kGroup;
+    while ((int64_t)gp.size() < s->ngroups) gp.push_back(-1);
+    const size_t esz = d->precision == FRM_PREC_F32C ? sizeof(float) : sizeof(double);
+    hipError_t e = hipMalloc(&s->probs, sizeof(MtProbDev) * d->nprob);
+    if (e == hipSuccess) e = hipMalloc(&s->gprob, sizeof(int) * gp.size());
+    if (e == hipSuccess) e = hipMalloc(&s->U, sizeof(double) * s->nU);
+    if (e == hipSuccess) e = hipMalloc(&s->hist, esz * (size_t)s->slots * s->ldh);
+    if (e == hipSuccess) e = hipMalloc(&s->part, sizeof(double) * (size_t)kMaxChunks * s->ldh);
+    if (e == hipSuccess) e = hipMalloc(&s->w, sizeof(double) * (size_t)d->nprob * s->ldw);
+    if (e == hipSuccess && d->precision == FRM_PREC_F32C)
+        e = hipMalloc(&s->w32, sizeof(float) * (size_t)d->nprob * s->ldw);
+    if (e == hipSuccess) e = hipMalloc(&s->d0, sizeof(double) * d->nprob);
+    if (e == hipSuccess) e = hipMalloc(&s->bad, sizeof(int));
+    if (e != hipSuccess) {
+        mt_free(s);
+        return check_hip(ctx, e, "hipMalloc(mt1d)");
+    }
+    // L1 weights: kappa(l+2) = (l+2)^(1-a) - (l+1)^(1-a) as the reference forms
+    // them (mt_minimal.cpp:926-931); psi weights are rebuilt every step.
+    std::vector<double> w((size_t)d->nprob * s->ldw, 0.0), d0(d->nprob);
+    std::vector<float> w32(d->precision == FRM_PREC_F32C ? w.size() : 0);
+    for (int p = 0; p < d->nprob; ++p) {
+        const double a = d->probs[p].alpha;
+        d0[p] = s->hprobs[p].Da;
+        if (d->kind == FRM_MT_L1)
+            for (int l = 0; l < s->ldw; ++l) {
+                const double kv = std::pow((double)(l + 2), 1.0 - a) - std::pow((double)(l + 1), 1.0 - a);
+                w[(size_t)p * s->ldw + l] = kv;
+                if (!w32.empty()) w32[(size_t)p * s->ldw + l] = (float)kv;
+            }
+    }
+    e = hipMemcpy(s->probs, s->hprobs.data(), sizeof(MtProbDev) * d->nprob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->gprob, gp.data(), sizeof(int) * gp.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !w32.empty())
+        e = hipMemcpy(s->w32, w32.data(), sizeof(float) * w32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d0, d0.data(), sizeof(double) * d0.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->bad, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(s->hist, 0, esz * (size_t)s->slots * s->ldh);
+    if (e != hipSuccess) {
+        mt_free(s);
+        return check_hip(ctx, e, "mt1d setup copies");
+    }
+    *out = s;
+    return FRM_OK;
+}
+
+int frm_mt1d_destroy(frm_mt1d* s) {
+    if (!s) return FRM_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    mt_free(s);
+    return FRM_OK;
+}
+
+int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
+    if (!s || !h_U) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_mt1d_set_state: null");
+    frm_ctx* ctx = s->ctx;
+    FRM_HIP(ctx, hipMemcpyAsync(s->U, h_U, sizeof(double) * s->nU, hipMemcpyHostToDevice, ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->tstep = 1;
+    s->depth = 0;
+    return FRM_OK;
+}
+
+int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
+    if (!s) return set_error(nullptr, FRM_E_INVALID, "frm_mt1d_step: null solver");
+    frm_ctx* ctx = s->ctx;
+    FRM_REQUIRE(ctx, nsteps >= 0, "frm_mt1d_step: nsteps %d", nsteps);
+    if (s->tstep - 1 + nsteps > s->capacity)
+        return set_error(ctx, FRM_E_RANGE, "frm_mt1d_step: %d steps exceed capacity %d",
+                         s->tstep - 1 + nsteps, s->capacity);
+    const bool f32 = s->precision == FRM_PREC_F32C;
+    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    for (int it = 0; it < nsteps; ++it) {
+        const int k = s->tstep;
+        const int depth = s->depth;  // = max(0, k - 3)
+        if (s->kind == FRM_MT_PSI) {
+            const int threads = 256;
+            dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
+            LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+            hipLaunchKernelGGL(k_psi_weights, grid, dim3(threads), 0, ctx->stream, s->probs,
+                               s->nprob, k, depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad);
+            FRM_HIP(ctx, hipGetLastError());
+        }
+        int nch = 0;
+        if (depth > 0) {
+            const int64_t waves = s->ldh / (2 * kWave);
+            nch = 1;
+            if (waves < kMemtermWaves) nch = (int)((kMemtermWaves + waves - 1) / waves);
+            nch = std::max(1, std::min<int>({nch, (depth + kMinChunk - 1) / kMinChunk, kMaxChunks}));
+            const int chunk = (depth + nch - 1) / nch;
+            dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
+            LaunchTimer t(ctx, FRM_K_MEMTERM, (double)depth * s->ldh * esz);
+            if (f32)
+                hipLaunchKernelGGL(k_mt_memterm_f32c, grid, dim3(kMemtermThreads), 0, ctx->stream,
+                                   (const float*)s->hist, s->ldh, depth, s->w32, s->ldw, s->gprob,
+                                   chunk, s->part);
+            else
+                hipLaunchKernelGGL(k_mt_memterm_f64, grid, dim3(kMemtermThreads), 0, ctx->stream,
+                                   (const double*)s->hist, s->ldh, depth, s->w, s->ldw, s->gprob,
+                                   chunk, s->part);
+            FRM_HIP(ctx, hipGetLastError());
+        }
+        MtSweepArgs a;
+        a.pr = s->probs;
+        a.nprob = s->nprob;
+        a.U = s->U;
+        a.part = s->part;
+        a.nch = nch;
+        a.ldh = s->ldh;
+        a.d0 = s->d0;
+        a.slot = (k >= 3) ? (void*)((char*)s->hist + esz * (size_t)depth * s->ldh) : nullptr;
+        a.slot_f32 = f32;
+        a.picard = s->picard;
+        {
+            LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
+            hipLaunchKernelGGL(k_mt_sweep<kMtCM>, dim3((unsigned)((s->nprob + 3) / 4)), dim3(256), 0,
+                               ctx->stream, a);
+            FRM_HIP(ctx, hipGetLastError());
+        }
+        if (k >= 3) s->depth = depth + 1;
+        s->tstep = k + 1;
+    }
+    return FRM_OK;
+}
+
+int frm_mt1d_get_state(frm_mt1d* s, double* h_U) {
+    if (!s || !h_U) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_mt1d_get_state: null");
+    frm_ctx* ctx = s->ctx;
+    int bad = 0;
+    FRM_HIP(ctx, hipMemcpyAsync(h_U, s->U, sizeof(double) * s->nU, hipMemcpyDeviceToHost, ctx->stream));
+    FRM_HIP(ctx, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (bad)
+        return set_error(ctx, FRM_E_STATE,
+                         "frm_mt1d: a psi weight series did not return a finite value (the "
+                         "reference falls back to adaptive quadrature there)");
+    return FRM_OK;
+}
+
+int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep) {
+    if (!s || !tstep) return set_error(nullptr, FRM_E_INVALID, "frm_mt1d_tstep: null");
+    *tstep = s->tstep;
+    return FRM_OK;
+}
+
+}  // extern "C"

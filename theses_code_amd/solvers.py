@@ -90,3 +90,62 @@ class TGSolver:
 
     def __del__(self):
         self.close()
+
+
+class MT1DBatch:
+    """A batch of H_solver problems (mt_minimal.cpp / mt_psi_caputo.cpp) stepped
+    together on the GPU.  `problems` is a list of dicts with keys of
+    frm_mt1d_problem (N, L, alpha, form, ...); the state is the list of the
+    problems' U arrays (N+2 values each, nodes 0..N and the ghost N+1)."""
+
+    FIELDS = ("N", "form", "top", "bottom_cond", "L", "alpha", "beta", "D", "k", "k0", "b",
+              "top_val", "Ev", "Hbottom")
+
+    def __init__(self, ctx: N.Context, problems, tau: float, capacity: int, U0,
+                 kind: int = N.MT_L1, precision: int = N.PREC_F64, picard: int = 1):
+        self.ctx = ctx
+        defaults = dict(form=N.FORM_CONST, top=N.TOP_AUTO, bottom_cond=1, L=1.0, beta=1.0, D=0.0,
+                        k=0.0, k0=0.0, b=0.0, top_val=0.0, Ev=0.0, Hbottom=0.0)
+        arr = (N.MT1DProblem * len(problems))()
+        for i, pr in enumerate(problems):
+            q = {**defaults, **pr}
+            for f in self.FIELDS:
+                setattr(arr[i], f, q[f])
+        self.sizes = [int(pr["N"]) + 2 for pr in problems]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(int)
+        self._arr = arr
+        d = N.MT1DDesc(len(problems), kind, capacity, precision, picard, tau, arr)
+        h = N._V()
+        N._check(N.lib().frm_mt1d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
+        self.h = h
+        self.set_state(U0)
+
+    def _pack(self, Us):
+        flat = np.concatenate([np.asarray(u, dtype=np.float64).reshape(-1) for u in Us])
+        assert flat.size == self.offsets[-1]
+        return np.ascontiguousarray(flat)
+
+    def set_state(self, Us):
+        N._check(N.lib().frm_mt1d_set_state(self.h, N._dp(self._pack(Us))), self.ctx.h)
+
+    def calc_step(self, nsteps: int = 1):
+        N._check(N.lib().frm_mt1d_step(self.h, nsteps), self.ctx.h)
+
+    @property
+    def tstep(self) -> int:
+        t = ct.c_int32()
+        N._check(N.lib().frm_mt1d_tstep(self.h, ct.byref(t)), self.ctx.h)
+        return t.value
+
+    def state(self):
+        out = np.empty(self.offsets[-1])
+        N._check(N.lib().frm_mt1d_get_state(self.h, N._dp(out)), self.ctx.h)
+        return [out[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.sizes))]
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().frm_mt1d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -53,3 +53,21 @@ def lod2d_case(n_interior: int, seed: int, vmax: float = 0.1, D: float = 1e-3,
 # the 2D Caputo convection-diffusion workload quoted by the headline metric
 C1_WORKLOAD = dict(name="lod2d_caputo_512x512_fp64", n_interior=512, alpha=0.7, tau=1e-3,
                    steps=5000, sigma=1.0, D=1e-3, vmax=0.1, seed=SEED_BASE + 1)
+
+
+# 1D Caputo diffusion in the mt_minimal form: N = 200 intervals on [0,1],
+# Dirichlet 0 at both ends, u0 = sin(pi x) + 0.01 N(0,1) zeroed at the ends
+C0_WORKLOAD = dict(name="mt1d_caputo_200_fp64", N=200, alpha=0.8, tau=1e-3, steps=2000, D=1e-3,
+                   seed=SEED_BASE + 0)
+
+
+def mt1d_c0_case(seed: int = C0_WORKLOAD["seed"], N: int = 200):
+    rng = np.random.default_rng(seed)
+    x = np.arange(N + 2) / N
+    u0 = np.sin(np.pi * x) + 0.01 * rng.normal(size=N + 2)
+    u0[0] = 0.0
+    u0[N] = 0.0
+    u0[N + 1] = 0.0
+    prob = dict(N=N, L=1.0, alpha=C0_WORKLOAD["alpha"], form=0, D=C0_WORKLOAD["D"], top=1,
+                top_val=0.0, bottom_cond=1, Hbottom=0.0)
+    return prob, u0
