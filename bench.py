@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--parity-steps", type=int, default=12)
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
+    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c3"],
+                    help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
+                         "c3: 4096 psi-Caputo problems, FP64 vs compensated FP32")
+    ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
     return ap.parse_args()
 
 
@@ -72,12 +76,73 @@ def cpu_baseline(case, cfg, depth, nsteps):
     }
 
 
+def run_mt1d(args, cfg_name):
+    """c0 / c3: the batched 1D H_solver; c3 times the FP64 and the compensated
+    FP32 forms on the same inputs and reports the FP32 form's error."""
+    from theses_code_amd import _native as N
+    from theses_code_amd.solvers import MT1DBatch
+    from theses_code_amd import workloads as Wl
+
+    if cfg_name == "c0":
+        cfg = Wl.C0_WORKLOAD
+        prob, u0 = Wl.mt1d_c0_case()
+        probs, U0, kind, picard = [prob], [u0], N.MT_L1, 1
+    else:
+        cfg = Wl.C3_WORKLOAD
+        probs, U0 = Wl.mt1d_c3_case(nprob=args.nprob or cfg["nprob"])
+        kind, picard = N.MT_PSI, cfg["picard"]
+    K = args.steps if args.steps != 5000 else cfg["steps"]
+    nodes = sum(p["N"] + 1 for p in probs)  # nodes 0..N of every problem
+    ctx = N.Context(0)
+    results = {}
+    for prec in ([N.PREC_F64, N.PREC_F32C] if cfg_name == "c3" else [N.PREC_F64]):
+        b = MT1DBatch(ctx, probs, cfg["tau"], K, U0, kind=kind, precision=prec, picard=picard)
+        ctx.timing_reset()
+        ctx.timing_enable((1 << N.K_MEMTERM) | (1 << N.K_OTHER) | (1 << N.K_SWEEP1D))
+        ctx.sync()
+        t0 = time.perf_counter()
+        b.calc_step(K)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        ctx.timing_enable(0)
+        kt = {name: ctx.timing(k) for name, k in (("memterm", N.K_MEMTERM), ("weights", N.K_OTHER),
+                                                   ("sweep", N.K_SWEEP1D))}
+        results[prec] = (dt, np.concatenate(b.state()), kt)
+        b.close()
+    dt, U64, kt = results[N.PREC_F64]
+    line = {"metric": METRIC, "value": nodes * K / dt, "unit": "node-steps/s", "n_gpus": 1,
+            "steps": K, "warmup": 0, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": cfg["name"], "problems": len(probs), "nodes": nodes,
+                       "tau": cfg["tau"], "picard": picard, "parallelism": "1 GPU"},
+            "kernels_ms": {k: v[1] for k, v in kt.items()}}
+    ms, nb = kt["memterm"][1], kt["memterm"][2]
+    line["roofline"] = {"bound": "hbm", "achieved": nb / (ms * 1e-3) / 1e9 if ms else 0.0,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
+                        "traffic": None, "kernel": "k_mt_memterm_f64"}
+    if N.PREC_F32C in results:
+        dt32, U32, kt32 = results[N.PREC_F32C]
+        d = U32 - U64
+        line["chosen_form"] = {
+            "form": "FP32 history, compensated FP32 sums, FP64 weights/Thomas/state",
+            "value": nodes * K / dt32, "speedup_vs_fp64": dt / dt32,
+            "max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(U64))),
+            "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(U64 * U64))),
+            "kernels_ms": {k: v[1] for k, v in kt32.items()}}
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the history sums shard by node with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.config != "c1":
+        run_mt1d(args, args.config)
+        return
 
     from theses_code_amd import _native as N
     from theses_code_amd.solvers import TGSolver

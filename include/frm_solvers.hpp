@@ -1,0 +1,105 @@
+// frm_solvers.hpp -- C++ solver classes over the C ABI (header-only).
+//
+// They keep the calc_step() interface and the public state names of the
+// reference's classes, so a caller written against TG_solver
+// (fr_d_t_g_2d.cpp:2037-2771) or H_solver (mt_minimal.cpp:276-1048,
+// mt_psi_caputo.cpp:369-1507) can switch to the GPU path:
+//   frm::TG_solver s(desc, C0);  s.calc_step();  s.fetch();  s.b_C[idx]
+// Errors from the ABI are thrown as frm::error (the reference has no status
+// codes; its callers test results with finite()).
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "frm.h"
+
+namespace frm {
+
+struct error : std::runtime_error {
+    int status;
+    error(int st, const std::string& what) : std::runtime_error(what), status(st) {}
+};
+
+inline void check(int st, const frm_ctx* ctx) {
+    if (st != FRM_OK) throw error(st, std::string(frm_strerror(st)) + ": " + frm_last_error(ctx));
+}
+
+// One device + one stream.
+class Context {
+   public:
+    explicit Context(int device = 0) { check(frm_ctx_create(device, &ctx_), nullptr); }
+    ~Context() { frm_ctx_destroy(ctx_); }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    frm_ctx* get() const { return ctx_; }
+    void sync() { check(frm_sync(ctx_), ctx_); }
+
+   private:
+    frm_ctx* ctx_ = nullptr;
+};
+
+// TG_solver (2D LOD, g(t) = t, full history) on the GPU.  b_C mirrors the
+// reference's field (sN+2)*(M+2), idx(i,j) = i + j*(sN+2); it is refreshed
+// from the device by fetch() (the reference's fr_ocl_get, 4386-4392).
+class TG_solver {
+   public:
+    TG_solver(Context& ctx, const frm_lod2d_desc& d, const double* C0)
+        : ctx_(ctx), sN(d.sN), M(d.M), b_C((size_t)(d.sN + 2) * (d.M + 2)) {
+        check(frm_lod2d_create(ctx.get(), &d, &s_), ctx.get());
+        check(frm_lod2d_set_state(s_, C0), ctx.get());
+        b_C.assign(C0, C0 + b_C.size());
+    }
+    ~TG_solver() { frm_lod2d_destroy(s_); }
+    TG_solver(const TG_solver&) = delete;
+    TG_solver& operator=(const TG_solver&) = delete;
+
+    void calc_step() { check(frm_lod2d_step(s_, 1), ctx_.get()); ++tstep; }
+    void calc_steps(int n) { check(frm_lod2d_step(s_, n), ctx_.get()); tstep += n; }
+    void fetch() { check(frm_lod2d_get_state(s_, b_C.data()), ctx_.get()); }
+
+    const int sN, M;
+    int tstep = 0;
+    std::vector<double> b_C;
+
+   private:
+    Context& ctx_;
+    frm_lod2d* s_ = nullptr;
+};
+
+// A batch of H_solver problems; b_U[p] mirrors problem p's U[0..N+1].
+class H_batch {
+   public:
+    H_batch(Context& ctx, const frm_mt1d_desc& d, const std::vector<std::vector<double>>& U0)
+        : ctx_(ctx), b_U(U0) {
+        check(frm_mt1d_create(ctx.get(), &d, &s_), ctx.get());
+        std::vector<double> packed;
+        for (const auto& u : U0) packed.insert(packed.end(), u.begin(), u.end());
+        check(frm_mt1d_set_state(s_, packed.data()), ctx.get());
+    }
+    ~H_batch() { frm_mt1d_destroy(s_); }
+    H_batch(const H_batch&) = delete;
+    H_batch& operator=(const H_batch&) = delete;
+
+    void calc_step() { check(frm_mt1d_step(s_, 1), ctx_.get()); ++tstep; }
+    void fetch() {
+        size_t n = 0;
+        for (const auto& u : b_U) n += u.size();
+        std::vector<double> packed(n);
+        check(frm_mt1d_get_state(s_, packed.data()), ctx_.get());
+        size_t o = 0;
+        for (auto& u : b_U) {
+            std::copy(packed.begin() + o, packed.begin() + o + u.size(), u.begin());
+            o += u.size();
+        }
+    }
+
+    int tstep = 1;  // the reference's H_solver starts at tstep = 1
+    std::vector<std::vector<double>> b_U;
+
+   private:
+    Context& ctx_;
+    frm_mt1d* s_ = nullptr;
+};
+
+}  // namespace frm

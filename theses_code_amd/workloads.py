@@ -71,3 +71,28 @@ def mt1d_c0_case(seed: int = C0_WORKLOAD["seed"], N: int = 200):
     prob = dict(N=N, L=1.0, alpha=C0_WORKLOAD["alpha"], form=0, D=C0_WORKLOAD["D"], top=1,
                 top_val=0.0, bottom_cond=1, Hbottom=0.0)
     return prob, u0
+
+
+# 4096 independent psi-Caputo moisture problems (mt_psi_caputo form) with
+# nonlinear diffusivity k(u) = k0 exp(b u); per-problem parameters drawn
+# from seeded sub-streams.  Domain [0,1], wet Dirichlet top u = 0.4, flux
+# (zero-gradient) bottom.
+C3_WORKLOAD = dict(name="mt1d_psi_batch4096_kexp", nprob=4096, Nmin=64, Nmax=1024, tau=1e-3,
+                   steps=3000, picard=3, seed=SEED_BASE + 3)
+
+
+def mt1d_c3_case(nprob: int = 4096, seed: int = C3_WORKLOAD["seed"], Nmin: int = 64,
+                 Nmax: int = 1024):
+    root = np.random.SeedSequence(seed)
+    probs, U0 = [], []
+    for child in root.spawn(nprob):
+        r = np.random.default_rng(child)
+        n = int(r.integers(Nmin, Nmax + 1))
+        probs.append(dict(N=n, L=1.0, alpha=float(r.uniform(0.5, 0.99)),
+                          beta=float(r.uniform(0.5, 1.5)), form=2, k0=float(r.uniform(1e-4, 1e-2)),
+                          b=float(r.uniform(0.0, 3.0)), top=1, top_val=0.4, bottom_cond=0))
+        u = r.uniform(0.1, 0.4, n + 2)
+        u[0] = 0.4
+        u[n + 1] = u[n]
+        U0.append(u)
+    return probs, U0
