@@ -735,3 +735,365 @@ void orc_mt_step(orc_mt* m) {
 
 void orc_mt_get(const orc_mt* m, double* U) { memcpy(U, m->U, (m->N + 2) * sizeof(double)); }
 double orc_mt_da(const orc_mt* m) { return m->Da; }
+
+/* ===================================================================== */
+/* Atangana-Baleanu path (MF_int_series.cpp)                              */
+/* ===================================================================== */
+#define MF_PI 3.14159265358979323846 /* MF_int_series.cpp:239 */
+static double mf_sign(double x) { return (x > 0.0) ? 1.0 : -1.0; } /* :331 */
+
+static int cmp_absdesc(const void* pa, const void* pb) {
+    double a = fabs(*(const double*)pa), b = fabs(*(const double*)pb);
+    return (a > b) ? -1 : (a < b) ? 1 : 0;
+}
+
+/* Sort by |value| descending, then add neighbours level by level; an odd
+ * element is folded into the last new sum (sum_sort_pyramid,
+ * MF_int_series.cpp:200-218, including its single-element doubling). */
+static double mf_pyramid(const double* in, int n) {
+    if (n == 0) return 0.0;
+    int cap = 2 * n + 4, size = n;
+    double* v = (double*)malloc(cap * sizeof(double));
+    memcpy(v, in, n * sizeof(double));
+    qsort(v, n, sizeof(double), cmp_absdesc);
+    int old_s = 0, old_e = 0;
+    do {
+        old_s = old_e;
+        old_e = size;
+        for (int i = old_s; i < (old_e - (old_e - old_s) % 2); i += 2) v[size++] = v[i] + v[i + 1];
+        if ((old_e - old_s) % 2) v[size - 1] += v[old_e - 1];
+    } while (size - old_e > 1);
+    double r = v[size - 1];
+    free(v);
+    return r;
+}
+
+typedef double (*mf_fn)(double a, double b, double z, double r);
+
+/* Trapezoid-plus-interior rule refined by doubling (integrate, :220-238). */
+static double mf_integrate(mf_fn func, double a, double b, double z, double r0, double r1,
+                           double eps, int sd, double prev) {
+    for (;;) {
+        double v1 = 0.5 * (r1 - r0) * (func(a, b, z, r0) + func(a, b, z, r1));
+        double v = 0.0;
+        for (int i = 1; i <= sd; i++) v += func(a, b, z, r0 + (i / ((double)(sd + 1))) * (r1 - r0));
+        v *= (r1 - r0);
+        if (isfinite(v1)) v += v1;
+        v /= (double)(sd + 1);
+        if (sd < 1000) {
+            if (sd == 1) prev = v1;
+            if (fabs(v - prev) < eps) return v;
+            sd = (int)(sd * 2.0);
+            prev = v;
+            continue;
+        }
+        return v;
+    }
+}
+
+static double mf_K(double a, double b, double z, double r) { /* :240-246 */
+    double k1 = (1.0 / (MF_PI * a)) * pow(r, (1 - b) / a) * exp(-pow(r, 1.0 / a));
+    double k2 = r * sin(MF_PI * (1 - b)) - z * sin(MF_PI * (1 - b + a));
+    double k3 = r * r - 2 * r * z * cos(MF_PI * a) + z * z;
+    return k1 * k2 / k3;
+}
+
+/* E_{a,b}(z): power series for z > 0, the integral representation for
+ * |z| > pi a (0 < a < 1, b < 1 + a), else paired terms summed by
+ * mf_pyramid (mittag_leffler, MF_int_series.cpp:247-302).  Pinned: golden
+ * table, E_1(z) = e^z, E_2(z) = cosh(sqrt z). */
+double orc_mittag_leffler(double a, double b, double z, double eps) {
+    int k0 = (int)(log(eps * (1 - z)) / log(z));
+    int kmin = 1 + (int)((1 - b) / a);
+    if ((z >= 1.0) || (z < 0.0)) k0 = (int)(-2.0 * log(eps));
+    if (k0 < kmin) k0 = kmin;
+    double res = 0.0, zz = 1.0;
+    if (z > 0) {
+        for (int i = 0; i < k0; i++) {
+            double v = zz / orc_gamma(b + a * i);
+            if (isfinite(v))
+                res += v;
+            else
+                break;
+            zz *= z;
+        }
+    } else {
+        double v1, v2;
+        if (((fabs(z) > MF_PI * a) && (b < (1 + a))) && ((a > 0) && (a < 1))) {
+            double r1 = 1.0;
+            v1 = 2.0 * fabs(z);
+            if (v1 > r1) r1 = v1;
+            v2 = pow(-log(MF_PI * eps / 6.0), a);
+            if (v2 > r1) r1 = v2;
+            res = mf_integrate(mf_K, a, b, z, 0.0, r1, eps, 1, 0.0);
+        } else {
+            zz = 1.0;
+            double* vals = (double*)malloc(sizeof(double) * (k0 / 2 + 2));
+            int nv = 0;
+            for (int i = 0; i < k0; i += 2) {
+                v1 = zz / orc_gamma(b + a * i);
+                zz *= z;
+                v2 = zz / orc_gamma(b + a * (i + 1));
+                zz *= z;
+                if (isfinite(v1 + v2))
+                    vals[nv++] = v1 + v2;
+                else
+                    break;
+            }
+            res = mf_pyramid(vals, nv);
+            free(vals);
+        }
+    }
+    return res;
+}
+
+/* Caches of MF_int_ksum (:332-435): log i, log Gamma(b k + 1), and the
+ * per-(k, l) log-magnitude and sign of the falling product; all values are
+ * functions of their keys, so the cache only saves time. */
+typedef struct {
+    double a, b;
+    int maxl, maxkiter;
+    double* ll;   /* [maxl] */
+    double* lg;   /* [maxkiter], < 0 = not yet */
+    double* lgs;  /* [maxkiter][maxl] */
+    double* sgn;  /* [maxkiter][maxl], 2 = not yet */
+} mf_cache;
+
+static void mf_cache_setup(mf_cache* c, double a, double b, int maxl, int maxkiter) {
+    if (c->ll && c->a == a && c->b == b && c->maxl == maxl && c->maxkiter == maxkiter) return;
+    free(c->ll); free(c->lg); free(c->lgs); free(c->sgn);
+    c->a = a; c->b = b; c->maxl = maxl; c->maxkiter = maxkiter;
+    c->ll = (double*)malloc(sizeof(double) * maxl);
+    c->ll[0] = 0.0;
+    for (int i = 1; i < maxl; i++) c->ll[i] = log((double)i);
+    c->lg = (double*)malloc(sizeof(double) * maxkiter);
+    for (int k = 0; k < maxkiter; k++) c->lg[k] = -1.0;
+    c->lgs = (double*)malloc(sizeof(double) * (size_t)maxkiter * maxl);
+    c->sgn = (double*)malloc(sizeof(double) * (size_t)maxkiter * maxl);
+    for (size_t q = 0; q < (size_t)maxkiter * maxl; q++) { c->lgs[q] = 0.0; c->sgn[q] = 2.0; }
+}
+
+static void mf_cache_free(mf_cache* c) {
+    free(c->ll); free(c->lg); free(c->lgs); free(c->sgn);
+    memset(c, 0, sizeof(*c));
+}
+
+/* l-th coefficient of the series of the AB integral (MF_int_ksum, :332-435) */
+static double mf_ksum(mf_cache* c, int l, double tj, double a, double b, int maxkiter, double eps) {
+    double k, kterm, ret = 0.0;
+    int zero_in_row = 0;
+    double lfact = orc_log_gamma(l + 1.0);
+    double la = log(a / (1.0 - a));
+    double ltj = log(tj);
+    double leps = log(eps);
+    double k1, k2, k3, k4, s4;
+    double s = pow(-1.0, l);
+    for (k = 0; k < maxkiter; k++) {
+        int ki = (int)k;
+        k1 = k * la;
+        if (c->lg[ki] < 0) {
+            k2 = orc_log_gamma(b * k + 1.0);
+            c->lg[ki] = k2;
+        } else
+            k2 = c->lg[ki];
+        k3 = (a * k - l) * ltj;
+        kterm = k1 + k3 - k2;
+        k4 = 0;
+        s4 = 1;
+        size_t q = (size_t)ki * c->maxl + l;
+        if (c->sgn[q] == 2) {
+            for (int i = 1; i <= l; i++) {
+                k4 += log(fabs(a * k - i + 1)) - c->ll[i];
+                s4 *= mf_sign(a * k - i + 1);
+            }
+            c->lgs[q] = k4;
+            c->sgn[q] = s4;
+        } else {
+            k4 = c->lgs[q];
+            s4 = c->sgn[q];
+        }
+        kterm += k4;
+        kterm = exp(kterm);
+        kterm *= s * s4 / (l + 1.0);
+        if (log(fabs(kterm)) < leps - lfact)
+            zero_in_row++;
+        else
+            zero_in_row = 0;
+        if (zero_in_row == 10) break;
+        if (!isfinite(kterm)) break;
+        ret += kterm;
+        s *= -1.0;
+    }
+    return ret;
+}
+
+/* integral of E_a(-a/(1-a)(tj-s)^a) over [ts, ts1], series form (MF_int, :436-455) */
+static double mf_int_series(mf_cache* c, double tj, double ts1, double ts, double a, double b,
+                            int maxl, int maxkiter, double eps) {
+    mf_cache_setup(c, a, b, maxl, maxkiter);
+    double ret = 0.0;
+    for (int l = 0; l < maxl; l++) {
+        double lk = pow(ts1, (l + 1.0)) - pow(ts, (l + 1.0));
+        double ksum = mf_ksum(c, l, tj, a, b, maxkiter, eps);
+        double lterm = lk * ksum;
+        if (isfinite(lterm)) ret += lterm;
+    }
+    return ret;
+}
+
+static double mf_Fv(double a, double v) { /* :457-464 */
+    double p = -a / (1.0 - a);
+    double k1 = 1.0 / (MF_PI * a);
+    double k2 = p * sin(MF_PI * a);
+    double k3 = v * v - 2 * v * p * cos(MF_PI * a) + p * p;
+    return k1 * k2 / k3;
+}
+
+/* the same integral through the integral representation of the ML function,
+ * a quadrature over maxl nodes in three segments (MF_intFv, :469-521) */
+static double mf_int_fv(double tj, double ts1, double ts, double a, int maxl, double eps) {
+    double ret = 0.0, lterm, C = 1.0, Fm, Fd, wk, vk, wk1, wk2, wk3, mC1, mC2;
+    do {
+        double ks = ts * pow(C, 1.0 / a);
+        double ks1 = ts1 * pow(C, 1.0 / a);
+        double k2 = log(fabs(mf_Fv(a, C))) - (1.0 / a) * log(C) - pow(C, 1.0 / a) * tj;
+        double v = fabs(exp(ks + k2) - exp(ks1 + k2));
+        if (v > eps)
+            C *= 1.5;
+        else
+            break;
+    } while (1);
+    Fm = -a * cos(MF_PI * a) / (1.0 - a);
+    Fd = 5 * (sqrt(3.0) / 3) * sqrt((a * a / ((1.0 - a) * (1.0 - a))) - Fm * Fm);
+    mC1 = Fm + Fd;
+    mC2 = Fm - Fd;
+    if (mC2 < 0.0) mC2 = 0.0;
+    if (mC1 < 0.0) mC1 = 0.0;
+    if (mC2 == 0) mC2 = mC1 / 2.0;
+    if (mC1 == 0) { mC1 = C / 3.0; mC2 = 2.0 * C / 3.0; }
+    wk1 = mC2 / (maxl / 3);
+    wk2 = (mC1 - mC2) / (maxl / 3);
+    wk3 = 2.0 * (C - mC2) / (maxl / 3);
+    for (int l = 0; l < maxl; l++) {
+        if (l < (maxl / 3)) { wk = wk1; vk = l * wk + 0.5 * wk; }
+        else if (l < 2 * (maxl / 3)) { wk = wk2; vk = mC2 + (l - (maxl / 3)) * wk + 0.5 * wk; }
+        else { wk = wk2 + (wk3 - wk2) * (double)(l - 2 * (maxl / 3)) / (maxl / 3); vk = mC1 + (l - 2 * (maxl / 3)) * wk + 0.5 * wk; }
+        double ks = ts * pow(vk, 1.0 / a);
+        double ks1 = ts1 * pow(vk, 1.0 / a);
+        double k2 = log(fabs(mf_Fv(a, vk))) - (1.0 / a) * log(vk) - pow(vk, 1.0 / a) * tj;
+        lterm = wk * (exp(ks + k2) - exp(ks1 + k2));
+        if (mf_Fv(a, vk) < 0) lterm = -lterm;
+        if (!isfinite(lterm)) break;
+        ret += lterm;
+    }
+    return ret;
+}
+
+/* series when (a/(1-a)) tj^a < 1, else the integral form (MF_int_choice, :523-530) */
+static double mf_int_choice(mf_cache* c, double tj, double ts1, double ts, double a, double b,
+                            int maxl, int maxkiter, double eps) {
+    double maxz = (a / (1 - a)) * pow(tj, a);
+    if ((maxz < 1.0) || (b != a)) return mf_int_series(c, tj, ts1, ts, a, b, maxl, maxkiter, eps);
+    return mf_int_fv(tj, ts1, ts, a, maxl, 1e-10);
+}
+
+void orc_ab_weights(double a, double tau, int maxl, int maxkiter, double eps, int D, double* out) {
+    mf_cache c;
+    memset(&c, 0, sizeof(c));
+    for (int d = 1; d <= D; ++d) out[d - 1] = mf_int_choice(&c, d * tau, tau, 0, a, a, maxl, maxkiter, eps);
+    mf_cache_free(&c);
+}
+
+static double mf_f(int F, double x, double t, double a) { /* :860-866 */
+    if (F == 0) return (24.0 / (1.0 - a)) * t * t * t * t * orc_mittag_leffler(a, 5.0, (-a / (1.0 - a)) * pow(t, a), 1e-20);
+    if (F == 1) return (2.0 * x - 1.0) * t * t - 2 * t * t + (2.0 / (1.0 - a)) * x * (x - 1.0) * t * t * orc_mittag_leffler(a, 3.0, (-a / (1.0 - a)) * pow(t, a), 1e-20);
+    if (F == 2) return (120.0 / (1.0 - a)) * t * t * t * t * t * sin(MF_PI * x) * orc_mittag_leffler(a, 6.0, (-a / (1.0 - a)) * pow(t, a), 1e-20) + MF_PI * t * t * t * t * t * (MF_PI * sin(MF_PI * x) + cos(MF_PI * x));
+    return 0.0;
+}
+static double mf_CC(int F, double x, double t) { /* :867-873 */
+    if (F == 0) return t * t * t * t;
+    if (F == 1) return x * (x - 1.0) * t * t;
+    if (F == 2) return t * t * t * t * t * sin(MF_PI * x);
+    return 0.0;
+}
+
+/* solve_and_check with simple = 1 (MF_int_series.cpp:875-958): full
+ * history of (C^{k+1}-C^k)/tau weighted by I(i-k) (MF_intF_simple,
+ * :614-718, I cached by lag on first use, i.e. I(d) = MF_int_choice(d tau,
+ * tau, 0)), current step Ii/(1-a) without 1/tau, Thomas with exact
+ * Dirichlet ends. Returns the mean squared error against CC. */
+double orc_ab_solve(double a, double T, int n, int m, int maxl, int maxkiter, double eps,
+                    int F, double* C_out) {
+    double h = 1.0 / m, tau = T / n;
+    double A = -(1.0 / (h * h)) - (1.0 / (2.0 * h));
+    double Cc = -(1.0 / (h * h)) + (1.0 / (2.0 * h));
+    double err = 0;
+    mf_cache c;
+    memset(&c, 0, sizeof(c));
+    double* C = (double*)calloc(m + 1, sizeof(double));
+    double* prevC = (double*)calloc(m + 1, sizeof(double));
+    double* Rp = (double*)calloc(m + 1, sizeof(double));
+    double* al = (double*)calloc(m + 1, sizeof(double));
+    double* bt = (double*)calloc(m + 1, sizeof(double));
+    double* I = (double*)calloc(n + 2, sizeof(double));       /* I(d), d = 2..n */
+    double* fv = (double*)calloc((size_t)(m + 1) * (n + 1), sizeof(double)); /* [j][k] */
+    double Ii = mf_int_choice(&c, tau, tau, 0, a, a, maxl, maxkiter, eps);
+    double B = (Ii / (1.0 - a)) + (2.0 / (h * h));
+    for (int i = 1; i <= n; i++) {
+        if (i >= 2) I[i] = mf_int_choice(&c, i * tau, 0 * tau + tau, 0 * tau, a, a, maxl, maxkiter, 1e-10);
+        for (int j = 0; j <= m; j++) {
+            Rp[j] = (Ii / (1.0 - a)) * C[j] + mf_f(F, j * h, i * tau, a);
+            if (i >= 2) {
+                double* fj = fv + (size_t)j * (n + 1);
+                fj[i - 2] = (C[j] - prevC[j]) / tau;
+                double ret = 0.0;
+                for (int tt = 0; tt < i - 1; tt++) ret += fj[tt] * I[i - tt];
+                Rp[j] -= ret / (1.0 - a);
+            }
+        }
+        for (int j = 0; j <= m; j++) prevC[j] = C[j];
+        al[0] = 0;
+        bt[0] = mf_CC(F, 0, i * tau);
+        for (int j = 1; j <= m; j++) {
+            al[j] = Cc / (B - A * al[j - 1]);
+            bt[j] = (Rp[j] - A * bt[j - 1]) / (B - A * al[j - 1]);
+        }
+        C[m] = mf_CC(F, m * h, i * tau);
+        for (int j = m - 1; j >= 0; j--) C[j] = bt[j] - al[j] * C[j + 1];
+        for (int j = 0; j <= m; j++) {
+            double d = fabs(C[j] - mf_CC(F, h * j, i * tau));
+            err += d * d;
+        }
+    }
+    if (C_out) memcpy(C_out, C, (m + 1) * sizeof(double));
+    free(C); free(prevC); free(Rp); free(al); free(bt); free(I); free(fv);
+    mf_cache_free(&c);
+    return err / ((m + 1.0) * n);
+}
+
+/* sum_{tt=0}^{n-2} Fvs[tt] I(n-tt) / (1-a) for n = 2..nmax, the per-node
+ * history of MF_intF_simple as solve_and_check calls it */
+void orc_ab_history(double a, double tau, int maxl, int maxkiter, double eps, int nmax,
+                    const double* fvs, double* out) {
+    mf_cache c;
+    memset(&c, 0, sizeof(c));
+    double* I = (double*)calloc(nmax + 2, sizeof(double));
+    (void)eps;
+    for (int n = 2; n <= nmax; ++n) {
+        I[n] = mf_int_choice(&c, n * tau, tau, 0, a, a, maxl, maxkiter, 1e-10);
+        double ret = 0.0;
+        for (int tt = 0; tt < n - 1; tt++) ret += fvs[tt] * I[n - tt];
+        out[n - 2] = ret / (1.0 - a);
+    }
+    free(I);
+    mf_cache_free(&c);
+}
+
+/* general AB integral (MF_int_choice) for the LOD weights at any lag */
+double orc_ab_int(double tj, double ts1, double ts, double a, int maxl, int maxkiter, double eps) {
+    mf_cache c;
+    memset(&c, 0, sizeof(c));
+    double v = mf_int_choice(&c, tj, ts1, ts, a, a, maxl, maxkiter, eps);
+    mf_cache_free(&c);
+    return v;
+}

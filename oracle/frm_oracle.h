@@ -87,6 +87,11 @@ void orc_ab_weights(double a, double tau, int maxl, int maxkiter, double eps, in
  * returns the mean squared error; if C_out != NULL writes the final C[0..m]. */
 double orc_ab_solve(double a, double T, int n, int m, int maxl, int maxkiter, double eps,
                     int F_func, double* C_out);
+/* per-node history of MF_intF_simple (mn = 1) for n = 2..nmax: out[n-2] */
+void orc_ab_history(double a, double tau, int maxl, int maxkiter, double eps, int nmax,
+                    const double* fvs, double* out);
+/* MF_int_choice(tj, ts1, ts, a, a, maxl, maxkiter, eps) */
+double orc_ab_int(double tj, double ts1, double ts, double a, int maxl, int maxkiter, double eps);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,9 @@ def lib():
             L.orc_ab_weights.argtypes = [_D, _D, _I, _I, _D, _I, _P]
             L.orc_ab_solve.restype = _D
             L.orc_ab_solve.argtypes = [_D, _D, _I, _I, _I, _I, _D, _I, _P]
+            L.orc_ab_history.argtypes = [_D, _D, _I, _I, _D, _I, _P, _P]
+            L.orc_ab_int.restype = _D
+            L.orc_ab_int.argtypes = [_D, _D, _D, _D, _I, _I, _D]
         _lib = L
     return _lib
 
@@ -199,3 +202,14 @@ def ab_solve(a, T, n, m, maxl, maxkiter, eps, F_func=0):
     C = np.empty(m + 1)
     err = lib().orc_ab_solve(a, T, n, m, maxl, maxkiter, eps, F_func, _p(C))
     return err, C
+
+
+def ab_history(a, tau, maxl, maxkiter, eps, fvs) -> np.ndarray:
+    fvs = np.ascontiguousarray(fvs, dtype=np.float64)
+    out = np.empty(fvs.size - 1)
+    lib().orc_ab_history(a, tau, maxl, maxkiter, eps, fvs.size, _p(fvs), _p(out))
+    return out
+
+
+def ab_int(tj, ts1, ts, a, maxl=50, maxkiter=1000, eps=1e-10) -> float:
+    return lib().orc_ab_int(tj, ts1, ts, a, maxl, maxkiter, eps)

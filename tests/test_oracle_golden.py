@@ -128,3 +128,54 @@ def test_mt_kappa_weights_telescope():
     out = O.mt_history(N, alpha, tau, snaps, os_ + 1)
     Da = tau ** (-alpha) / math.gamma(2 - alpha)
     np.testing.assert_allclose(out, Da * ((os_ - 1) ** (1 - alpha) - 1.0), rtol=1e-13)
+
+
+# ---------------------------------------------------------------- AB path
+def test_ab_weights_history_solve_match_reference(golden):
+    m, z = golden("mf")
+    for q, (a, tau, maxl, maxk, eps, D) in enumerate(m["weights"]):
+        np.testing.assert_allclose(O.ab_weights(a, tau, maxl, maxk, eps, D), z[f"w{q}"],
+                                   rtol=1e-14)
+    for q, (a, tau, maxl, maxk, eps, nmax) in enumerate(m["hist"]):
+        got = O.ab_history(a, tau, maxl, maxk, eps, z[f"fvs{q}"])
+        np.testing.assert_allclose(got, z[f"h{q}"], rtol=1e-13,
+                                   atol=1e-13 * np.max(np.abs(z[f"h{q}"])))
+    for q, (a, T, n, mm, simple, maxl, maxk, eps, F) in enumerate(m["solve"]):
+        err, _ = O.ab_solve(a, T, n, mm, maxl, maxk, eps, F)
+        assert err == pytest.approx(z["solve_err"][q], rel=1e-12)
+
+
+def test_mittag_leffler_matches_reference(golden):
+    _, z = golden("mf")
+    got = np.array([O.mittag_leffler(*row) for row in z["ml_in"]])
+    np.testing.assert_allclose(got, z["ml_out"], rtol=1e-14, atol=1e-300)
+
+
+def test_mittag_leffler_special_cases():
+    from math import cos, cosh, erfc, exp, sqrt
+    for zz in [0.3, 0.9, 2.0, -0.5, -2.0, -4.0]:
+        assert O.mittag_leffler(1.0, 1.0, zz, 1e-15) == pytest.approx(exp(zz), rel=1e-12)
+        want = cosh(sqrt(zz)) if zz > 0 else cos(sqrt(-zz))
+        assert O.mittag_leffler(2.0, 1.0, zz, 1e-15) == pytest.approx(want, rel=1e-11)
+    for zz in [0.2, 0.7, -0.3, -1.0, -2.5]:
+        want = exp(zz * zz) * erfc(-zz)
+        assert O.mittag_leffler(0.5, 1.0, zz, 1e-15) == pytest.approx(want, rel=1e-8)
+
+
+def test_ab_weights_against_quadrature():
+    # I(d) = integral over [(d-1) tau, d tau] of E_a(-a/(1-a) u^a) du; the
+    # integrand is bounded, so a substitution u = (d-1)tau + tau v^2 and
+    # Gauss-Legendre in v resolve the u^a behaviour at u = 0.  The
+    # reference's methods truncate: the l-series at d = 1 (tau/tj = 1, its
+    # terms do not decay) and the maxl-node quadrature of the integral form
+    # (d = 30 here) are good to ~2e-4; the other lags to ~1e-8.
+    a, tau = 0.6, 0.02
+    lam = a / (1 - a)
+    w = O.ab_weights(a, tau, 60, 1000, 1e-12, 30)
+    xg, wg = np.polynomial.legendre.leggauss(200)
+    v = 0.5 * (xg + 1)
+    for d, rel in [(1, 5e-4), (2, 1e-7), (5, 1e-7), (12, 1e-7), (30, 5e-4)]:
+        u = (d - 1) * tau + tau * v * v
+        f = np.array([O.mittag_leffler(a, 1.0, -lam * ui ** a, 1e-15) for ui in u])
+        q = 0.5 * np.sum(wg * f * 2 * tau * v)
+        assert w[d - 1] == pytest.approx(q, rel=rel), d
