@@ -120,6 +120,12 @@ typedef struct frm_tridiag_desc {
 int frm_tridiag(frm_ctx* ctx, const frm_tridiag_desc* desc, const double* a, const double* b,
                 const double* c, const double* d, double* x);
 
+/* memory kernels of the 2D LOD solver */
+enum {
+    FRM_KERNEL_CAPUTO = 0, /* (t-s)^-alpha / Gamma(1-alpha): fr_d_t_g_2d.cpp kb(), g(t) = t */
+    FRM_KERNEL_AB = 1      /* E_alpha(-alpha/(1-alpha) (t-s)^alpha) / (1-alpha): MF_int_series.cpp */
+};
+
 /* --------------------------------------- 2D LOD solver (TG_solver) ------
  * fr_d_t_g_2d.cpp TG_solver_<func=0 (g(t)=t), fixed_d, kform=0> with
  * dt_alg = 0 (full history).  Grid nodes i = 0..sN+1, j = 0..M+1 at
@@ -127,7 +133,16 @@ int frm_tridiag(frm_ctx* ctx, const frm_tridiag_desc* desc, const double* a, con
  * (Dirichlet C1 at i = 0, zero gradient at i = sN), column systems
  * i = 1..sN-1 (zero gradient at both ends).  The sweep coefficients are the
  * lower (A_) and upper (B) off-diagonals per node and direction
- * (fr_d_t_g_2d.cpp:3242-3265); the diagonal is (sigma/tau)*kb + A_ + B. */
+ * (fr_d_t_g_2d.cpp:3242-3265); the diagonal is (sigma/tau)*kb + A_ + B.
+ * Caputo weights kb(s, ja, sa, l) are the kernel's integrals over the LOD
+ * sub-intervals (fr_d_t_g_2d.cpp:3112-3131).  With FRM_KERNEL_AB both
+ * half-steps use the full-step weights of MF_intF_simple (MF_int_series.cpp:
+ * 614-718), I(d) = MF_int_choice(d tau, tau, 0, alpha, alpha, ab_maxl,
+ * ab_maxkiter, ab_eps) / (1 - alpha): current weight I(1), history weights
+ * I(k+1)/2 for the increment k steps back, no half-step correction; the two
+ * half-steps add up to the full-step AB discretisation (a half-interval
+ * current weight, as for Caputo, would make the constant mode grow because
+ * the AB kernel is bounded). */
 typedef struct frm_lod2d_desc {
     int32_t sN, M;
     int32_t capacity;  /* max number of steps (history depth) */
@@ -137,6 +152,10 @@ typedef struct frm_lod2d_desc {
     const double* rowB;
     const double* colA;
     const double* colB;
+    int32_t kernel;      /* FRM_KERNEL_*; 0 = Caputo */
+    int32_t ab_maxl;     /* AB series terms / quadrature nodes (0: 50) */
+    int32_t ab_maxkiter; /* AB inner series limit (0: 1000) */
+    double ab_eps;       /* AB series tolerance (0: 1e-10) */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);

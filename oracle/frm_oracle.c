@@ -12,6 +12,7 @@
  */
 #include "frm_oracle.h"
 #include <float.h>
+void orc_mf_cache_free(void* c);
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -131,7 +132,15 @@ double orc_tg_kb(int func, double p, double alpha, double tau, double int_err, i
     return 0.0;
 }
 
+/* forward declaration from the AB section below */
+static double orc_tg_ab_weight(orc_tg* t, double d, double l);
+
 struct orc_tg {
+    int ab_maxl, ab_maxkiter;
+    double ab_eps;
+    void* ab_cache;          /* mf_cache of the AB series */
+    double* ab_w;            /* cached canonical weights, [2][cap_ab] by 2*d */
+    int cap_ab;
     int sN, M, ld, func;
     double p, alpha, tau, sigma, C1, S, int_err;
     double *rowA, *rowB, *colA, *colB, *C;
@@ -194,6 +203,8 @@ void orc_tg_destroy(orc_tg* t) {
     free(t->old);
     free(t->rowA); free(t->rowB); free(t->colA); free(t->colB); free(t->C);
     free(t->kb); free(t->Al); free(t->Bt); free(t->Om);
+    free(t->ab_w);
+    if (t->ab_cache) { orc_mf_cache_free(t->ab_cache); free(t->ab_cache); }
     free(t);
 }
 
@@ -206,10 +217,19 @@ static void tg_fill_kb(orc_tg* t, int n) {
         t->kb = (double*)realloc(t->kb, 8 * (size_t)t->cap_kb * sizeof(double));
     }
     for (int s = 0; s <= n; ++s)
-        for (int v = 0; v < 8; ++v)
-            TG_KB(t, v, s) = orc_tg_kb(t->func, t->p, t->alpha, t->tau, t->int_err, n, s,
-                                       (v & 1) ? 1.0 : 0.5, (v & 2) ? 0.5 : 0.0,
-                                       (v & 4) ? 1.0 : 0.5);
+        for (int v = 0; v < 8; ++v) {
+            double ja = (v & 1) ? 1.0 : 0.5, sa = (v & 2) ? 0.5 : 0.0, l = (v & 4) ? 1.0 : 0.5;
+            if (t->func == ORC_KERNEL_AB) {
+                /* full-step AB weights in both half-steps: current I(1),
+                 * history I(n-s+1)/2, no half-step correction */
+                if (s == n)
+                    TG_KB(t, v, s) = (v == 0 || v == 3) ? orc_tg_ab_weight(t, 1.0, 1.0) : 0.0;
+                else
+                    TG_KB(t, v, s) = (v == 4 || v == 5) ? 0.5 * orc_tg_ab_weight(t, n - s + 1.0, 1.0) : 0.0;
+            } else
+                TG_KB(t, v, s) = orc_tg_kb(t->func, t->p, t->alpha, t->tau, t->int_err, n, s, ja,
+                                           sa, l);
+        }
 }
 
 /* One sweep along a line (U1 = al1, Om1, bt1, back-substitution:
@@ -1087,6 +1107,41 @@ void orc_ab_history(double a, double tau, int maxl, int maxkiter, double eps, in
     }
     free(I);
     mf_cache_free(&c);
+}
+
+void orc_mf_cache_free(void* c) { mf_cache_free((mf_cache*)c); }
+
+/* AB kernel in the LOD form: kb = MF_int_choice(d tau, l tau, 0) / (1 - a),
+ * cached by (2d, l). */
+static double orc_tg_ab_weight(orc_tg* t, double d, double l) {
+    int key = (int)(2.0 * d + 0.5);
+    int li = l == 1.0 ? 1 : 0;
+    if (key >= t->cap_ab) {
+        int c0 = t->cap_ab;
+        t->cap_ab = 2 * key + 16;
+        /* re-layout: [li][key] with the new capacity */
+        double* nw = (double*)malloc(2 * (size_t)t->cap_ab * sizeof(double));
+        for (int q = 0; q < 2 * t->cap_ab; ++q) nw[q] = NAN;
+        for (int L = 0; L < 2; ++L)
+            for (int q = 0; q < c0; ++q) nw[(size_t)L * t->cap_ab + q] = t->ab_w[(size_t)L * c0 + q];
+        free(t->ab_w);
+        t->ab_w = nw;
+    }
+    double* slot = &t->ab_w[(size_t)li * t->cap_ab + key];
+    if (isnan(*slot)) {
+        if (!t->ab_cache) t->ab_cache = calloc(1, sizeof(mf_cache));
+        *slot = mf_int_choice((mf_cache*)t->ab_cache, d * t->tau, l * t->tau, 0, t->alpha,
+                              t->alpha, t->ab_maxl, t->ab_maxkiter, t->ab_eps) /
+                (1.0 - t->alpha);
+    }
+    return *slot;
+}
+
+void orc_tg_set_ab(orc_tg* t, int maxl, int maxkiter, double eps) {
+    t->func = ORC_KERNEL_AB;
+    t->ab_maxl = maxl;
+    t->ab_maxkiter = maxkiter;
+    t->ab_eps = eps;
 }
 
 /* general AB integral (MF_int_choice) for the LOD weights at any lag */

@@ -50,6 +50,12 @@ int orc_tg_tstep(const orc_tg* t);
  * rows A_=(V2/dL)(dv/dL+1/2), B=(V2/dL)(dv/dL-1/2); columns V2*dv/dLx^2. */
 void orc_tg_coefs_ref(int sN, int M, const double* V2, double dv, double dL, double dLx,
                       double* rowA, double* rowB, double* colA, double* colB);
+/* Switch the LOD restatement to the Atangana-Baleanu kernel: both
+ * half-steps use the full-step weights I(d) = MF_int_choice(d tau, tau, 0,
+ * a, a, maxl, maxkiter, eps)/(1-a) of MF_intF_simple (MF_int_series.cpp:
+ * 614-718): current I(1), history I(k+1)/2, no half-step correction. */
+enum { ORC_KERNEL_AB = 10 };
+void orc_tg_set_ab(orc_tg* t, int maxl, int maxkiter, double eps);
 /* Grid spacings the reference constructor derives (fr_d_t_g_2d.cpp:2337-2388). */
 void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx);
 

@@ -43,12 +43,12 @@ def test_lod2d_matches_reference_golden(ctx, golden, name):
     s.close()
 
 
-def _adv_case(rng, sN, M):
+def _adv_case(rng, sN, M, D=1e-3):
     nn = (sN + 2) * (M + 2)
     vx = rng.uniform(-0.1, 0.1, nn)
     vy = rng.uniform(-0.1, 0.1, nn)
     h = 1.0 / (sN + 1)
-    coefs = S.coefs_advection(vx, vy, 1e-3, h)
+    coefs = S.coefs_advection(vx, vy, D, h)
     C0 = rng.uniform(0, 1, nn)
     return coefs, C0
 
@@ -99,4 +99,22 @@ def test_lod2d_full_size_linearity_and_constant(ctx):
     g = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, np.full(nn, 0.37), capacity=steps, C1=0.37)
     g.calc_step(steps)
     np.testing.assert_allclose(g.b_C, 0.37, rtol=1e-13)
+    g.close()
+
+
+@pytest.mark.parametrize("alpha,steps", [(0.3, 60), (0.6, 80), (0.9, 50)])
+def test_lod2d_ab_kernel_matches_oracle(ctx, alpha, steps):
+    # Atangana-Baleanu kernel in the LOD form: both sides use the full-step
+    # weights I(d) = MF_int_choice(d tau, tau, 0)/(1-a) of MF_intF_simple
+    rng = np.random.default_rng(int(alpha * 100))
+    sN, M = 40, 31
+    coefs, C0 = _adv_case(rng, sN, M, D=5e-2)
+    tau, sigma = 1e-2, 1.0
+    ab = (50, 1000, 1e-10)
+    o = O.TGSolver(sN, M, alpha, tau, sigma, 0.1, 0.0, *coefs, C0, ab=ab)
+    g = S.TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=steps, C1=0.1,
+                   kernel=N.KERNEL_AB, ab=ab)
+    o.step(steps)
+    g.calc_step(steps)
+    assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
     g.close()

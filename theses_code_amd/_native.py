@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libfrm.so")
 
 FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
 PREC_F64, PREC_F32C, PREC_MFMA_BF16 = 0, 1, 2
+KERNEL_CAPUTO, KERNEL_AB = 0, 1
 K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER = 0, 1, 2, 3, 4
 
 _D, _I, _I64, _U32 = ct.c_double, ct.c_int, ct.c_int64, ct.c_uint32
@@ -38,7 +39,9 @@ class TridiagDesc(ct.Structure):
 class Lod2dDesc(ct.Structure):
     _fields_ = [("sN", ct.c_int32), ("M", ct.c_int32), ("capacity", ct.c_int32),
                 ("precision", ct.c_int32), ("alpha", _D), ("tau", _D), ("sigma", _D),
-                ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P)]
+                ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P),
+                ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
+                ("ab_eps", _D)]
 
 
 class MT1DProblem(ct.Structure):

@@ -26,6 +26,7 @@
 
 #include "frm_history.hpp"
 #include "frm_wavescan.hpp"
+#include "frm_weights.hpp"
 
 struct frm_lod2d {
     frm_ctx* ctx = nullptr;
@@ -261,6 +262,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: coefficient arrays missing");
     FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64, "frm_lod2d_create: precision %d not built",
                 d->precision);
+    FRM_REQUIRE(ctx, d->kernel == FRM_KERNEL_CAPUTO || d->kernel == FRM_KERNEL_AB,
+                "frm_lod2d_create: kernel %d", d->kernel);
     const int cmr = pick_cm(d->sN - 1), cmc = pick_cm(d->M - 1);
     FRM_REQUIRE(ctx, cmr && cmc, "frm_lod2d_create: lines longer than %d unknowns", 64 * 32);
     FRM_HIP(ctx, hipSetDevice(ctx->device));
@@ -281,11 +284,23 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     s->sg = d->sigma / d->tau;
     s->cm_rows = cmr;
     s->cm_cols = cmc;
+    // Caputo: the kernel's integrals over the LOD sub-intervals, as
+    // TG_solver::kb (rows target t_{n+1/2}, columns t_{n+1}).
+    // Atangana-Baleanu: the kernel is bounded, so a half-interval current
+    // weight is smaller than the first history weight and the constant mode
+    // of the zero-gradient columns would grow; both half-steps therefore take
+    // the full-step weights of MF_intF_simple, I(d) = MF_int_choice(d tau,
+    // tau, 0)/(1-alpha), current I(1), with half of the memory term each, so
+    // that the two half-steps add up to the full-step AB discretisation.
+    frm::AbIntegrator ab(d->alpha, d->ab_maxl > 0 ? d->ab_maxl : 50,
+                         d->ab_maxkiter > 0 ? d->ab_maxkiter : 1000,
+                         d->ab_eps > 0 ? d->ab_eps : 1e-10);
+    const bool is_ab = d->kernel == FRM_KERNEL_AB;
+    auto abI = [&](int dd) { return ab.integral(dd * d->tau, d->tau, 0.0) / (1.0 - d->alpha); };
     // current-step weights: kb(n,1/2,0,1/2), kb(n,1,1/2,1/2), kb(n,1,0,1/2)
-    s->kr0 = caputo_kb(d->alpha, d->tau, 0.5, 0.5);
-    s->kc0 = caputo_kb(d->alpha, d->tau, 0.5, 0.5);
-    s->kch = caputo_kb(d->alpha, d->tau, 1.0, 0.5);
-
+    s->kr0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
+    s->kc0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
+    s->kch = is_ab ? 0.0 : caputo_kb(d->alpha, d->tau, 1.0, 0.5);
     const int64_t nodes = (int64_t)s->NX * s->NY;
     const size_t nn = (size_t)s->ld * (d->M + 2);
     int rc = hist_alloc(ctx, nodes, d->capacity, d->precision, &s->hist);
@@ -306,8 +321,12 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     // d = k+1/2, l = 1; columns kb(s,1,0,1) is d = k+1, l = 1
     std::vector<double> w(2 * (size_t)d->capacity);
     for (int k = 1; k <= d->capacity; ++k) {
-        w[k - 1] = caputo_kb(d->alpha, d->tau, k + 0.5, 1.0);
-        w[d->capacity + k - 1] = caputo_kb(d->alpha, d->tau, k + 1.0, 1.0);
+        if (is_ab) {
+            w[k - 1] = w[d->capacity + k - 1] = 0.5 * abI(k + 1);
+        } else {
+            w[k - 1] = caputo_kb(d->alpha, d->tau, k + 0.5, 1.0);
+            w[d->capacity + k - 1] = caputo_kb(d->alpha, d->tau, k + 1.0, 1.0);
+        }
     }
     // line coefficients
     std::vector<double> rc3((size_t)3 * (d->M - 1) * s->NX), cc3((size_t)3 * (d->sN - 1) * s->NY);

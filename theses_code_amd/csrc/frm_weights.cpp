@@ -1,0 +1,205 @@
+// Host FP64 generators of Atangana-Baleanu kernel integrals.
+//
+// I(tj, ts1, ts) = integral over [ts, ts1] of E_a(-a/(1-a) (tj - s)^a) ds is
+// evaluated with the two methods of MF_int_series.cpp: the double series of
+// MF_int / MF_int_ksum (332-455), used when (a/(1-a)) tj^a < 1, and the
+// maxl-node quadrature of the ML function's integral representation of
+// MF_intFv (457-521) otherwise (MF_int_choice, 523-530).  The series stops
+// after ten consecutive terms below eps/l!, so its value depends on the
+// operation order; the arithmetic below follows the reference's order so
+// that the GPU path sees the reference's weights.
+#include <cmath>
+#include <vector>
+
+#include "frm_weights.hpp"
+
+namespace frm {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+double cody_gamma(double x);
+
+double cody_log_gamma(double x) {
+    if (x <= 0.0) return cody_log_gamma(x + 1.0) - std::log(x);
+    if (x < 12.0) return std::log(std::fabs(cody_gamma(x)));
+    static const double c[8] = {1.0 / 12.0,   -1.0 / 360.0,        1.0 / 1260.0, -1.0 / 1680.0,
+                                1.0 / 1188.0, -691.0 / 360360.0, 1.0 / 156.0,  -3617.0 / 122400.0};
+    const double z = 1.0 / (x * x);
+    double acc = c[7];
+    for (int k = 6; k >= 0; --k) {
+        acc *= z;
+        acc += c[k];
+    }
+    return (x - 0.5) * std::log(x) - x + 0.91893853320467274178032973640562 + acc / x;
+}
+
+// W. J. Cody's rational approximation on (1,2) with argument reduction (the
+// published coefficient table every reference file carries).
+double cody_gamma(double x) {
+    static const double p[8] = {-1.71618513886549492533811E+0, 2.47656508055759199108314E+1,
+                                -3.79804256470945635097577E+2, 6.29331155312818442661052E+2,
+                                8.66966202790413211295064E+2,  -3.14512729688483675254357E+4,
+                                -3.61444134186911729807069E+4, 6.64561438202405440627855E+4};
+    static const double q[8] = {-3.08402300119738975254353E+1, 3.15350626979604161529144E+2,
+                                -1.01515636749021914166146E+3, -3.10777167157231109440444E+3,
+                                2.25381184209801510330112E+4,  4.75584627752788110767815E+3,
+                                -1.34659959864969306392456E+5, -1.15132259675553483497211E+5};
+    if (x <= 0.0) return (1.0 / x) * cody_gamma(x + 1.0);
+    if (x < 0.001) return 1.0 / (x * (1.0 + 0.577215664901532860606512090 * x));
+    if (x < 12.0) {
+        double y = x;
+        int shift = 0;
+        const bool below = y < 1.0;
+        if (below)
+            y += 1.0;
+        else {
+            shift = (int)std::floor(y) - 1;
+            y -= shift;
+        }
+        double num = 0.0, den = 1.0;
+        const double z = y - 1;
+        for (int k = 0; k < 8; ++k) {
+            num = (num + p[k]) * z;
+            den = den * z + q[k];
+        }
+        double r = num / den + 1.0;
+        if (below)
+            r /= (y - 1.0);
+        else
+            for (int k = 0; k < shift; ++k) r *= y++;
+        return r;
+    }
+    if (x > 171.624) return 1e308 * 10.0;
+    return std::exp(cody_log_gamma(x));
+}
+
+double sgn(double x) { return x > 0.0 ? 1.0 : -1.0; }
+
+double spectral(double a, double v) {  // Fv of the integral representation
+    const double p = -a / (1.0 - a);
+    return (1.0 / (kPi * a)) * (p * std::sin(kPi * a)) / (v * v - 2 * v * p * std::cos(kPi * a) + p * p);
+}
+}  // namespace
+
+AbIntegrator::AbIntegrator(double a, int maxl, int maxkiter, double eps)
+    : a_(a), maxl_(maxl), maxkiter_(maxkiter), eps_(eps) {
+    logi_.resize(maxl);
+    logi_[0] = 0.0;
+    for (int i = 1; i < maxl; ++i) logi_[i] = std::log((double)i);
+    lgk_.assign(maxkiter, -1.0);
+    prodlog_.assign((size_t)maxkiter * maxl, 0.0);
+    prodsgn_.assign((size_t)maxkiter * maxl, 2.0);
+}
+
+// coefficient of (ts1^(l+1) - ts^(l+1)) in the series (MF_int_ksum)
+double AbIntegrator::ksum(int l, double tj) {
+    const double a = a_;
+    double ret = 0.0;
+    int small_run = 0;
+    const double lfact = cody_log_gamma(l + 1.0);
+    const double la = std::log(a / (1.0 - a));
+    const double ltj = std::log(tj);
+    const double leps = std::log(eps_);
+    double s = std::pow(-1.0, l);
+    for (double k = 0; k < maxkiter_; k++) {
+        const int ki = (int)k;
+        const double k1 = k * la;
+        double k2;
+        if (lgk_[ki] < 0) {
+            k2 = cody_log_gamma(a * k + 1.0);
+            lgk_[ki] = k2;
+        } else
+            k2 = lgk_[ki];
+        const double k3 = (a * k - l) * ltj;
+        double term = k1 + k3 - k2;
+        const size_t q = (size_t)ki * maxl_ + l;
+        double k4 = 0, s4 = 1;
+        if (prodsgn_[q] == 2) {
+            for (int i = 1; i <= l; i++) {
+                k4 += std::log(std::fabs(a * k - i + 1)) - logi_[i];
+                s4 *= sgn(a * k - i + 1);
+            }
+            prodlog_[q] = k4;
+            prodsgn_[q] = s4;
+        } else {
+            k4 = prodlog_[q];
+            s4 = prodsgn_[q];
+        }
+        term += k4;
+        term = std::exp(term);
+        term *= s * s4 / (l + 1.0);
+        if (std::log(std::fabs(term)) < leps - lfact)
+            small_run++;
+        else
+            small_run = 0;
+        if (small_run == 10) break;
+        if (!std::isfinite(term)) break;
+        ret += term;
+        s *= -1.0;
+    }
+    return ret;
+}
+
+double AbIntegrator::integral(double tj, double ts1, double ts) {
+    const double a = a_;
+    const double maxz = (a / (1 - a)) * std::pow(tj, a);
+    if (maxz < 1.0) {
+        double ret = 0.0;
+        for (int l = 0; l < maxl_; l++) {
+            const double lk = std::pow(ts1, (l + 1.0)) - std::pow(ts, (l + 1.0));
+            const double lt = lk * ksum(l, tj);
+            if (std::isfinite(lt)) ret += lt;
+        }
+        return ret;
+    }
+    // integral representation, quadrature over maxl nodes in three segments
+    const double eps = 1e-10;
+    const int maxl = maxl_;
+    double C = 1.0;
+    for (;;) {
+        const double ks = ts * std::pow(C, 1.0 / a), ks1 = ts1 * std::pow(C, 1.0 / a);
+        const double k2 = std::log(std::fabs(spectral(a, C))) - (1.0 / a) * std::log(C) -
+                          std::pow(C, 1.0 / a) * tj;
+        if (std::fabs(std::exp(ks + k2) - std::exp(ks1 + k2)) > eps)
+            C *= 1.5;
+        else
+            break;
+    }
+    const double Fm = -a * std::cos(kPi * a) / (1.0 - a);
+    const double Fd = 5 * (std::sqrt(3.0) / 3) * std::sqrt((a * a / ((1.0 - a) * (1.0 - a))) - Fm * Fm);
+    double mC1 = Fm + Fd, mC2 = Fm - Fd;
+    if (mC2 < 0.0) mC2 = 0.0;
+    if (mC1 < 0.0) mC1 = 0.0;
+    if (mC2 == 0) mC2 = mC1 / 2.0;
+    if (mC1 == 0) {
+        mC1 = C / 3.0;
+        mC2 = 2.0 * C / 3.0;
+    }
+    const double wk1 = mC2 / (maxl / 3), wk2 = (mC1 - mC2) / (maxl / 3),
+                 wk3 = 2.0 * (C - mC2) / (maxl / 3);
+    double ret = 0.0;
+    for (int l = 0; l < maxl; l++) {
+        double wk, vk;
+        if (l < (maxl / 3)) {
+            wk = wk1;
+            vk = l * wk + 0.5 * wk;
+        } else if (l < 2 * (maxl / 3)) {
+            wk = wk2;
+            vk = mC2 + (l - (maxl / 3)) * wk + 0.5 * wk;
+        } else {
+            wk = wk2 + (wk3 - wk2) * (double)(l - 2 * (maxl / 3)) / (maxl / 3);
+            vk = mC1 + (l - 2 * (maxl / 3)) * wk + 0.5 * wk;
+        }
+        const double ks = ts * std::pow(vk, 1.0 / a), ks1 = ts1 * std::pow(vk, 1.0 / a);
+        const double k2 = std::log(std::fabs(spectral(a, vk))) - (1.0 / a) * std::log(vk) -
+                          std::pow(vk, 1.0 / a) * tj;
+        double lt = wk * (std::exp(ks + k2) - std::exp(ks1 + k2));
+        if (spectral(a, vk) < 0) lt = -lt;
+        if (!std::isfinite(lt)) break;
+        ret += lt;
+    }
+    return ret;
+}
+
+}  // namespace frm

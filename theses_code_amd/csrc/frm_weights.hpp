@@ -1,0 +1,24 @@
+// Host-side FP64 weight generators.
+#pragma once
+#include <vector>
+
+namespace frm {
+
+// Integrals of the Atangana-Baleanu kernel E_a(-a/(1-a) u^a) by the methods
+// of MF_int_series.cpp (series / integral representation), with the
+// per-(k, l) caches of the series kept across calls.
+class AbIntegrator {
+   public:
+    AbIntegrator(double a, int maxl, int maxkiter, double eps);
+    // integral over [ts, ts1] of E_a(-a/(1-a) (tj - s)^a) ds
+    double integral(double tj, double ts1, double ts);
+
+   private:
+    double ksum(int l, double tj);
+    double a_;
+    int maxl_, maxkiter_;
+    double eps_;
+    std::vector<double> logi_, lgk_, prodlog_, prodsgn_;
+};
+
+}  // namespace frm
