@@ -36,7 +36,8 @@ enum {
 enum {
     FRM_PREC_F64 = 0,      /* plain FP64 history and FP64 accumulation */
     FRM_PREC_F32C = 1,     /* FP32 history, compensated (Kahan) FP32 sums */
-    FRM_PREC_MFMA_BF16 = 2 /* blocked Toeplitz on MFMA, split hi+lo BF16, FP32 acc */
+    FRM_PREC_MFMA_BF16 = 2, /* blocked Toeplitz on MFMA, split hi+lo BF16, FP32 acc */
+    FRM_PREC_SOE = 3        /* sum-of-exponentials recurrences (series history) */
 };
 
 /* kernel-class ids for frm_timing_get */
@@ -156,6 +157,9 @@ typedef struct frm_lod2d_desc {
     int32_t ab_maxl;     /* AB series terms / quadrature nodes (0: 50) */
     int32_t ab_maxkiter; /* AB inner series limit (0: 1000) */
     double ab_eps;       /* AB series tolerance (0: 1e-10) */
+    int32_t ab_exact;    /* 1: AB weights to FP64 accuracy instead of the MF_int_choice methods */
+    int32_t soe_terms;   /* FRM_PREC_SOE: exponentials (0: 32, max 64) */
+    int32_t soe_f32;     /* FRM_PREC_SOE: 1 stores the mode states in FP32 */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);
@@ -169,6 +173,13 @@ int frm_lod2d_tstep(const frm_lod2d* s, int32_t* tstep);
 /* Device pointer of the state (reference layout), for callers that keep
  * the field on the GPU. */
 int frm_lod2d_state_ptr(frm_lod2d* s, double** d_C);
+/* FRM_PREC_SOE: largest relative error of the fitted exponential sum over
+ * the history weights it replaces (lags 2..capacity); 0 for full history. */
+int frm_lod2d_fit_error(const frm_lod2d* s, double* max_rel);
+/* Host copies of the weights in use: out[0] = row current weight, out[1] =
+ * column current weight, out[2] = half-step correction, then the row and
+ * column history weights for lags 1..n (2 + 1 + 2n values). */
+int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out);
 
 /* ------------------------- batched 1D H_solver (mt_minimal / mt_psi_caputo)
  * A batch of independent 1D problems stepped together, each one

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfrm.so")
 
 FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
-PREC_F64, PREC_F32C, PREC_MFMA_BF16 = 0, 1, 2
+PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE = 0, 1, 2, 3
 KERNEL_CAPUTO, KERNEL_AB = 0, 1
 K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER = 0, 1, 2, 3, 4
 
@@ -41,7 +41,8 @@ class Lod2dDesc(ct.Structure):
                 ("precision", ct.c_int32), ("alpha", _D), ("tau", _D), ("sigma", _D),
                 ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P),
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
-                ("ab_eps", _D)]
+                ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
+                ("soe_f32", ct.c_int32)]
 
 
 class MT1DProblem(ct.Structure):
@@ -89,6 +90,8 @@ SIGNATURES = {
     "frm_lod2d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
     "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
+    "frm_lod2d_fit_error": (_I, [_V, _P]),
+    "frm_lod2d_weights": (_I, [_V, ct.c_int32, _P]),
     "frm_mt1d_create": (_I, [_V, ct.POINTER(MT1DDesc), ct.POINTER(_V)]),
     "frm_mt1d_destroy": (_I, [_V]),
     "frm_mt1d_set_state": (_I, [_V, _P]),

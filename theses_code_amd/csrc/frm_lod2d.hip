@@ -26,6 +26,7 @@
 
 #include "frm_history.hpp"
 #include "frm_wavescan.hpp"
+#include "frm_soe.hpp"
 #include "frm_weights.hpp"
 
 struct frm_lod2d {
@@ -44,6 +45,15 @@ struct frm_lod2d {
     double* rcoef = nullptr;   // rows:    [3][M-1][NX]  a, f, al
     double* ccoef = nullptr;   // columns: [3][sN-1][NY] a, f, al
     int cm_rows = 0, cm_cols = 0;
+    int precision = FRM_PREC_F64;
+    std::vector<double> hw;    // host copy: kr0, kc0, kch, then [2][capacity] weights
+    // sum-of-exponentials form
+    int soe_L = 0, soe_f32 = 0;
+    double soe_err = 0.0;
+    void* soe_H = nullptr;     // [L][ld] mode states (FP32 or FP64)
+    double* soe_q = nullptr;   // [L]
+    double* soe_c = nullptr;   // [2][L]
+    double soe_w1r = 0.0, soe_w1c = 0.0;  // exact lag-1 weights
 };
 
 namespace frm {
@@ -180,6 +190,45 @@ __global__ __launch_bounds__(256) void k_lod_cols(ColArgs p) {
     }
 }
 
+// ------------------------------------------------------------- SOE memory term
+// History weights for lags k >= 2 replaced by sum_l c_{t,l} q_l^(k-2): per
+// node, the mode states H_l = sum_{s <= n-2} q_l^(n-2-s) Delta_s follow
+// H_l <- q_l H_l + Delta_{n-2} (the increment that just reached lag 2), and
+// lag 1 is summed exactly from a two-slot ring of increments.  The decay
+// multiply is done in FP64 so that slow modes keep their exact factor when
+// the states are stored in FP32.
+constexpr int kSoeLag0 = 2;
+
+template <typename HT>
+__global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const double* __restrict__ ring,
+                                                     int64_t ld, int n, int L,
+                                                     const double* __restrict__ q,
+                                                     const double* __restrict__ c, double w1r,
+                                                     double w1c, double* __restrict__ part) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld) return;
+    const bool fold = n >= kSoeLag0;
+    const double x = fold ? ring[(int64_t)((n - kSoeLag0) % kSoeLag0) * ld + i] : 0.0;
+    double s0 = 0.0, s1 = 0.0;
+    for (int l = 0; l < L; ++l) {
+        double h = (double)H[(int64_t)l * ld + i];
+        if (fold) {
+            h = q[l] * h + x;
+            H[(int64_t)l * ld + i] = (HT)h;
+            h = (double)(HT)h;
+        }
+        s0 += c[l] * h;
+        s1 += c[L + l] * h;
+    }
+    if (n >= 1) {
+        const double y = ring[(int64_t)((n - 1) % kSoeLag0) * ld + i];
+        s0 += w1r * y;
+        s1 += w1c * y;
+    }
+    part[i] = s0;
+    part[ld + i] = s1;
+}
+
 // ------------------------------------------------------------- host helpers
 // Thomas alpha coefficients of one line in the reference's order
 // (al1: fr_d_t_g_2d.cpp:3267-3304, R: 3257-3265); f = Al[i+1]/B(i), or the
@@ -241,6 +290,9 @@ static void lod_free(frm_lod2d* s) {
     hipFree(s->w);
     hipFree(s->rcoef);
     hipFree(s->ccoef);
+    hipFree(s->soe_H);
+    hipFree(s->soe_q);
+    hipFree(s->soe_c);
     delete s;
 }
 
@@ -260,8 +312,10 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: need 0 < alpha < 1 and tau > 0");
     FRM_REQUIRE(ctx, d->rowA && d->rowB && d->colA && d->colB,
                 "frm_lod2d_create: coefficient arrays missing");
-    FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64, "frm_lod2d_create: precision %d not built",
-                d->precision);
+    FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE,
+                "frm_lod2d_create: precision %d not built", d->precision);
+    FRM_REQUIRE(ctx, d->soe_terms >= 0 && d->soe_terms <= 64, "frm_lod2d_create: soe_terms %d",
+                d->soe_terms);
     FRM_REQUIRE(ctx, d->kernel == FRM_KERNEL_CAPUTO || d->kernel == FRM_KERNEL_AB,
                 "frm_lod2d_create: kernel %d", d->kernel);
     const int cmr = pick_cm(d->sN - 1), cmc = pick_cm(d->M - 1);
@@ -296,14 +350,25 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                          d->ab_maxkiter > 0 ? d->ab_maxkiter : 1000,
                          d->ab_eps > 0 ? d->ab_eps : 1e-10);
     const bool is_ab = d->kernel == FRM_KERNEL_AB;
-    auto abI = [&](int dd) { return ab.integral(dd * d->tau, d->tau, 0.0) / (1.0 - d->alpha); };
+    std::vector<double> abx;
+    if (is_ab && d->ab_exact) {
+        abx.resize(d->capacity + 2);
+        frm::ab_exact_weights(d->alpha, d->tau, d->capacity + 2, abx.data());
+    }
+    auto abI = [&](int dd) {
+        if (!abx.empty()) return abx[dd - 1] / (1.0 - d->alpha);
+        return ab.integral(dd * d->tau, d->tau, 0.0) / (1.0 - d->alpha);
+    };
     // current-step weights: kb(n,1/2,0,1/2), kb(n,1,1/2,1/2), kb(n,1,0,1/2)
     s->kr0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
     s->kc0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
     s->kch = is_ab ? 0.0 : caputo_kb(d->alpha, d->tau, 1.0, 0.5);
     const int64_t nodes = (int64_t)s->NX * s->NY;
     const size_t nn = (size_t)s->ld * (d->M + 2);
-    int rc = hist_alloc(ctx, nodes, d->capacity, d->precision, &s->hist);
+    s->precision = d->precision;
+    // the SOE form keeps only a two-slot ring of increments
+    int rc = hist_alloc(ctx, nodes, d->precision == FRM_PREC_SOE ? kSoeLag0 : d->capacity,
+                        FRM_PREC_F64, &s->hist);
     if (rc) {
         lod_free(s);
         return rc;
@@ -343,6 +408,28 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         line_coefs(s->NY, d->colA + base, d->colB + base, s->ld, s->sg * s->kc0, 1.0, oa,
                    oa + lcols * s->NY, oa + 2 * lcols * s->NY);
     }
+    s->hw.assign({s->kr0, s->kc0, s->kch});
+    s->hw.insert(s->hw.end(), w.begin(), w.end());
+    if (d->precision == FRM_PREC_SOE) {
+        s->soe_L = d->soe_terms > 0 ? d->soe_terms : 32;
+        s->soe_f32 = d->soe_f32;
+        s->soe_w1r = w[0];
+        s->soe_w1c = w[d->capacity];
+        frm::SoeFit fit = frm::soe_fit({w.data(), w.data() + d->capacity}, d->capacity, kSoeLag0,
+                                       s->soe_L, d->tau);
+        s->soe_err = fit.max_rel;
+        const size_t hsz = (s->soe_f32 ? sizeof(float) : sizeof(double)) * (size_t)s->soe_L * s->hist->ld;
+        e = hipMalloc(&s->soe_H, hsz);
+        if (e == hipSuccess) e = hipMalloc(&s->soe_q, sizeof(double) * s->soe_L);
+        if (e == hipSuccess) e = hipMalloc(&s->soe_c, sizeof(double) * 2 * s->soe_L);
+        if (e == hipSuccess) e = hipMemcpy(s->soe_q, fit.q.data(), sizeof(double) * s->soe_L, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(s->soe_c, fit.c.data(), sizeof(double) * 2 * s->soe_L, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(s->soe_H, 0, hsz);
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMalloc(lod2d SOE)");
+        }
+    }
     e = hipMemcpy(s->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(s->rcoef, rc3.data(), rc3.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -371,6 +458,11 @@ int frm_lod2d_set_state(frm_lod2d* s, const double* h_C) {
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->tstep = 0;
     s->hist->depth = 0;
+    if (s->soe_H) {
+        const size_t hsz = (s->soe_f32 ? sizeof(float) : sizeof(double)) * (size_t)s->soe_L * s->hist->ld;
+        FRM_HIP(ctx, hipMemsetAsync(s->soe_H, 0, hsz, ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return FRM_OK;
 }
 
@@ -378,16 +470,34 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     if (!s) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_step: null solver");
     frm_ctx* ctx = s->ctx;
     FRM_REQUIRE(ctx, nsteps >= 0, "frm_lod2d_step: nsteps %d", nsteps);
-    if (s->hist->depth + nsteps > s->capacity)
+    const bool soe = s->precision == FRM_PREC_SOE;
+    if (s->tstep + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
-                         s->hist->depth + nsteps, s->capacity);
+                         s->tstep + nsteps, s->capacity);
     const double S_half = s->S * 0.5;
     const int wpb = 256 / kWave;
     for (int it = 0; it < nsteps; ++it) {
         const int n = s->tstep;
         int nch = 0;
-        int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);
-        if (rc) return rc;
+        if (soe) {
+            nch = 1;
+            const int64_t ld = s->hist->ld;
+            LaunchTimer t(ctx, FRM_K_MEMTERM,
+                          (double)ld * (2.0 * s->soe_L * (s->soe_f32 ? 4 : 8) + 2 * 8 + 2 * 8));
+            const dim3 grid((unsigned)((ld + 255) / 256));
+            if (s->soe_f32)
+                hipLaunchKernelGGL(k_soe_memterm<float>, grid, dim3(256), 0, ctx->stream,
+                                   (float*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
+                                   s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
+            else
+                hipLaunchKernelGGL(k_soe_memterm<double>, grid, dim3(256), 0, ctx->stream,
+                                   (double*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
+                                   s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
+            FRM_HIP(ctx, hipGetLastError());
+        } else {
+            int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);
+            if (rc) return rc;
+        }
         {
             RowArgs a;
             a.C = s->C;
@@ -412,7 +522,8 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             ColArgs a;
             a.C = s->C;
             a.Cprev = s->Cprev;
-            a.slot = (double*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld;
+            a.slot = (double*)s->hist->data +
+                     (int64_t)(soe ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
             a.coef = s->ccoef;
             a.part = s->hist->part;
             a.ldh = s->hist->ld;
@@ -430,7 +541,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_COLS, 0.0);
             FRM_HIP(ctx, launch_cols(s->cm_cols, dim3((a.lines + wpb - 1) / wpb), ctx->stream, a));
         }
-        s->hist->depth += 1;
+        if (!soe) s->hist->depth += 1;
         s->tstep = n + 1;
     }
     return FRM_OK;
@@ -448,6 +559,25 @@ int frm_lod2d_get_state(frm_lod2d* s, double* h_C) {
 int frm_lod2d_tstep(const frm_lod2d* s, int32_t* tstep) {
     if (!s || !tstep) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_tstep: null");
     *tstep = s->tstep;
+    return FRM_OK;
+}
+
+int frm_lod2d_fit_error(const frm_lod2d* s, double* max_rel) {
+    if (!s || !max_rel) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_fit_error: null");
+    *max_rel = s->soe_err;
+    return FRM_OK;
+}
+
+int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out) {
+    if (!s || !out || n < 0 || n > s->capacity)
+        return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_weights: bad arguments");
+    out[0] = s->hw[0];
+    out[1] = s->hw[1];
+    out[2] = s->hw[2];
+    for (int k = 0; k < n; ++k) {
+        out[3 + k] = s->hw[3 + k];
+        out[3 + n + k] = s->hw[3 + s->capacity + k];
+    }
     return FRM_OK;
 }
 

@@ -203,3 +203,52 @@ double AbIntegrator::integral(double tj, double ts1, double ts) {
 }
 
 }  // namespace frm
+
+namespace frm {
+
+// I(d) = integral over [(d-1) tau, d tau] of E_a(-lam u^a) du, lam = a/(1-a),
+// to FP64 accuracy: d = 1 by the series t E_{a,2}(-lam t^a) =
+// sum_k (-lam)^k t^(a k + 1) / Gamma(a k + 2); d >= 2 from the spectral
+// representation E_a(-lam u^a) = int_0^inf mu(r) e^(-r u) dr with
+// mu(r) = lam sin(pi a) r^(a-1) / (pi (r^(2a) + 2 lam r^a cos(pi a) + lam^2)),
+// integrated over x = log r by the trapezoid rule (the integrand is analytic
+// in a strip around the real x axis and decays exponentially at both ends,
+// so the rule converges geometrically in the step).
+void ab_exact_weights(double a, double tau, int D, double* out) {
+    const double lam = a / (1.0 - a);
+    {
+        const double t = tau, z = -lam * std::pow(t, a);
+        double s = 0.0;
+        for (int k = 0; k < 400; ++k) {
+            const double tk = std::pow(z, k) / std::exp(std::lgamma(a * k + 2));
+            s += tk;
+            if (k > 5 && std::fabs(tk) < 1e-18 * std::fabs(s)) break;
+        }
+        out[0] = t * s;
+    }
+    if (D < 2) return;
+    const double h = 0.01;
+    const double x0 = std::log(1e-16) / a + std::log(lam) / a - 5.0;
+    const double x1 = std::log(60.0 / tau);
+    const int nx = (int)((x1 - x0) / h) + 1;
+    std::vector<double> base(nx), fac(nx), q(nx);
+    const double sa = std::sin(kPi * a), ca = std::cos(kPi * a);
+    for (int j = 0; j < nx; ++j) {
+        const double r = std::exp(x0 + j * h);
+        const double ra = std::pow(r, a);
+        const double mu = lam * sa / kPi * std::pow(r, a - 1) / (ra * ra + 2 * lam * ra * ca + lam * lam);
+        base[j] = mu * (-std::expm1(-r * tau)) * h;  // (1 - e^{-r tau})/r * r dx
+        q[j] = std::exp(-r * tau);
+        fac[j] = q[j];  // e^{-r (d-1) tau} for d = 2
+    }
+    for (int d = 2; d <= D; ++d) {
+        double s = 0.0;
+        for (int j = 0; j < nx; ++j) {
+            s += base[j] * fac[j];
+            fac[j] *= q[j];
+        }
+        out[d - 1] = s;
+    }
+}
+
+}  // namespace frm

@@ -21,4 +21,8 @@ class AbIntegrator {
     std::vector<double> logi_, lgk_, prodlog_, prodsgn_;
 };
 
+// Accurate FP64 integrals I(d) = int_{(d-1)tau}^{d tau} E_a(-a/(1-a) u^a) du,
+// d = 1..D (series for d = 1, spectral representation otherwise).
+void ab_exact_weights(double a, double tau, int D, double* out);
+
 }  // namespace frm

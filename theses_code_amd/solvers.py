@@ -52,12 +52,15 @@ class TGSolver:
 
     def __init__(self, ctx: N.Context, sN: int, M: int, alpha: float, tau: float, sigma: float,
                  coefs, C0: np.ndarray, capacity: int, C1: float = 1.0, S: float = 0.0,
-                 precision: int = N.PREC_F64, kernel: int = N.KERNEL_CAPUTO, ab=(50, 1000, 1e-10)):
+                 precision: int = N.PREC_F64, kernel: int = N.KERNEL_CAPUTO, ab=(50, 1000, 1e-10),
+                 ab_exact: bool = False, soe_terms: int = 32, soe_f32: bool = False):
         self.ctx, self.sN, self.M = ctx, sN, M
         nn = (sN + 2) * (M + 2)
         self._coefs = [np.ascontiguousarray(c, dtype=np.float64).reshape(nn) for c in coefs]
         d = N.Lod2dDesc(sN, M, capacity, precision, alpha, tau, sigma, C1, S,
-                        *[N._dp(c) for c in self._coefs], kernel, ab[0], ab[1], ab[2])
+                        *[N._dp(c) for c in self._coefs], kernel, ab[0], ab[1], ab[2],
+                        int(ab_exact), soe_terms, int(soe_f32))
+        self.capacity = capacity
         h = N._V()
         N._check(N.lib().frm_lod2d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
         self.h = h
@@ -82,6 +85,20 @@ class TGSolver:
         out = np.empty((self.M + 2) * (self.sN + 2))
         N._check(N.lib().frm_lod2d_get_state(self.h, N._dp(out)), self.ctx.h)
         return out.reshape(self.M + 2, self.sN + 2)
+
+    @property
+    def fit_error(self) -> float:
+        """Largest relative error of the SOE fit over the weights it replaces."""
+        v = ct.c_double()
+        N._check(N.lib().frm_lod2d_fit_error(self.h, ct.byref(v)), self.ctx.h)
+        return v.value
+
+    def weights(self, n: int):
+        """(row current, column current, half-step correction, row weights,
+        column weights) for lags 1..n."""
+        out = np.empty(3 + 2 * n)
+        N._check(N.lib().frm_lod2d_weights(self.h, n, N._dp(out)), self.ctx.h)
+        return out[0], out[1], out[2], out[3:3 + n], out[3 + n:]
 
     def close(self):
         if getattr(self, "h", None):
