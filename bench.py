@@ -43,10 +43,14 @@ def parse():
     ap.add_argument("--parity-steps", type=int, default=12)
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
-    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c3"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c2", "c3", "c4"],
                     help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
-                         "c3: 4096 psi-Caputo problems, FP64 vs compensated FP32")
+                         "c2: 2D LOD 1024x1024 plain FP64 vs series history; c3: 4096 "
+                         "psi-Caputo problems, FP64 vs compensated FP32; c4: 16 AB sets, plain "
+                         "FP64 vs 64-term series")
     ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
+    ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
+    ap.add_argument("--soe-f32", action="store_true", help="c2/c4: FP32 mode states")
     return ap.parse_args()
 
 
@@ -134,14 +138,99 @@ def run_mt1d(args, cfg_name):
     ctx.close()
 
 
+def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, **kw):
+    from theses_code_amd.solvers import TGSolver
+
+    sN, M, coefs, C0 = case
+    s = TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=K, C1=0.0, S=0.0,
+                 precision=precision, **kw)
+    ctx.timing_reset()
+    ctx.timing_enable((1 << N.K_MEMTERM) | (1 << N.K_ROWS) | (1 << N.K_COLS))
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.calc_step(K)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ctx.timing_enable(0)
+    kt = {name: ctx.timing(k)[1] for name, k in (("memterm", N.K_MEMTERM), ("rows", N.K_ROWS),
+                                                 ("cols", N.K_COLS))}
+    out = s.b_C
+    fit = s.fit_error
+    s.close()
+    return dt, out, kt, fit
+
+
+def run_lod_forms(args, cfg_name):
+    """c2 / c4: plain FP64 full history against the series-history form on the
+    same inputs; reports both rates, the speedup and the error vs FP64."""
+    from theses_code_amd import _native as N
+    from theses_code_amd import workloads as Wl
+
+    ctx = N.Context(0)
+    if cfg_name == "c2":
+        cfg = Wl.C2_WORKLOAD
+        runs = [dict(alpha=cfg["alpha"], case=Wl.lod2d_case(cfg["n_interior"], cfg["seed"],
+                                                            vmax=cfg["vmax"], D=cfg["D"]))]
+        kw = dict(kernel=N.KERNEL_CAPUTO)
+    else:
+        cfg = Wl.C4_WORKLOAD
+        sets = Wl.c4_sets(nsets=args.sets or cfg["nsets"])
+        runs = [dict(alpha=p["alpha"], case=Wl.ab_diffusion_case(p["n_interior"], p["seed"],
+                                                                 D=cfg["D"])) for p in sets]
+        kw = dict(kernel=N.KERNEL_AB, ab_exact=True)
+    K = args.steps if args.steps != 5000 else cfg["steps"]
+    tot_nodes, t_plain, t_soe, errs, rms, fits = 0, 0.0, 0.0, [], [], []
+    kt_plain, kt_soe = {}, {}
+    for r in runs:
+        sN, M = r["case"][0], r["case"][1]
+        nodes = (sN - 1) * (M - 1)
+        tot_nodes += nodes
+        dp, up, kp, _ = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
+                                  N.PREC_F64, **kw)
+        ds, us, ks, fit = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
+                                   N.PREC_SOE, soe_terms=cfg["soe_terms"],
+                                   soe_f32=args.soe_f32, **kw)
+        t_plain += dp
+        t_soe += ds
+        d = us - up
+        errs.append(float(np.max(np.abs(d)) / np.max(np.abs(up))))
+        rms.append(float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(up * up))))
+        fits.append(fit)
+        for k, v in kp.items():
+            kt_plain[k] = kt_plain.get(k, 0.0) + v
+        for k, v in ks.items():
+            kt_soe[k] = kt_soe.get(k, 0.0) + v
+        print(f"# set alpha={r['alpha']:.3f} grid={sN - 1}^2 plain {dp:.3f}s series {ds:.3f}s "
+              f"err {errs[-1]:.2e}", file=sys.stderr, flush=True)
+    line = {"metric": METRIC, "value": tot_nodes * K / t_soe, "unit": "node-steps/s", "n_gpus": 1,
+            "steps": K, "warmup": 0, "ms_per_step": t_soe * 1e3 / K, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 sweeps, series history",
+            "data": "synthetic",
+            "config": {"workload": cfg["name"], "sets": len(runs), "nodes": tot_nodes,
+                       "tau": cfg["tau"], "soe_terms": cfg["soe_terms"],
+                       "soe_states": "fp32" if args.soe_f32 else "fp64", "parallelism": "1 GPU"},
+            "chosen_form": {"form": f"{cfg['soe_terms']}-term sum-of-exponentials history",
+                            "value": tot_nodes * K / t_soe, "seconds": t_soe,
+                            "kernels_ms": kt_soe, "max_fit_rel_err": max(fits),
+                            "max_rel_err_vs_fp64": max(errs), "rms_rel_err_vs_fp64": max(rms)},
+            "plain_fp64": {"value": tot_nodes * K / t_plain, "seconds": t_plain,
+                           "kernels_ms": kt_plain},
+            "speedup_vs_plain_fp64": t_plain / t_soe}
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the history sums shard by node with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-    if args.config != "c1":
+    if args.config in ("c0", "c3"):
         run_mt1d(args, args.config)
+        return
+    if args.config in ("c2", "c4"):
+        run_lod_forms(args, args.config)
         return
 
     from theses_code_amd import _native as N

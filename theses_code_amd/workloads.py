@@ -96,3 +96,37 @@ def mt1d_c3_case(nprob: int = 4096, seed: int = C3_WORKLOAD["seed"], Nmin: int =
         u[n + 1] = u[n]
         U0.append(u)
     return probs, U0
+
+
+# 2D Caputo model at 1024 x 1024, alpha 0.6, 4000 steps (precision study)
+C2_WORKLOAD = dict(name="lod2d_caputo_1024x1024", n_interior=1024, alpha=0.6, tau=1e-3,
+                   steps=4000, sigma=1.0, D=1e-3, vmax=0.1, seed=SEED_BASE + 2, soe_terms=32)
+
+# 16 Atangana-Baleanu diffusion parameter sets (LOD form), sides 1024..2048,
+# 2000 steps, Gaussian random initial fields (correlation length 0.1)
+C4_WORKLOAD = dict(name="lod2d_ab_16sets", nsets=16, side_min=1024, side_max=2048, tau=1e-3,
+                   steps=2000, sigma=1.0, D=1e-3, seed=SEED_BASE + 4, soe_terms=64)
+
+
+def c4_sets(nsets: int = 16, seed: int = C4_WORKLOAD["seed"], side_min: int = 1024,
+            side_max: int = 2048):
+    """Per-set (alpha, n_interior, seed) drawn from seeded sub-streams."""
+    out = []
+    for child in np.random.SeedSequence(seed).spawn(nsets):
+        r = np.random.default_rng(child)
+        out.append(dict(alpha=float(r.uniform(0.3, 0.9)),
+                        n_interior=int(r.integers(side_min, side_max + 1)),
+                        seed=int(r.integers(0, 2 ** 31))))
+    return out
+
+
+def ab_diffusion_case(n_interior: int, seed: int, D: float = 1e-3):
+    """Pure diffusion (no velocity) with a Gaussian random initial field."""
+    rng = np.random.default_rng(seed)
+    sN = M = n_interior + 1
+    h = 1.0 / sN
+    ng = sN + 2
+    C0 = gaussian_random_field(ng, 0.1, h, rng)
+    C0 = 0.5 + 0.1 * C0
+    zero = np.zeros((ng, ng))
+    return sN, M, coefs_advection(zero, zero, D, h), C0
