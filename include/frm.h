@@ -47,7 +47,9 @@ enum {
     FRM_K_COLS = 2,    /* column half-step sweep (+ history push) */
     FRM_K_SWEEP1D = 3, /* batched 1D sweep */
     FRM_K_OTHER = 4,
-    FRM_K_COUNT = 5
+    FRM_K_BLOCKFAR = 5, /* blocked Toeplitz far-part product on MFMA; its
+                           "bytes" are the product's flops (2*128*K*nodes) */
+    FRM_K_COUNT = 6
 };
 
 typedef struct frm_ctx frm_ctx;

@@ -1,0 +1,131 @@
+"""Blocked Toeplitz memory term on MFMA (FP32 increments, split hi+lo BF16
+products with FP32 accumulation over the settled past of each 64-step block,
+FP64 near part) against the full-history FP64 path.
+
+Chosen-form bar: <= 1e-6 max-norm relative error vs FP64 on the same inputs;
+the RMS relative error is asserted alongside so that a drift confined to few
+nodes and a uniform one are both bounded."""
+import numpy as np
+import pytest
+
+from theses_code_amd import _native as N
+from theses_code_amd import solvers as S
+from theses_code_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def errs(a, b):
+    d = a - b
+    return (float(np.max(np.abs(d)) / np.max(np.abs(b))),
+            float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(b * b))))
+
+
+def _case(seed, sN, M, D):
+    rng = np.random.default_rng(seed)
+    nn = (sN + 2) * (M + 2)
+    coefs = S.coefs_advection(rng.uniform(-0.1, 0.1, nn), rng.uniform(-0.1, 0.1, nn), D,
+                              1.0 / (sN + 1))
+    return coefs, rng.uniform(0, 1, nn)
+
+
+def _pair(ctx, sN, M, alpha, steps, seed=5, D=1e-3, kernel=N.KERNEL_CAPUTO, cap=None,
+          smooth=False):
+    if smooth:  # the benchmark fields: curl of a Gaussian random field, Gaussian blobs
+        sN, M, coefs, C0 = W.lod2d_case(sN - 1, seed, D=D)
+        C0 = C0.reshape(-1)
+    else:
+        coefs, C0 = _case(seed, sN, M, D)
+    cap = cap or steps
+    full = S.TGSolver(ctx, sN, M, alpha, 1e-3, 1.0, coefs, C0, capacity=cap, C1=0.0,
+                      kernel=kernel, ab_exact=kernel == N.KERNEL_AB)
+    mf = S.TGSolver(ctx, sN, M, alpha, 1e-3, 1.0, coefs, C0, capacity=cap, C1=0.0,
+                    kernel=kernel, ab_exact=kernel == N.KERNEL_AB, precision=N.PREC_MFMA_BF16)
+    return full, mf
+
+
+# Within the first block only the FP64 near part over FP32 increments runs.
+def test_mfma_first_block_is_fp32_increments(ctx):
+    full, mf = _pair(ctx, 64, 48, 0.7, 63)
+    full.calc_step(63)
+    mf.calc_step(63)
+    mx, rms = errs(mf.b_C, full.b_C)
+    print(f"first block: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-7 and rms <= 1e-7
+    full.close()
+    mf.close()
+
+
+# Smooth velocity fields at cell Peclet number 0.3: with white-noise
+# velocities at Peclet 1.5 the central-difference LOD scheme itself grows,
+# and a growing solution amplifies any rounding difference.  alpha stays
+# above 0.37: below it the row half-step's first history weight
+# kb(3/2, 1) = c (1.5^(1-a) - 0.5^(1-a)) exceeds its current weight
+# kb(1/2, 1/2) = c 0.5^(1-a) (ratio 3^(1-a) - 1), and the half-step scheme
+# amplifies on the FP64 full-history path as well (~1e3 per 10 steps at 0.3).
+@pytest.mark.parametrize("alpha", [0.5, 0.6, 0.9])
+def test_mfma_caputo_vs_full_history(ctx, alpha):
+    steps = 330  # five block boundaries and a partial block
+    full, mf = _pair(ctx, 64, 64, alpha, steps, D=5e-3, smooth=True)
+    full.calc_step(steps)
+    mf.calc_step(steps)
+    mx, rms = errs(mf.b_C, full.b_C)
+    print(f"alpha {alpha}: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-6 and rms <= 1e-6
+    full.close()
+    mf.close()
+
+
+def test_mfma_ab_vs_full_history(ctx):
+    steps = 200
+    full, mf = _pair(ctx, 50, 40, 0.6, steps, seed=6, D=5e-2, kernel=N.KERNEL_AB)
+    full.calc_step(steps)
+    mf.calc_step(steps)
+    mx, rms = errs(mf.b_C, full.b_C)
+    print(f"AB: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-6 and rms <= 1e-6
+    full.close()
+    mf.close()
+
+
+# Steps issued one at a time, in uneven groups, and after a restart give the
+# same bits: the far part depends only on the block start.
+def test_mfma_step_grouping_and_restart(ctx):
+    sN, M, steps = 40, 33, 150
+    coefs, C0 = _case(7, sN, M, 1e-3)
+    a = S.TGSolver(ctx, sN, M, 0.5, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0,
+                   precision=N.PREC_MFMA_BF16)
+    a.calc_step(steps)
+    ra = a.b_C
+    b = S.TGSolver(ctx, sN, M, 0.5, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0,
+                   precision=N.PREC_MFMA_BF16)
+    for k in (1, 62, 1, 1, 64, 21):
+        b.calc_step(k)
+    assert b.tstep == steps
+    assert np.array_equal(b.b_C, ra)
+    b.set_state(C0)
+    b.calc_step(steps)
+    assert np.array_equal(b.b_C, ra)
+    a.close()
+    b.close()
+
+
+# The 512^2 benchmark grid and fields over 2000 steps: long histories (K up
+# to 1984 in the far product).
+def test_mfma_long_history(ctx):
+    sN, steps = 513, 2000
+    full, mf = _pair(ctx, sN, sN, 0.6, steps, seed=9, smooth=True)
+    full.calc_step(steps)
+    mf.calc_step(steps)
+    mx, rms = errs(mf.b_C, full.b_C)
+    print(f"long: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-6 and rms <= 1e-6
+    full.close()
+    mf.close()

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as ct
 import os
+import weakref
 
 import numpy as np
 
@@ -16,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfrm.so")
 FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
 PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE = 0, 1, 2, 3
 KERNEL_CAPUTO, KERNEL_AB = 0, 1
-K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER = 0, 1, 2, 3, 4
+K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER, K_BLOCKFAR = 0, 1, 2, 3, 4, 5
 
 _D, _I, _I64, _U32 = ct.c_double, ct.c_int, ct.c_int64, ct.c_uint32
 _P, _V, _S = ct.POINTER(ct.c_double), ct.c_void_p, ct.c_size_t
@@ -138,6 +139,12 @@ class Context:
         h = _V()
         _check(lib().frm_ctx_create(device, ct.byref(h)))
         self.h = h
+        self._owned = weakref.WeakSet()
+
+    def adopt(self, obj):
+        """Registers an object with a close() that must run before the context
+        is destroyed (solvers hold the context's stream)."""
+        self._owned.add(obj)
 
     def sync(self):
         _check(lib().frm_sync(self.h), self.h)
@@ -182,6 +189,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for obj in list(getattr(self, "_owned", ())):
+                obj.close()
             lib().frm_ctx_destroy(self.h)
             self.h = None
 

@@ -109,8 +109,10 @@ __global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b
 int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
                frm_hist** out) {
     FRM_REQUIRE(ctx, ctx && out && n_nodes > 0 && capacity > 0, "frm_hist_create: bad arguments");
-    FRM_REQUIRE(ctx, precision == FRM_PREC_F64, "frm_hist_create: precision %d not supported",
-                precision);
+    FRM_REQUIRE(ctx, precision == FRM_PREC_F64 || precision == FRM_PREC_MFMA_BF16,
+                "frm_hist_create: precision %d not supported", precision);
+    // FP64 increments, or FP32 increments for the blocked MFMA form
+    const size_t esz = precision == FRM_PREC_F64 ? sizeof(double) : sizeof(float);
     frm_hist* h = new frm_hist();
     h->ctx = ctx;
     h->n_nodes = n_nodes;
@@ -118,7 +120,7 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
     h->capacity = capacity;
     h->precision = precision;
     h->max_chunks = kMaxChunks;
-    size_t bytes = (size_t)capacity * h->ld * sizeof(double);
+    size_t bytes = (size_t)capacity * h->ld * esz;
     hipError_t e = hipMalloc(&h->data, bytes);
     if (e != hipSuccess) {
         delete h;
@@ -184,6 +186,8 @@ extern "C" {
 
 int frm_hist_create(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
                     frm_hist** out) {
+    FRM_REQUIRE(ctx, precision == FRM_PREC_F64, "frm_hist_create: precision %d not supported",
+                precision);
     return hist_alloc(ctx, n_nodes, capacity, precision, out);
 }
 

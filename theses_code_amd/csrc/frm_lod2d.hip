@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "frm_history.hpp"
+#include "frm_mfma.hpp"
 #include "frm_wavescan.hpp"
 #include "frm_soe.hpp"
 #include "frm_weights.hpp"
@@ -54,6 +55,11 @@ struct frm_lod2d {
     double* soe_q = nullptr;   // [L]
     double* soe_c = nullptr;   // [2][L]
     double soe_w1r = 0.0, soe_w1c = 0.0;  // exact lag-1 weights
+    // blocked MFMA form: FP32 increments in hist, BF16-exact weight parts,
+    // far part of the current 64-step block
+    float* whi = nullptr;      // [2][capacity]
+    float* wlo = nullptr;      // [2][capacity]
+    float* F = nullptr;        // [128][ld]
 };
 
 namespace frm {
@@ -131,6 +137,7 @@ struct ColArgs {
     double* C;
     const double* Cprev;
     double* slot;        // history slot for Delta_n (compact node order)
+    float* slotf;        // FP32 history slot instead of slot when non-null
     const double* coef;  // [3][lines][n]
     const double* part;  // target 1 partials at part + ldh
     int64_t ldh;
@@ -180,7 +187,10 @@ __global__ __launch_bounds__(256) void k_lod_cols(ColArgs p) {
         const int e = e0 + k;
         if (e < n) {
             Ccol[(int64_t)(e + 1) * ld] = v[k];
-            p.slot[(int64_t)e * p.NX + (i - 1)] = v[k] - cp[k];
+            if (p.slotf)
+                p.slotf[(int64_t)e * p.NX + (i - 1)] = (float)(v[k] - cp[k]);
+            else
+                p.slot[(int64_t)e * p.NX + (i - 1)] = v[k] - cp[k];
         }
     }
     if (lane == 0) Ccol[0] = p.al1 * v[0] + p.b0;
@@ -250,6 +260,16 @@ static void line_coefs(int n, const double* A, const double* B, int64_t stride, 
     }
 }
 
+// nearest BF16 value (round to nearest even) of x, as a float
+static float bf16_round(double x) {
+    float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
 static int pick_cm(int n) {
     if (n <= 64 * 4) return 4;
     if (n <= 64 * 8) return 8;
@@ -293,6 +313,9 @@ static void lod_free(frm_lod2d* s) {
     hipFree(s->soe_H);
     hipFree(s->soe_q);
     hipFree(s->soe_c);
+    hipFree(s->whi);
+    hipFree(s->wlo);
+    hipFree(s->F);
     delete s;
 }
 
@@ -312,7 +335,9 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: need 0 < alpha < 1 and tau > 0");
     FRM_REQUIRE(ctx, d->rowA && d->rowB && d->colA && d->colB,
                 "frm_lod2d_create: coefficient arrays missing");
-    FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE,
+    FRM_REQUIRE(ctx,
+                d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE ||
+                    d->precision == FRM_PREC_MFMA_BF16,
                 "frm_lod2d_create: precision %d not built", d->precision);
     FRM_REQUIRE(ctx, d->soe_terms >= 0 && d->soe_terms <= 64, "frm_lod2d_create: soe_terms %d",
                 d->soe_terms);
@@ -366,9 +391,11 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     const int64_t nodes = (int64_t)s->NX * s->NY;
     const size_t nn = (size_t)s->ld * (d->M + 2);
     s->precision = d->precision;
-    // the SOE form keeps only a two-slot ring of increments
+    // the SOE form keeps only a two-slot ring of increments, the blocked
+    // MFMA form FP32 increments
+    const bool mf = d->precision == FRM_PREC_MFMA_BF16;
     int rc = hist_alloc(ctx, nodes, d->precision == FRM_PREC_SOE ? kSoeLag0 : d->capacity,
-                        FRM_PREC_F64, &s->hist);
+                        mf ? FRM_PREC_MFMA_BF16 : FRM_PREC_F64, &s->hist);
     if (rc) {
         lod_free(s);
         return rc;
@@ -430,6 +457,22 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
             return check_hip(ctx, e, "hipMalloc(lod2d SOE)");
         }
     }
+    if (mf) {
+        std::vector<float> hi(w.size()), lo(w.size());
+        for (size_t k = 0; k < w.size(); ++k) {
+            hi[k] = bf16_round(w[k]);
+            lo[k] = bf16_round(w[k] - (double)hi[k]);
+        }
+        e = hipMalloc(&s->whi, hi.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&s->wlo, lo.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * kBlockSteps * s->hist->ld);
+        if (e == hipSuccess) e = hipMemcpy(s->whi, hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(s->wlo, lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMalloc(lod2d MFMA)");
+        }
+    }
     e = hipMemcpy(s->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(s->rcoef, rc3.data(), rc3.size() * sizeof(double), hipMemcpyHostToDevice);
@@ -471,6 +514,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     frm_ctx* ctx = s->ctx;
     FRM_REQUIRE(ctx, nsteps >= 0, "frm_lod2d_step: nsteps %d", nsteps);
     const bool soe = s->precision == FRM_PREC_SOE;
+    const bool mf = s->precision == FRM_PREC_MFMA_BF16;
     if (s->tstep + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
                          s->tstep + nsteps, s->capacity);
@@ -494,6 +538,20 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                                    (double*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
                                    s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
             FRM_HIP(ctx, hipGetLastError());
+        } else if (mf) {
+            const int64_t ld = s->hist->ld;
+            const int N0 = n / kBlockSteps * kBlockSteps;
+            const float* h32 = (const float*)s->hist->data;
+            if (N0 > 0 && n == N0) {
+                LaunchTimer t(ctx, FRM_K_BLOCKFAR, 2.0 * 2 * kBlockSteps * N0 * (double)s->hist->n_nodes);
+                if (block_far(ctx->stream, h32, ld, N0, s->whi, s->wlo, s->capacity, s->F))
+                    return check_hip(ctx, hipGetLastError(), "block_far");
+            }
+            nch = 1;
+            LaunchTimer t(ctx, FRM_K_MEMTERM,
+                          (double)ld * ((n - N0) * 4.0 + (N0 > 0 ? 2 * 4 : 0) + 2 * 8));
+            if (block_near(ctx->stream, h32, ld, N0, n, s->w, s->capacity, s->F, N0 > 0, s->hist->part))
+                return check_hip(ctx, hipGetLastError(), "block_near");
         } else {
             int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);
             if (rc) return rc;
@@ -524,6 +582,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             a.Cprev = s->Cprev;
             a.slot = (double*)s->hist->data +
                      (int64_t)(soe ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
+            a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
             a.coef = s->ccoef;
             a.part = s->hist->part;
             a.ldh = s->hist->ld;

@@ -64,6 +64,7 @@ class TGSolver:
         h = N._V()
         N._check(N.lib().frm_lod2d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
         self.h = h
+        ctx.adopt(self)
         self.set_state(C0)
 
     def set_state(self, C: np.ndarray):
@@ -135,6 +136,7 @@ class MT1DBatch:
         h = N._V()
         N._check(N.lib().frm_mt1d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
         self.h = h
+        ctx.adopt(self)
         self.set_state(U0)
 
     def _pack(self, Us):
