@@ -63,14 +63,14 @@ def test_mfma_first_block_is_fp32_increments(ctx):
     mf.close()
 
 
-# Smooth velocity fields at cell Peclet number 0.3: with white-noise
-# velocities at Peclet 1.5 the central-difference LOD scheme itself grows,
-# and a growing solution amplifies any rounding difference.  alpha stays
-# above 0.37: below it the row half-step's first history weight
-# kb(3/2, 1) = c (1.5^(1-a) - 0.5^(1-a)) exceeds its current weight
-# kb(1/2, 1/2) = c 0.5^(1-a) (ratio 3^(1-a) - 1), and the half-step scheme
-# amplifies on the FP64 full-history path as well (~1e3 per 10 steps at 0.3).
-@pytest.mark.parametrize("alpha", [0.5, 0.6, 0.9])
+# Smooth velocity fields at cell Peclet number 0.3 (with white-noise
+# velocities at Peclet 1.5 the central-difference scheme grows), alpha in
+# the range where the half-step LOD scheme is bounded: on this 63^2 case the
+# FP64 full-history solution (which matches the reference's results) grows
+# from max 0.7 to 1.4e4 over 330 steps at alpha = 0.5 and by ~1e5 per 30
+# steps at 0.4, while at 0.55 it decays.  In a growing solution every
+# rounding difference grows with it, so no form can meet a relative bar there.
+@pytest.mark.parametrize("alpha", [0.6, 0.75, 0.9])
 def test_mfma_caputo_vs_full_history(ctx, alpha):
     steps = 330  # five block boundaries and a partial block
     full, mf = _pair(ctx, 64, 64, alpha, steps, D=5e-3, smooth=True)

@@ -43,8 +43,8 @@ struct frm_lod2d {
     double* C = nullptr;       // state, reference layout
     double* Cprev = nullptr;   // C_n at interior nodes, compact node order
     double* w = nullptr;       // [2][capacity] lag-indexed history weights
-    double* rcoef = nullptr;   // rows:    [3][M-1][NX]  a, f, al
-    double* ccoef = nullptr;   // columns: [3][sN-1][NY] a, f, al
+    double* rcoef = nullptr;   // rows:    a, f, al interleaved per line (coef_at)
+    double* ccoef = nullptr;   // columns: the same
     int cm_rows = 0, cm_cols = 0;
     int precision = FRM_PREC_F64;
     std::vector<double> hw;    // host copy: kr0, kc0, kch, then [2][capacity] weights
@@ -78,10 +78,34 @@ static double caputo_kb(double alpha, double tau, double d, double l) {
 }
 
 // ------------------------------------------------------------- kernels
+// Coefficient layout: for line L and array c (0: a, 1: f, 2: al), element
+// e = lane*CM + k of the line (lane's chunk, position k) is stored at
+//   ((L*3 + c)*CM + k)*64 + lane,
+// so that a wavefront's k-th load of a coefficient is 64 consecutive doubles.
+// The state, the partial sums and Cprev are staged through LDS: read in
+// element order (consecutive lanes, consecutive nodes), written to LDS at
+// e + e/CM (chunk-major with one pad double per chunk, so that the chunk
+// reads lane*(CM+1) + k spread over the banks), read back per chunk.
+template <int CM>
+__device__ inline int64_t coef_at(int line, int c, int k, int lane) {
+    return ((int64_t)(line * 3 + c) * CM + k) * kWave + lane;
+}
+
+template <int CM>
+__device__ inline void load_coefs(const double* coef, int line, int lane, int e0, int n,
+                                  double (&a)[CM], double (&f)[CM], double (&al)[CM]) {
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        a[k] = coef[coef_at<CM>(line, 0, k, lane)];
+        f[k] = coef[coef_at<CM>(line, 1, k, lane)];
+        al[k] = coef[coef_at<CM>(line, 2, k, lane)];
+    }
+}
+
 struct RowArgs {
     double* C;
     double* Cprev;
-    const double* coef;  // [3][lines][n]
+    const double* coef;  // interleaved, see coef_at
     const double* part;  // partial sums [nch][2][ldh], target 0
     int64_t ldh;
     int nch;
@@ -89,47 +113,59 @@ struct RowArgs {
     double negsk, S_half, sg, b0, al1;
 };
 
+constexpr int kRowWaves = 4;  // row lines per workgroup
+
 template <int CM>
 __global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
-    const int lane = threadIdx.x & 63;
-    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (line >= p.lines) return;  // whole wavefront exits together
-    const int j = line + 1;
+    constexpr int S = CM + 1;
+    __shared__ double lds[kRowWaves][kWave * S];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int line = blockIdx.x * kRowWaves + wv;
+    const bool live = line < p.lines;  // wave-uniform; dead waves still reach the barriers
     const int n = p.n;
     const int e0 = lane * CM;
-    const double* ca = p.coef + (int64_t)line * n;
-    const double* cf = ca + (int64_t)p.lines * n;
-    const double* cl = cf + (int64_t)p.lines * n;
-    double a[CM], f[CM], al[CM], v[CM];
-    double* Crow = p.C + (int64_t)j * p.ld;
-    const int64_t qbase = (int64_t)(j - 1) * p.NX;
-#pragma unroll
-    for (int k = 0; k < CM; ++k) {
-        const int e = e0 + k;
-        if (e < n) {
-            a[k] = ca[e];
-            f[k] = cf[e];
-            al[k] = cl[e];
-            const double u = Crow[e + 1];
-            p.Cprev[qbase + e] = u;
-            double ts = 0.0;
-            for (int c = 0; c < p.nch; ++c) ts += p.part[(int64_t)c * 2 * p.ldh + qbase + e];
-            double om = p.negsk * u;
-            om += p.S_half;
-            om += p.sg * ts;
-            v[k] = om;
-        } else {
-            a[k] = f[k] = al[k] = v[k] = 0.0;
+    double* buf = lds[wv];
+    double* Crow = p.C + (int64_t)(line + 1) * p.ld;
+    const int64_t qbase = (int64_t)line * p.NX;
+    if (live) {
+        // om in element order: C_n (copied to Cprev) and the history sum
+        for (int r = 0; r < CM; ++r) {
+            const int e = r * kWave + lane;
+            if (e < n) {
+                const double u = Crow[e + 1];
+                p.Cprev[qbase + e] = u;
+                double ts = 0.0;
+                for (int c = 0; c < p.nch; ++c) ts += p.part[(int64_t)c * 2 * p.ldh + qbase + e];
+                double om = p.negsk * u;
+                om += p.S_half;
+                om += p.sg * ts;
+                buf[e + e / CM] = om;
+            }
         }
     }
-    const double un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+    __syncthreads();
+    double a[CM], f[CM], al[CM], v[CM];
+    double un = 0.0;
+    if (live) {
+        load_coefs<CM>(p.coef, line, lane, e0, n, a, f, al);
 #pragma unroll
-    for (int k = 0; k < CM; ++k)
-        if (e0 + k < n) Crow[e0 + k + 1] = v[k];
-    if (lane == 0) Crow[0] = p.al1 * v[0] + p.b0;
-    if (lane == 63) {
-        Crow[n + 1] = un;
-        Crow[n + 2] = un;
+        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? buf[lane * S + k] : 0.0;
+        un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+            if (e0 + k < n) buf[lane * S + k] = v[k];
+    }
+    __syncthreads();
+    if (live) {
+        for (int r = 0; r < CM; ++r) {
+            const int e = r * kWave + lane;
+            if (e < n) Crow[e + 1] = buf[e + e / CM];
+        }
+        if (lane == 0) Crow[0] = p.al1 * v[0] + p.b0;
+        if (lane == 63) {
+            Crow[n + 1] = un;
+            Crow[n + 2] = un;
+        }
     }
 }
 
@@ -138,65 +174,98 @@ struct ColArgs {
     const double* Cprev;
     double* slot;        // history slot for Delta_n (compact node order)
     float* slotf;        // FP32 history slot instead of slot when non-null
-    const double* coef;  // [3][lines][n]
+    const double* coef;  // interleaved, see coef_at
     const double* part;  // target 1 partials at part + ldh
     int64_t ldh;
     int nch;
     int ld, lines, n, NX;
+    int groups;          // column groups of kColW lines
     double negsk, S_half, sg, kch, b0, al1;
 };
 
+// Column lines per workgroup: a row segment of W columns is one contiguous
+// W*8-byte piece of the state, Cprev and partial sums.  W = 8 up to 1024
+// unknowns per line; 4 above, where a lane's registers hold 32 elements.
 template <int CM>
-__global__ __launch_bounds__(256) void k_lod_cols(ColArgs p) {
-    const int lane = threadIdx.x & 63;
-    const int line = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (line >= p.lines) return;
-    const int i = line + 1;
+constexpr int col_width() {
+    return CM >= 32 ? 4 : 8;
+}
+
+template <int CM>
+__global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
+    constexpr int kColW = col_width<CM>();
+    constexpr int S = CM + 1;
+    __shared__ double lds[kColW][kWave * S];
+    // Workgroups are dealt to the 8 XCDs round-robin (b -> XCD b % 8); the
+    // group order below gives each XCD a contiguous run of column groups so
+    // that neighbouring groups, which share cache lines, share an L2.
+    const int per = gridDim.x / 8;
+    const int g = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (g >= p.groups) return;  // whole workgroup, before any barrier
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int line = g * kColW + wv;
+    const bool live = line < p.lines;
     const int n = p.n;
     const int e0 = lane * CM;
-    const double* ca = p.coef + (int64_t)line * n;
-    const double* cf = ca + (int64_t)p.lines * n;
-    const double* cl = cf + (int64_t)p.lines * n;
-    double a[CM], f[CM], al[CM], v[CM], cp[CM];
-    double* Ccol = p.C + i;
     const int64_t ld = p.ld;
-#pragma unroll
-    for (int k = 0; k < CM; ++k) {
-        const int e = e0 + k;
-        if (e < n) {
-            a[k] = ca[e];
-            f[k] = cf[e];
-            al[k] = cl[e];
-            const int64_t q = (int64_t)e * p.NX + (i - 1);  // node (i, j = e+1)
-            const double u = Ccol[(int64_t)(e + 1) * ld];
-            cp[k] = p.Cprev[q];
-            double ts = 0.0;
-            for (int c = 0; c < p.nch; ++c) ts += p.part[((int64_t)c * 2 + 1) * p.ldh + q];
-            ts += p.kch * (cp[k] - u);
-            double om = p.negsk * u;
-            om += p.S_half;
-            om += p.sg * ts;
-            v[k] = om;
-        } else {
-            a[k] = f[k] = al[k] = v[k] = cp[k] = 0.0;
+    // staging map: column tc of the group, rows tr + 64 r
+    const int tc = threadIdx.x % kColW, tr = threadIdx.x / kColW;
+    const int tline = g * kColW + tc;
+    const bool tlive = tline < p.lines;
+    constexpr int kRowsPerPass = kColW * 64 / kColW;  // 64
+    if (tlive) {
+        const int i = tline + 1;
+        for (int r = 0; r < CM; ++r) {
+            const int e = r * kRowsPerPass + tr;
+            if (e < n) {
+                const int64_t q = (int64_t)e * p.NX + (i - 1);  // node (i, j = e+1)
+                const double u = p.C[(int64_t)(e + 1) * ld + i];
+                const double cp = p.Cprev[q];
+                double ts = 0.0;
+                for (int c = 0; c < p.nch; ++c) ts += p.part[((int64_t)c * 2 + 1) * p.ldh + q];
+                ts += p.kch * (cp - u);
+                double om = p.negsk * u;
+                om += p.S_half;
+                om += p.sg * ts;
+                lds[tc][e + e / CM] = om;
+            }
         }
     }
-    const double un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+    __syncthreads();
+    double a[CM], f[CM], al[CM], v[CM];
+    double un = 0.0;
+    if (live) {
+        double* buf = lds[wv];
+        load_coefs<CM>(p.coef, line, lane, e0, n, a, f, al);
 #pragma unroll
-    for (int k = 0; k < CM; ++k) {
-        const int e = e0 + k;
-        if (e < n) {
-            Ccol[(int64_t)(e + 1) * ld] = v[k];
-            if (p.slotf)
-                p.slotf[(int64_t)e * p.NX + (i - 1)] = (float)(v[k] - cp[k]);
-            else
-                p.slot[(int64_t)e * p.NX + (i - 1)] = v[k] - cp[k];
+        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? buf[lane * S + k] : 0.0;
+        un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+            if (e0 + k < n) buf[lane * S + k] = v[k];
+        double* Ccol = p.C + (line + 1);
+        if (lane == 0) Ccol[0] = p.al1 * v[0] + p.b0;
+        if (lane == 63) {
+            Ccol[(int64_t)(n + 1) * ld] = un;
+            Ccol[(int64_t)(n + 2) * ld] = un;
         }
     }
-    if (lane == 0) Ccol[0] = p.al1 * v[0] + p.b0;
-    if (lane == 63) {
-        Ccol[(int64_t)(n + 1) * ld] = un;
-        Ccol[(int64_t)(n + 2) * ld] = un;
+    __syncthreads();
+    if (tlive) {
+        const int i = tline + 1;
+        for (int r = 0; r < CM; ++r) {
+            const int e = r * kRowsPerPass + tr;
+            if (e < n) {
+                const int64_t q = (int64_t)e * p.NX + (i - 1);
+                const double u = lds[tc][e + e / CM];
+                p.C[(int64_t)(e + 1) * ld + i] = u;
+                const double d = u - p.Cprev[q];
+                if (p.slotf)
+                    p.slotf[q] = (float)d;
+                else
+                    p.slot[q] = d;
+            }
+        }
     }
 }
 
@@ -270,6 +339,20 @@ static float bf16_round(double x) {
     return f;
 }
 
+// [3][lines][n] (a, f, al per line) -> the interleaved layout of coef_at
+static std::vector<double> interleave_coefs(const std::vector<double>& plain, int lines, int n,
+                                            int cm) {
+    std::vector<double> out((size_t)lines * 3 * cm * kWave, 0.0);
+    for (int L = 0; L < lines; ++L)
+        for (int c = 0; c < 3; ++c)
+            for (int e = 0; e < n; ++e) {
+                const int lane = e / cm, k = e % cm;
+                out[(((size_t)L * 3 + c) * cm + k) * kWave + lane] =
+                    plain[((size_t)c * lines + L) * n + e];
+            }
+    return out;
+}
+
 static int pick_cm(int n) {
     if (n <= 64 * 4) return 4;
     if (n <= 64 * 8) return 8;
@@ -293,10 +376,10 @@ static hipError_t launch_rows(int cm, dim3 grid, hipStream_t st, const Args& a) 
 template <typename Args>
 static hipError_t launch_cols(int cm, dim3 grid, hipStream_t st, const Args& a) {
     switch (cm) {
-        case 4: hipLaunchKernelGGL(k_lod_cols<4>, grid, dim3(256), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(k_lod_cols<8>, grid, dim3(256), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(k_lod_cols<16>, grid, dim3(256), 0, st, a); break;
-        case 32: hipLaunchKernelGGL(k_lod_cols<32>, grid, dim3(256), 0, st, a); break;
+        case 4: hipLaunchKernelGGL(k_lod_cols<4>, grid, dim3(col_width<4>() * 64), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(k_lod_cols<8>, grid, dim3(col_width<8>() * 64), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(k_lod_cols<16>, grid, dim3(col_width<16>() * 64), 0, st, a); break;
+        case 32: hipLaunchKernelGGL(k_lod_cols<32>, grid, dim3(col_width<32>() * 64), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -403,8 +486,10 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     hipError_t e = hipMalloc(&s->C, nn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->Cprev, nodes * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->w, (size_t)2 * d->capacity * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&s->rcoef, (size_t)3 * (d->M - 1) * s->NX * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&s->ccoef, (size_t)3 * (d->sN - 1) * s->NY * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMalloc(&s->rcoef, (size_t)3 * (d->M - 1) * cmr * kWave * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMalloc(&s->ccoef, (size_t)3 * (d->sN - 1) * cmc * kWave * sizeof(double));
     if (e != hipSuccess) {
         lod_free(s);
         return check_hip(ctx, e, "hipMalloc(lod2d)");
@@ -435,6 +520,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         line_coefs(s->NY, d->colA + base, d->colB + base, s->ld, s->sg * s->kc0, 1.0, oa,
                    oa + lcols * s->NY, oa + 2 * lcols * s->NY);
     }
+    rc3 = interleave_coefs(rc3, (int)lrows, s->NX, cmr);
+    cc3 = interleave_coefs(cc3, (int)lcols, s->NY, cmc);
     s->hw.assign({s->kr0, s->kc0, s->kch});
     s->hw.insert(s->hw.end(), w.begin(), w.end());
     if (d->precision == FRM_PREC_SOE) {
@@ -597,8 +684,10 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             a.kch = s->kch;
             a.b0 = 0.0;
             a.al1 = 1.0;
+            const int cw = s->cm_cols >= 32 ? col_width<32>() : col_width<16>();
+            a.groups = (a.lines + cw - 1) / cw;
             LaunchTimer t(ctx, FRM_K_COLS, 0.0);
-            FRM_HIP(ctx, launch_cols(s->cm_cols, dim3((a.lines + wpb - 1) / wpb), ctx->stream, a));
+            FRM_HIP(ctx, launch_cols(s->cm_cols, dim3((a.groups + 7) / 8 * 8), ctx->stream, a));
         }
         if (!soe) s->hist->depth += 1;
         s->tstep = n + 1;
