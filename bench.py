@@ -50,7 +50,11 @@ def parse():
                          "FP64 vs 64-term series")
     ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
     ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="c3: problems timed through the one-solver-per-problem plain path")
     ap.add_argument("--soe-f32", action="store_true", help="c2/c4: FP32 mode states")
+    ap.add_argument("--breakdown-plain", action="store_true",
+                    help="c2/c4: also time plain FP64 per kernel (a second full run)")
     ap.add_argument("--forms", default="soe,mfma",
                     help="c2/c4: comma-separated chosen forms to time against plain FP64 "
                          "(soe, mfma)")
@@ -117,6 +121,33 @@ def run_mt1d(args, cfg_name):
         results[prec] = (dt, np.concatenate(b.state()), kt)
         b.close()
     dt, U64, kt = results[N.PREC_F64]
+    per_problem = None
+    if cfg_name == "c3" and args.plain_sample > 0:
+        # The plain FP64 drop-in as the reference's callers use it: one solver
+        # object per problem (H_solver per problem), stepped one after the
+        # other on the context's stream -- timed on the first `plain_sample`
+        # problems of the batch (bounded), every step of the run.
+        P = min(args.plain_sample, len(probs))
+        sample_nodes = sum(p["N"] + 1 for p in probs[:P])
+        offs = np.concatenate([[0], np.cumsum([p["N"] + 2 for p in probs])])
+        singles = [MT1DBatch(ctx, [probs[i]], cfg["tau"], K, [U0[i]], kind=kind,
+                             precision=N.PREC_F64, picard=picard) for i in range(P)]
+        ctx.sync()
+        t0 = time.perf_counter()
+        for b in singles:
+            b.calc_step(K)
+        ctx.sync()
+        dts = time.perf_counter() - t0
+        dmax = 0.0
+        for i, b in enumerate(singles):
+            u = b.state()[0]
+            ub = U64[offs[i]:offs[i + 1]]
+            dmax = max(dmax, float(np.max(np.abs(u - ub)) / np.max(np.abs(ub))))
+            b.close()
+        per_problem = {"form": "plain FP64 drop-in, one solver per problem, sequential launches",
+                       "value": sample_nodes * K / dts, "seconds": dts, "problems_timed": P,
+                       "sample_nodes": sample_nodes,
+                       "max_rel_diff_vs_batched_fp64": dmax}
     line = {"metric": METRIC, "value": nodes * K / dt, "unit": "node-steps/s", "n_gpus": 1,
             "steps": K, "warmup": 0, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -132,16 +163,28 @@ def run_mt1d(args, cfg_name):
         dt32, U32, kt32 = results[N.PREC_F32C]
         d = U32 - U64
         line["chosen_form"] = {
-            "form": "FP32 history, compensated FP32 sums, FP64 weights/Thomas/state",
-            "value": nodes * K / dt32, "speedup_vs_fp64": dt / dt32,
+            "form": "batched launch of all problems, FP32 history, compensated FP32 sums, "
+                    "FP64 weights/Thomas/state",
+            "value": nodes * K / dt32, "speedup_vs_batched_fp64": dt / dt32,
             "max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(U64))),
             "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(U64 * U64))),
             "kernels_ms": {k: v[1] for k, v in kt32.items()}}
+        line["batched_fp64"] = {"value": nodes * K / dt, "seconds": dt,
+                                "kernels_ms": line["kernels_ms"]}
+        line["value"] = nodes * K / dt32
+        line["ms_per_step"] = dt32 * 1e3 / K
+        line["dtype"] = "f64 weights/Thomas/state, f32 compensated history"
+        if per_problem:
+            line["plain_fp64_per_problem"] = per_problem
+            line["speedup_vs_plain_fp64"] = line["value"] / per_problem["value"]
     print(json.dumps(line))
     ctx.close()
 
 
-def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, **kw):
+def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
+    """One full run from t = 0.  Wall time is taken with per-launch timing off
+    unless `timing` (then every launch is bracketed by HIP events and the
+    per-kernel totals are returned; the wall time then includes the events)."""
     from theses_code_amd.solvers import TGSolver
 
     sN, M, coefs, C0 = case
@@ -150,7 +193,7 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, **kw):
     classes = (("memterm", N.K_MEMTERM), ("rows", N.K_ROWS), ("cols", N.K_COLS),
                ("blockfar", N.K_BLOCKFAR))
     ctx.timing_reset()
-    ctx.timing_enable(sum(1 << k for _, k in classes))
+    ctx.timing_enable(sum(1 << k for _, k in classes) if timing else 0)
     ctx.sync()
     t0 = time.perf_counter()
     s.calc_step(K)
@@ -158,7 +201,7 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, **kw):
     dt = time.perf_counter() - t0
     ctx.timing_enable(0)
     kt, work = {}, {}
-    for name, k in classes:
+    for name, k in classes if timing else ():
         n, ms, by = ctx.timing(k)
         if n:
             kt[name] = ms
@@ -205,13 +248,19 @@ def run_lod_forms(args, cfg_name):
         tot_nodes += nodes
         dp, up, kp, _, _ = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
                                     **kw, **{"precision": N.PREC_F64})
+        if args.breakdown_plain:
+            kp = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
+                          timing=True, **kw, **{"precision": N.PREC_F64})[2]
         t_plain += dp
         for k, v in kp.items():
             kt_plain[k] = kt_plain.get(k, 0.0) + v
         msg = f"# set alpha={r['alpha']:.3f} grid={sN - 1}^2 plain {dp:.3f}s"
         for f in forms:
-            ds, us, ks, fit, wk = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"],
-                                           cfg["sigma"], K, **kw, **fkw[f])
+            ds, us, _, fit, _ = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"],
+                                         cfg["sigma"], K, **kw, **fkw[f])
+            # per-kernel breakdown from a second, event-timed run
+            _, _, ks, _, wk = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"],
+                                       K, timing=True, **kw, **fkw[f])
             q = res[f]
             q["seconds"] += ds
             d = us - up
@@ -254,7 +303,7 @@ def run_lod_forms(args, cfg_name):
             "chosen_form": dict(out_forms[best], name=best),
             "forms": out_forms,
             "plain_fp64": {"value": tot_nodes * K / t_plain, "seconds": t_plain,
-                           "kernels_ms": kt_plain},
+                           "kernels_ms": kt_plain or None},
             "speedup_vs_plain_fp64": t_plain / out_forms[best]["seconds"]}
     print(json.dumps(line))
     ctx.close()

@@ -162,6 +162,9 @@ typedef struct frm_lod2d_desc {
     int32_t ab_exact;    /* 1: AB weights to FP64 accuracy instead of the MF_int_choice methods */
     int32_t soe_terms;   /* FRM_PREC_SOE: exponentials (0: 32, max 64) */
     int32_t soe_f32;     /* FRM_PREC_SOE: 1 stores the mode states in FP32 */
+    int32_t mfma_parts;  /* FRM_PREC_MFMA_BF16: BF16 parts per operand, 2 (three
+                            products, ~2^-16) or 3 (six products, exact FP32
+                            increments, ~2^-24); 0: 2 for Caputo, 3 for AB */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);

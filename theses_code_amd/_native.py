@@ -43,7 +43,7 @@ class Lod2dDesc(ct.Structure):
                 ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P),
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
                 ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
-                ("soe_f32", ct.c_int32)]
+                ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32)]
 
 
 class MT1DProblem(ct.Structure):

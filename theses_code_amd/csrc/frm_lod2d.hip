@@ -20,6 +20,7 @@
 // With g(t) = t and a time-independent operator the tridiagonal matrices do
 // not change between steps, so their Thomas alpha coefficients are computed
 // once on the host in FP64, with the reference's formula and order.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -57,8 +58,9 @@ struct frm_lod2d {
     double soe_w1r = 0.0, soe_w1c = 0.0;  // exact lag-1 weights
     // blocked MFMA form: FP32 increments in hist, BF16-exact weight parts,
     // far part of the current 64-step block
-    float* whi = nullptr;      // [2][capacity]
-    float* wlo = nullptr;      // [2][capacity]
+    void* atab = nullptr;      // weight-operand fragments (atab_build)
+    int atab_chunks = 0;
+    int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
     float* F = nullptr;        // [128][ld]
 };
 
@@ -396,8 +398,7 @@ static void lod_free(frm_lod2d* s) {
     hipFree(s->soe_H);
     hipFree(s->soe_q);
     hipFree(s->soe_c);
-    hipFree(s->whi);
-    hipFree(s->wlo);
+    hipFree(s->atab);
     hipFree(s->F);
     delete s;
 }
@@ -424,6 +425,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: precision %d not built", d->precision);
     FRM_REQUIRE(ctx, d->soe_terms >= 0 && d->soe_terms <= 64, "frm_lod2d_create: soe_terms %d",
                 d->soe_terms);
+    FRM_REQUIRE(ctx, d->mfma_parts == 0 || d->mfma_parts == 2 || d->mfma_parts == 3,
+                "frm_lod2d_create: mfma_parts %d (2 or 3)", d->mfma_parts);
     FRM_REQUIRE(ctx, d->kernel == FRM_KERNEL_CAPUTO || d->kernel == FRM_KERNEL_AB,
                 "frm_lod2d_create: kernel %d", d->kernel);
     const int cmr = pick_cm(d->sN - 1), cmc = pick_cm(d->M - 1);
@@ -545,16 +548,27 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         }
     }
     if (mf) {
-        std::vector<float> hi(w.size()), lo(w.size());
+        // BF16 parts of the FP64 weights: w = p0 + p1 + p2 + O(2^-24 w)
+        // default parts: two for the Caputo weights (whose products are
+        // dominated by the few large recent lags), three for the bounded AB
+        // kernel, whose slowly decaying weights make the far part a large
+        // share of the memory term
+        const int nparts = d->mfma_parts ? d->mfma_parts : (is_ab ? 3 : 2);
+        s->mfma_nprod = nparts == 2 ? 3 : 6;
+        std::vector<float> parts(3 * w.size());
         for (size_t k = 0; k < w.size(); ++k) {
-            hi[k] = bf16_round(w[k]);
-            lo[k] = bf16_round(w[k] - (double)hi[k]);
+            double r = w[k];
+            for (int p = 0; p < 3; ++p) {
+                parts[p * w.size() + k] = bf16_round(r);
+                r -= (double)parts[p * w.size() + k];
+            }
         }
-        e = hipMalloc(&s->whi, hi.size() * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc(&s->wlo, lo.size() * sizeof(float));
+        s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk);
+        const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
+        e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
         if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * kBlockSteps * s->hist->ld);
-        if (e == hipSuccess) e = hipMemcpy(s->whi, hi.data(), hi.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(s->wlo, lo.data(), lo.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->atab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             lod_free(s);
             return check_hip(ctx, e, "hipMalloc(lod2d MFMA)");
@@ -631,7 +645,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             const float* h32 = (const float*)s->hist->data;
             if (N0 > 0 && n == N0) {
                 LaunchTimer t(ctx, FRM_K_BLOCKFAR, 2.0 * 2 * kBlockSteps * N0 * (double)s->hist->n_nodes);
-                if (block_far(ctx->stream, h32, ld, N0, s->whi, s->wlo, s->capacity, s->F))
+                if (block_far(ctx->stream, h32, ld, N0, s->atab, s->atab_chunks, s->mfma_nprod, s->F))
                     return check_hip(ctx, hipGetLastError(), "block_far");
             }
             nch = 1;

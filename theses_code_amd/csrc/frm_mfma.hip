@@ -12,6 +12,8 @@
 // products hi*hi + hi*lo + lo*hi accumulate in FP32 on
 // v_mfma_f32_16x16x32_bf16; the near part (at most 63 slots) is summed per
 // step in FP64.
+#include <cstring>
+
 #include "frm_mfma.hpp"
 
 namespace frm {
@@ -27,13 +29,24 @@ constexpr int kBpad = 40; // LDS row length of a node's 32 slots (bf16), padded
 // Wavefront w owns row tiles 2w, 2w+1 (16 rows each) and all 4 column
 // tiles; lane l holds A[l%16][8(l/16)+q], B[8(l/16)+q][l%16] and
 // D[4(l/16)+r][l%16] of each 16x16x32 product.
-__global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ hist, int64_t ld,
-                                                   int N0, const float* __restrict__ whi,
-                                                   const float* __restrict__ wlo, int ldw,
-                                                   float* __restrict__ F) {
-    __shared__ __attribute__((aligned(16))) __bf16 sBh[kNt][kBpad];
-    __shared__ __attribute__((aligned(16))) __bf16 sBl[kNt][kBpad];
-    __shared__ float sW[2][2][96];  // [hi/lo][target][y], y = 63 - j + k
+//
+// Operands are split into BF16 parts x = x0 + x1 (+ x2): NP = 3 forms
+// x0 y0 + x0 y1 + x1 y0 (relative error ~2^-16); NP = 6 adds x0 y2 + x2 y0 +
+// x1 y1 and takes each FP32 increment exactly (three 8-bit parts), leaving
+// ~2^-24 -- the FP32 accumulation's own level.
+//
+// The history chunk (32 slots x 64 nodes, FP32) is loaded as float4 (4 nodes
+// of one slot per thread and slot pair), one chunk ahead of the products
+// (register double buffer), split into BF16 parts and transposed to
+// node-major LDS rows [node][slot].  The weight operand of a chunk depends
+// only on d = N0 - s0 (Toeplitz), so its fragments come ready-made from a
+// table built once per solver (atab_build), in register order.
+template <int NP>
+__global__ __launch_bounds__(256, 2) void k_block_far(const float* __restrict__ hist, int64_t ld,
+                                                      int N0, const bf16x8* __restrict__ atab,
+                                                      float* __restrict__ F) {
+    constexpr int NS = NP == 6 ? 3 : 2;  // BF16 parts per operand
+    __shared__ __attribute__((aligned(16))) __bf16 sB[NS][kNt][kBpad];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kg = lane >> 4, lc = lane & 15;
     const int64_t i0 = (int64_t)blockIdx.x * kNt;
@@ -42,51 +55,64 @@ __global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ his
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int c = tid & 63, kq = tid >> 6;  // staging: node c, slots kq*8 .. kq*8+7
+    // loader map: nodes 4*ng .. 4*ng+3 of slots 2*sp, 2*sp+1 of the chunk
+    const int ng = tid & 15, sp = tid >> 4;
+    float4 x0, x1;
+    auto load = [&](int s0) {
+        const int s = s0 + 2 * sp;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        x0 = s < N0 ? *reinterpret_cast<const float4*>(hist + (int64_t)s * ld + i0 + 4 * ng) : z;
+        x1 = s + 1 < N0 ? *reinterpret_cast<const float4*>(hist + (int64_t)(s + 1) * ld + i0 + 4 * ng)
+                        : z;
+    };
+    load(0);
     for (int s0 = 0; s0 < N0; s0 += kKc) {
-        // ---- stage 32 slots x 64 nodes of history as BF16 hi/lo, node-major in LDS
-        bf16x8 hv, lv;
+        // ---- store the loaded chunk: BF16 parts, node-major, slot pairs packed
+        {
+            const float a[4] = {x0.x, x0.y, x0.z, x0.w}, b[4] = {x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int s = s0 + kq * 8 + q;
-            const float x = s < N0 ? hist[(int64_t)s * ld + i0 + c] : 0.f;
-            const __bf16 h = (__bf16)x;
-            hv[q] = h;
-            lv[q] = (__bf16)(x - (float)h);
-        }
-        *reinterpret_cast<bf16x8*>(&sBh[c][kq * 8]) = hv;
-        *reinterpret_cast<bf16x8*>(&sBl[c][kq * 8]) = lv;
-        // ---- lag table of the chunk: lag = N0 - s0 + 63 - y (>= 1 inside the history)
-        for (int e = tid; e < 2 * 95; e += 256) {
-            const int t = e / 95, y = e - t * 95;
-            const int lag = N0 - s0 + 63 - y;
-            const bool ok = lag >= 1 && lag <= ldw;
-            sW[0][t][y] = ok ? whi[(int64_t)t * ldw + lag - 1] : 0.f;
-            sW[1][t][y] = ok ? wlo[(int64_t)t * ldw + lag - 1] : 0.f;
+            for (int q = 0; q < 4; ++q) {
+                float ra = a[q], rb = b[q];
+#pragma unroll
+                for (int p = 0; p < NS; ++p) {
+                    const __bf16 ha = (__bf16)ra, hb = (__bf16)rb;
+                    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+                    bf16x2 v;
+                    v[0] = ha;
+                    v[1] = hb;
+                    *reinterpret_cast<bf16x2*>(&sB[p][4 * ng + q][2 * sp]) = v;
+                    ra -= (float)ha;
+                    rb -= (float)hb;
+                }
+            }
         }
         __syncthreads();
-        bf16x8 bh[4], bl[4];
+        if (s0 + kKc < N0) load(s0 + kKc);  // next chunk in flight during the products
+        bf16x8 bp[NS][4];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            bh[ct] = *reinterpret_cast<const bf16x8*>(&sBh[ct * 16 + lc][kg * 8]);
-            bl[ct] = *reinterpret_cast<const bf16x8*>(&sBl[ct * 16 + lc][kg * 8]);
-        }
+        for (int p = 0; p < NS; ++p)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+                bp[p][ct] = *reinterpret_cast<const bf16x8*>(&sB[p][ct * 16 + lc][kg * 8]);
 #pragma unroll
         for (int rr = 0; rr < 2; ++rr) {
             const int rt = wave * 2 + rr;
-            const int t = rt >> 2, j = (rt & 3) * 16 + lc;
-            bf16x8 ah, al;
+            const bf16x8* at = atab + ((int64_t)((N0 - s0) / kKc - 1) * 8 + rt) * 3 * 64 + lane;
+            bf16x8 ap[NS];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int y = 63 - j + kg * 8 + q;
-                ah[q] = (__bf16)sW[0][t][y];
-                al[q] = (__bf16)sW[1][t][y];
-            }
+            for (int p = 0; p < NS; ++p) ap[p] = at[p * 64];
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
-                acc[rr][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[ct], acc[rr][ct], 0, 0, 0);
-                acc[rr][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[ct], acc[rr][ct], 0, 0, 0);
-                acc[rr][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[ct], acc[rr][ct], 0, 0, 0);
+                f32x4 c = acc[rr][ct];
+                if constexpr (NP == 6) {  // small terms first
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1], bp[1][ct], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[2][ct], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[2], bp[0][ct], c, 0, 0, 0);
+                }
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[1][ct], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1], bp[0][ct], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[0][ct], c, 0, 0, 0);
+                acc[rr][ct] = c;
             }
         }
         __syncthreads();
@@ -136,12 +162,44 @@ __global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hi
     }
 }
 
-int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const float* whi,
-              const float* wlo, int ldw, float* F) {
-    if (ld % kNt || N0 <= 0) return -1;
+int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const void* atab,
+              int atab_chunks, int nprod, float* F) {
+    if (ld % kNt || N0 <= 0 || N0 % kBlockSteps || N0 / kKc > atab_chunks ||
+        (nprod != 3 && nprod != 6))
+        return -1;
     const dim3 grid((unsigned)(ld / kNt)), block(256);
-    hipLaunchKernelGGL(k_block_far, grid, block, 0, st, hist, ld, N0, whi, wlo, ldw, F);
+    const bf16x8* a = (const bf16x8*)atab;
+    if (nprod == 6)
+        hipLaunchKernelGGL(k_block_far<6>, grid, block, 0, st, hist, ld, N0, a, F);
+    else
+        hipLaunchKernelGGL(k_block_far<3>, grid, block, 0, st, hist, ld, N0, a, F);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Host: the weight-operand fragments for d = 32, 64, ..., 32*chunks.  Entry
+// [(d/32 - 1)*8 + rt][p][lane] holds, for row tile rt (target t = rt/4, steps
+// j = 16(rt%4) + lane%16) and k = 8(lane/16) + q, the BF16 part p of
+// W_t(d + j - k) (zero outside lags 1..ldw); W parts given as floats
+// wparts[(p*2 + t)*ldw + lag - 1].
+std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks) {
+    std::vector<uint16_t> out((size_t)chunks * 8 * 3 * 64 * 8);
+    for (int c = 0; c < chunks; ++c) {
+        const int d = (c + 1) * kKc;
+        for (int rt = 0; rt < 8; ++rt)
+            for (int p = 0; p < 3; ++p)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int q = 0; q < 8; ++q) {
+                        const int t = rt >> 2, j = (rt & 3) * 16 + (lane & 15);
+                        const int k = (lane >> 4) * 8 + q;
+                        const int lag = d + j - k;
+                        float v = 0.f;
+                        if (lag >= 1 && lag <= ldw) v = wparts[((size_t)p * 2 + t) * ldw + lag - 1];
+                        uint32_t u;
+                        std::memcpy(&u, &v, 4);  // v is BF16-exact: the top half is its bits
+                        out[((((size_t)c * 8 + rt) * 3 + p) * 64 + lane) * 8 + q] = (uint16_t)(u >> 16);
+                    }
+    }
+    return out;
 }
 
 int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int n, const double* w,

@@ -1,6 +1,7 @@
 // Blocked Toeplitz memory term on MFMA (launchers in frm_mfma.hip).
 #pragma once
 #include <cstdint>
+#include <vector>
 
 #include "frm_internal.hpp"
 
@@ -9,11 +10,17 @@ namespace frm {
 constexpr int kBlockSteps = 64;  // steps that share one far-part product
 constexpr int kFarNodes = 64;    // nodes per far-part workgroup (ld is a multiple)
 
+constexpr int kFarChunk = 32;    // history slots per product k-chunk
+
 // F[t*64 + j][i] = sum_{s < N0} W_t(N0 + j - s) Delta_s[i], t = 0, 1, j < 64,
-// with W_t(lag) = (whi + wlo)[t*ldw + lag - 1] (BF16-exact hi/lo parts) and
-// zero for lags beyond ldw; hist holds FP32 increments with slot stride ld.
-int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const float* whi,
-              const float* wlo, int ldw, float* F);
+// N0 a multiple of 64, from the weight-fragment table of atab_build (which
+// must cover N0/32 chunks); hist holds FP32 increments with slot stride ld.
+// nprod = 3 uses two BF16 parts per operand, 6 uses three.
+int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const void* atab,
+              int atab_chunks, int nprod, float* F);
+// Host: weight-fragment table for chunks d = 32 .. 32*chunks from the BF16
+// parts of the weights, wparts[(p*2 + t)*ldw + lag - 1], p < 3.
+std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks);
 // part[t*ld + i] = F[t*64 + n - N0][i] (if use_F) + sum_{N0 <= s < n} w_t(n - s) Delta_s[i]
 int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int n, const double* w,
                int ldw, const float* F, int use_F, double* part);
