@@ -56,8 +56,8 @@ def parse():
     ap.add_argument("--breakdown-plain", action="store_true",
                     help="c2/c4: also time plain FP64 per kernel (a second full run)")
     ap.add_argument("--forms", default="soe,mfma",
-                    help="c2/c4: comma-separated chosen forms to time against plain FP64 "
-                         "(soe, mfma)")
+                    help="comma-separated chosen forms timed against plain FP64 on the same "
+                         "workload (soe, mfma; empty for none)")
     return ap.parse_args()
 
 
@@ -284,10 +284,13 @@ def run_lod_forms(args, cfg_name):
             o["states"] = "fp32" if args.soe_f32 else "fp64"
             o["max_fit_rel_err"] = max(q["fits"])
         if f == "mfma" and q["kernels_ms"].get("blockfar"):
-            # useful flops of the far-part products (the three BF16 partial
-            # products run 3x these on the matrix cores)
+            # useful flops of the far-part products; the split products run
+            # 3x (two BF16 parts, Caputo default) or 6x (three parts, AB
+            # default) these on the matrix cores
+            nprod = 6 if cfg_name == "c4" else 3
             o["blockfar_tflops"] = q["work"]["blockfar"] / (q["kernels_ms"]["blockfar"] * 1e-3) / 1e12
-            o["blockfar_mfma_tflops"] = 3 * o["blockfar_tflops"]
+            o["blockfar_split_products"] = nprod
+            o["blockfar_mfma_tflops"] = nprod * o["blockfar_tflops"]
         out_forms[f] = o
     ok = [f for f in forms if out_forms[f]["max_rel_err_vs_fp64"] <= 1e-6]
     best = max(ok or forms, key=lambda f: out_forms[f]["value"])
@@ -373,17 +376,44 @@ def main():
     value = nodes * K / dt
 
     achieved = (nbytes / (ms * 1e-3) / 1e9) if ms > 0 else 0.0
+    # HBM bytes per launch: the PMC-measured ratio of counted to algorithmic
+    # bytes of this kernel (profiles/memterm_lod2d_512_traffic.md) times this
+    # run's algorithmic bytes per launch
     traffic = None
     tf = os.path.join(REPO, "profiles", "memterm_lod2d_512_traffic.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and launches:
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            traffic = json.load(f)["traffic_over_algorithmic"] * nbytes / launches
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "k_memterm_f64<2>", "launches": launches,
                 "avg_launch_ms": ms / launches if launches else None,
                 "bytes_per_launch_avg": nbytes / launches if launches else None}
 
+    # the chosen forms on the same workload and inputs, against this plain run
+    chosen = {}
+    if args.forms:
+        s.close()
+        for f in args.forms.split(","):
+            pr = {"mfma": N.PREC_MFMA_BF16, "soe": N.PREC_SOE}[f]
+            sf = TGSolver(ctx, sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], coefs, C0,
+                          capacity=cap, C1=0.0, S=0.0, precision=pr)
+            if W:
+                sf.calc_step(W)
+                ctx.sync()
+                sf.set_state(C0)
+            ctx.sync()
+            t0 = time.perf_counter()
+            sf.calc_step(K)
+            ctx.sync()
+            dtf = time.perf_counter() - t0
+            d = sf.b_C - final
+            chosen[f] = {"form": FORMS[f], "value": nodes * K / dtf, "seconds": dtf,
+                         "speedup_vs_plain_fp64": dt / dtf,
+                         "max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(final))),
+                         "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) /
+                                                      np.sqrt(np.mean(final * final)))}
+            sf.close()
     if not args.no_copy_probe:
         roofline["measured_copy_GBs"] = ctx.copy_bandwidth()
     cpu = None
@@ -413,6 +443,8 @@ def main():
         "parity": parity,
         "final_field_checksum": float(np.sum(final)),
     }
+    if chosen:
+        line["chosen_forms_same_workload"] = chosen
     print(json.dumps(line))
     s.close()
     ctx.close()
