@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--parity-steps", type=int, default=12)
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
-    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c2", "c3", "c4"],
+    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "ab1d"],
                     help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
                          "c2: 2D LOD 1024x1024 plain FP64 vs series history; c3: 4096 "
                          "psi-Caputo problems, FP64 vs compensated FP32; c4: 16 AB sets, plain "
@@ -177,6 +177,91 @@ def run_mt1d(args, cfg_name):
         if per_problem:
             line["plain_fp64_per_problem"] = per_problem
             line["speedup_vs_plain_fp64"] = line["value"] / per_problem["value"]
+    print(json.dumps(line))
+    ctx.close()
+
+
+def run_ab1d(args):
+    """Batched 1D Atangana-Baleanu solves (solve_and_check form): the batch,
+    the one-solver-per-problem drop-in on a sample, and the oracle's CPU solve
+    on a sample; parity of the batch against the oracle on a few problems."""
+    from oracle import oracle as O
+    from theses_code_amd import _native as N
+    from theses_code_amd import workloads as Wl
+    from theses_code_amd.solvers import MT1DBatch
+
+    cfg = Wl.AB1D_WORKLOAD
+    probs = Wl.ab1d_case(nprob=args.nprob or cfg["nprob"])
+    K = args.steps if args.steps != 5000 else cfg["steps"]
+    tau = cfg["tau"]
+    nodes = sum(p["N"] + 1 for p in probs)
+    ctx = N.Context(0)
+    U0 = [np.zeros(p["N"] + 2) for p in probs]
+    t0 = time.perf_counter()
+    b = MT1DBatch(ctx, probs, tau, K, U0, kind=N.MT_AB)
+    setup = time.perf_counter() - t0
+    ctx.timing_reset()
+    ctx.timing_enable((1 << N.K_MEMTERM) | (1 << N.K_SWEEP1D))
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.calc_step(K)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ctx.timing_enable(0)
+    kt = {name: ctx.timing(k) for name, k in (("memterm", N.K_MEMTERM), ("sweep", N.K_SWEEP1D))}
+    states = b.state()
+    b.close()
+    # parity: the first problems against the oracle's solve_and_check
+    par = 0.0
+    for i in range(min(3, len(probs))):
+        p = probs[i]
+        _, Cref = O.ab_solve(p["alpha"], K * tau, K, p["N"], cfg["maxl"], cfg["maxkiter"],
+                             cfg["eps"], p["src"])
+        par = max(par, float(np.max(np.abs(states[i][:p["N"] + 1] - Cref)) / np.max(np.abs(Cref))))
+    # the plain drop-in: one solver per problem, on a sample
+    P = min(args.plain_sample, len(probs))
+    singles = [MT1DBatch(ctx, [probs[i]], tau, K, [U0[i]], kind=N.MT_AB) for i in range(P)]
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s1 in singles:
+        s1.calc_step(K)
+    ctx.sync()
+    dts = time.perf_counter() - t0
+    for s1 in singles:
+        s1.close()
+    sample_nodes = sum(p["N"] + 1 for p in probs[:P])
+    # CPU: the oracle's solve on a bounded sample of problems
+    cpu = None
+    if args.cpu_steps > 0:
+        t0 = time.perf_counter()
+        cn = 0
+        for p in probs[:4]:
+            O.ab_solve(p["alpha"], K * tau, K, p["N"], cfg["maxl"], cfg["maxkiter"], cfg["eps"],
+                       p["src"])
+            cn += p["N"] + 1
+        dcpu = time.perf_counter() - t0
+        cpu = {"value": cn * K / dcpu, "unit": "node-steps/s", "cores": 1, "kind": "port",
+               "sample": f"oracle solve_and_check restatement, first 4 problems, all {K} steps "
+                         "(weights included, as the reference computes them in the run)",
+               "seconds": dcpu}
+    ms, nb = kt["memterm"][1], kt["memterm"][2]
+    line = {"metric": METRIC, "value": nodes * K / dt, "unit": "node-steps/s", "n_gpus": 1,
+            "steps": K, "warmup": 0, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": cfg["name"], "problems": len(probs), "nodes": nodes,
+                       "tau": tau, "parallelism": "1 GPU"},
+            "setup_seconds": setup,
+            "kernels_ms": {k: v[1] for k, v in kt.items()},
+            "roofline": {"bound": "hbm", "achieved": nb / (ms * 1e-3) / 1e9 if ms else 0.0,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
+                         "traffic": None, "kernel": "k_mt_memterm_f64"},
+            "parity": {"vs": "oracle solve_and_check restatement", "problems": 3,
+                       "max_rel_err": par},
+            "plain_fp64_per_problem": {"value": sample_nodes * K / dts, "seconds": dts,
+                                       "problems_timed": P},
+            "speedup_vs_plain_fp64": (nodes * K / dt) / (sample_nodes * K / dts),
+            "cpu_baseline": cpu}
     print(json.dumps(line))
     ctx.close()
 
@@ -323,6 +408,9 @@ def main():
         return
     if args.config in ("c2", "c4"):
         run_lod_forms(args, args.config)
+        return
+    if args.config == "ab1d":
+        run_ab1d(args)
         return
 
     from theses_code_amd import _native as N

@@ -202,7 +202,7 @@ int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out);
  * re-solves the step with coefficients from the latest iterate (1 = the
  * reference's lagged coefficients).  Problem p's nodes are packed at offset
  * sum_{q<p} (N_q + 2) of the state array. */
-enum { FRM_MT_L1 = 0, FRM_MT_PSI = 1 };
+enum { FRM_MT_L1 = 0, FRM_MT_PSI = 1, FRM_MT_AB = 2 };
 enum { FRM_FORM_CONST = 0, FRM_FORM_CV2 = 1, FRM_FORM_KEXP = 2 };
 enum { FRM_TOP_AUTO = 0, FRM_TOP_DIRICHLET = 1, FRM_TOP_FLUX = 2 };
 
@@ -216,11 +216,17 @@ typedef struct frm_mt1d_problem {
     double top_val;      /* Dirichlet value for FRM_TOP_DIRICHLET */
     double Ev;           /* top flux term of Uc() (evaporation minus inflow) */
     double Hbottom;
+    /* kind FRM_MT_AB only (see below) */
+    int32_t src;         /* manufactured solution F_func: 0, 1 or 2 */
+    int32_t ab_maxl;     /* MF_int_choice series terms / quadrature nodes (0: 1000) */
+    int32_t ab_maxkiter; /* inner series limit (0: 1000) */
+    int32_t ab_exact;    /* 1: FP64-accurate kernel integrals */
+    double ab_eps;       /* series tolerance of the current-step weight (0: 1e-10) */
 } frm_mt1d_problem;
 
 typedef struct frm_mt1d_desc {
     int32_t nprob;
-    int32_t kind;      /* FRM_MT_L1 or FRM_MT_PSI */
+    int32_t kind;      /* FRM_MT_L1, FRM_MT_PSI or FRM_MT_AB */
     int32_t capacity;  /* max steps */
     int32_t precision; /* FRM_PREC_F64 or FRM_PREC_F32C */
     int32_t picard;    /* Picard iterations per step, >= 1 */
@@ -235,6 +241,21 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U);
 int frm_mt1d_step(frm_mt1d* s, int32_t nsteps);
 int frm_mt1d_get_state(frm_mt1d* s, double* h_U);
 int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
+
+/* Kind FRM_MT_AB: the Atangana-Baleanu test problem of solve_and_check
+ * (MF_int_series.cpp:875-958, simple = 1) for each problem of the batch:
+ * D_a C = C_xx - C_x + f on [0,1], h = 1/N, C = 0 at t = 0, exact Dirichlet
+ * ends C = CC(x, t), full history of (C^{k+1}-C^k)/tau weighted by
+ * I(d) = MF_int_choice(d tau, tau, 0) (MF_intF_simple, 614-718), current
+ * weight I(1)/(1-alpha) without 1/tau, and the textbook Thomas sweep with
+ * A = -1/h^2 - 1/(2h), B = I(1)/(1-alpha) + 2/h^2, Cc = -1/h^2 + 1/(2h).
+ * f and CC are the reference's manufactured solutions F_func = src
+ * (860-873; their Mittag-Leffler factors are evaluated in FP64 on the host
+ * at creation).  Fields used: N, alpha, src, ab_*; form/top/... ignored;
+ * picard must be 1; precision FRM_PREC_F64.  frm_mt1d_ab_error returns per
+ * problem the reference's error measure sum_steps sum_j (C_j - CC)^2 /
+ * ((N+1) steps) over the steps taken since the last set_state. */
+int frm_mt1d_ab_error(frm_mt1d* s, double* out);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,13 @@ class H_batch {
         }
     }
 
+    // kind FRM_MT_AB: solve_and_check's error measure per problem
+    std::vector<double> ab_error() {
+        std::vector<double> e(b_U.size());
+        check(frm_mt1d_ab_error(s_, e.data()), ctx_.get());
+        return e;
+    }
+
     int tstep = 1;  // the reference's H_solver starts at tstep = 1
     std::vector<std::vector<double>> b_U;
 

@@ -49,7 +49,9 @@ class Lod2dDesc(ct.Structure):
 class MT1DProblem(ct.Structure):
     _fields_ = [("N", ct.c_int32), ("form", ct.c_int32), ("top", ct.c_int32),
                 ("bottom_cond", ct.c_int32), ("L", _D), ("alpha", _D), ("beta", _D), ("D", _D),
-                ("k", _D), ("k0", _D), ("b", _D), ("top_val", _D), ("Ev", _D), ("Hbottom", _D)]
+                ("k", _D), ("k0", _D), ("b", _D), ("top_val", _D), ("Ev", _D), ("Hbottom", _D),
+                ("src", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
+                ("ab_exact", ct.c_int32), ("ab_eps", _D)]
 
 
 class MT1DDesc(ct.Structure):
@@ -58,7 +60,7 @@ class MT1DDesc(ct.Structure):
                 ("probs", ct.POINTER(MT1DProblem))]
 
 
-MT_L1, MT_PSI = 0, 1
+MT_L1, MT_PSI, MT_AB = 0, 1, 2
 FORM_CONST, FORM_CV2, FORM_KEXP = 0, 1, 2
 TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
 
@@ -99,6 +101,7 @@ SIGNATURES = {
     "frm_mt1d_step": (_I, [_V, ct.c_int32]),
     "frm_mt1d_get_state": (_I, [_V, _P]),
     "frm_mt1d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_mt1d_ab_error": (_I, [_V, _P]),
 }
 
 _lib = None

@@ -18,10 +18,13 @@
 //      the push of the new increment.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "frm_history.hpp"
 #include "frm_wavescan.hpp"
+#include "frm_weights.hpp"
 
 namespace frm {
 
@@ -30,7 +33,7 @@ constexpr int kMtCM = 16;         // unknowns per lane in the sweep (N - 1 <= 10
 constexpr int kMtMaxN = 64 * kMtCM + 1;
 
 struct MtProbDev {
-    int N, form, top, bottom_cond;
+    int N, form, top, bottom_cond, src;
     int64_t uoff, hoff;
     double dL, Dg, Da, alpha, beta, D, k, k0, b, top_val, Ev, Hbottom;
 };
@@ -437,6 +440,104 @@ __global__ __launch_bounds__(256) void k_mt_sweep(MtSweepArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- AB kind
+// The manufactured solutions of MF_int_series.cpp:860-873 (F_func = src);
+// E is the Mittag-Leffler factor of the source at time t.
+constexpr double kAbPi = 3.14159265358979323846;
+
+__host__ __device__ inline double ab_CC(int F, double x, double t) {
+    if (F == 0) return t * t * t * t;
+    if (F == 1) return x * (x - 1.0) * t * t;
+    if (F == 2) return t * t * t * t * t * sin(kAbPi * x);
+    return 0.0;
+}
+
+__device__ inline double ab_f(int F, double x, double t, double a, double E) {
+    if (F == 0) return (24.0 / (1.0 - a)) * t * t * t * t * E;
+    if (F == 1) return (2.0 * x - 1.0) * t * t - 2 * t * t + (2.0 / (1.0 - a)) * x * (x - 1.0) * t * t * E;
+    if (F == 2)
+        return (120.0 / (1.0 - a)) * t * t * t * t * t * sin(kAbPi * x) * E +
+               kAbPi * t * t * t * t * t * (kAbPi * sin(kAbPi * x) + cos(kAbPi * x));
+    return 0.0;
+}
+
+struct AbSweepArgs {
+    const MtProbDev* pr;
+    int nprob;
+    double* U;
+    const double* part;  // history sums sum_k I(lag+1) (C^{k+1}-C^k)/tau, [nch][ldh]
+    int nch;
+    int64_t ldh;
+    const double* d0;    // I(1)/(1-alpha) per problem
+    const double* cf;    // per history node: -1/(B - A Al_{j-1})
+    const double* cal;   // per history node: -Al_j
+    const double* ml;    // [nprob][ldml] source factor at step k
+    int ldml;
+    int k;               // step being taken (1-based), t = k tau
+    double tau;
+    double* slot;        // history slot for (C^k - C^{k-1})/tau
+    double* err;         // per problem: running sum of (C_j - CC)^2
+};
+
+// One wavefront per problem: right-hand side Rp_j = I(1)/(1-a) C_j + f -
+// history/(1-a) (MF_int_series.cpp:922-931), the Thomas sweep with the
+// exact Dirichlet ends (935-945) through the scan-seeded line solve (the
+// constant alpha recurrence is precomputed), the increment push and the
+// squared error against CC (947-951).
+template <int CM>
+__global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + wv;
+    if (p >= a.nprob) return;  // whole wavefront; the kernel has no block barrier
+    const MtProbDev P = a.pr[p];
+    const int m = P.N, n = m - 1, e0 = lane * CM;
+    const double h = P.dL, al = P.alpha;
+    const double t = a.k * a.tau;
+    const double E = a.ml[(int64_t)p * a.ldml + (a.k - 1)];
+    const double Ii1a = a.d0[p];
+    const double A = -(1.0 / (h * h)) - (1.0 / (2.0 * h));
+    double* C = a.U + P.uoff;
+    double ca[CM], cf[CM], cal[CM], v[CM], c0[CM];
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        ca[k] = cf[k] = cal[k] = v[k] = c0[k] = 0.0;
+        if (e < n) {
+            const int j = e + 1;
+            c0[k] = C[j];
+            double hs = 0.0;
+            for (int c = 0; c < a.nch; ++c) hs += a.part[(int64_t)c * a.ldh + P.hoff + e];
+            double rp = Ii1a * c0[k] + ab_f(P.src, j * h, t, al, E);
+            if (a.k >= 2) rp -= hs / (1.0 - al);
+            v[k] = rp;
+            ca[k] = A;
+            cf[k] = a.cf[P.hoff + e];
+            cal[k] = a.cal[P.hoff + e];
+        }
+    }
+    const double b0 = ab_CC(P.src, 0.0, t), un = ab_CC(P.src, m * h, t);
+    line_solve<CM>(ca, cf, cal, v, e0, n, b0, lane, &un);
+    double es = 0.0;
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        const int e = e0 + k;
+        if (e < n) {
+            const int j = e + 1;
+            C[j] = v[k];
+            if (a.slot) a.slot[P.hoff + e] = (v[k] - c0[k]) / a.tau;
+            const double d = fabs(v[k] - ab_CC(P.src, h * j, t));
+            es += d * d;
+        }
+    }
+    if (lane == 0) {
+        C[0] = b0;  // bt[0] - al[0] C[1] with al[0] = 0
+        C[m] = un;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) es += __shfl_xor(es, o);
+    if (lane == 0) a.err[p] += es;  // the two ends equal CC exactly
+}
+
 }  // namespace frm
 
 using namespace frm;
@@ -461,6 +562,11 @@ struct frm_mt1d {
     double* d0 = nullptr;     // per-problem current-step coefficient
     int* bad = nullptr;       // set when a psi weight is not finite
     int ldw = 0;
+    // kind AB
+    double* ab_cf = nullptr;  // [ldh] -1/(B - A Al_{j-1})
+    double* ab_cal = nullptr; // [ldh] -Al_j
+    double* ab_ml = nullptr;  // [nprob][capacity] source factors
+    double* ab_err = nullptr; // [nprob]
 };
 
 static void mt_free(frm_mt1d* s) {
@@ -474,7 +580,86 @@ static void mt_free(frm_mt1d* s) {
     hipFree(s->w32);
     hipFree(s->d0);
     hipFree(s->bad);
+    hipFree(s->ab_cf);
+    hipFree(s->ab_cal);
+    hipFree(s->ab_ml);
+    hipFree(s->ab_err);
     delete s;
+}
+
+// Kind AB: kernel integrals I(d) (history weights I(lag + 1), current
+// I(1)/(1-a)), the constant Thomas alpha recurrence of solve_and_check
+// (MF_int_series.cpp:906-940) and the Mittag-Leffler factors of the source
+// for every step, all in FP64 on the host.
+static int ab_setup(frm_ctx* ctx, frm_mt1d* s, const frm_mt1d_desc* d) {
+    const int cap = s->capacity;
+    std::vector<double> w((size_t)s->nprob * s->ldw, 0.0), d0(s->nprob);
+    std::vector<double> cf(s->ldh, 0.0), cal(s->ldh, 0.0), ml((size_t)s->nprob * cap);
+    // problems with the same order and weight method share I(1..cap+1), and
+    // with the same order and source the Mittag-Leffler factors
+    std::map<std::tuple<double, int, int, double, int>, std::vector<double>> icache;
+    std::map<std::pair<double, int>, std::vector<double>> mcache;
+    for (int p = 0; p < s->nprob; ++p) {
+        const frm_mt1d_problem& q = d->probs[p];
+        const MtProbDev& P = s->hprobs[p];
+        const double a = q.alpha, tau = d->tau;
+        const int maxl = q.ab_maxl > 0 ? q.ab_maxl : 1000;
+        const int maxk = q.ab_maxkiter > 0 ? q.ab_maxkiter : 1000;
+        const double eps = q.ab_eps > 0 ? q.ab_eps : 1e-10;
+        const auto key = std::make_tuple(a, maxl, maxk, eps, q.ab_exact ? 1 : 0);
+        auto it = icache.find(key);
+        if (it == icache.end()) {
+            std::vector<double> I(cap + 1);  // I[d-1] = I(d)
+            if (q.ab_exact) {
+                ab_exact_weights(a, tau, cap + 1, I.data());
+            } else {
+                // the current weight takes the problem's eps, the history
+                // 1e-10 (MF_int_series.cpp:917 and the cached I of
+                // MF_intF_simple)
+                AbIntegrator cur(a, maxl, maxk, eps);
+                AbIntegrator hist(a, maxl, maxk, 1e-10);
+                I[0] = cur.integral(tau, tau, 0.0);
+                for (int dd = 2; dd <= cap + 1; ++dd) I[dd - 1] = hist.integral(dd * tau, tau, 0.0);
+            }
+            it = icache.emplace(key, std::move(I)).first;
+        }
+        const std::vector<double>& I = it->second;
+        const double Ii = I[0];
+        for (int l = 1; l <= s->ldw; ++l) w[(size_t)p * s->ldw + l - 1] = I[l];
+        d0[p] = Ii / (1.0 - a);
+        const int m = P.N;
+        const double h = 1.0 / m;
+        const double A = -(1.0 / (h * h)) - (1.0 / (2.0 * h));
+        const double B = (Ii / (1.0 - a)) + (2.0 / (h * h));
+        const double Cc = -(1.0 / (h * h)) + (1.0 / (2.0 * h));
+        double al = 0.0;
+        for (int j = 1; j < m; ++j) {
+            const double den = B - A * al;
+            al = Cc / den;
+            cf[P.hoff + j - 1] = -1.0 / den;
+            cal[P.hoff + j - 1] = -al;
+        }
+        const double beta = q.src == 0 ? 5.0 : (q.src == 1 ? 3.0 : 6.0);
+        auto mt = mcache.find({a, q.src});
+        if (mt == mcache.end()) {
+            std::vector<double> v(cap);
+            for (int k = 1; k <= cap; ++k)
+                v[k - 1] = ml_eval(a, beta, (-a / (1.0 - a)) * std::pow(k * tau, a), 1e-20);
+            mt = mcache.emplace(std::make_pair(a, q.src), std::move(v)).first;
+        }
+        std::copy(mt->second.begin(), mt->second.end(), ml.begin() + (size_t)p * cap);
+    }
+    hipError_t e = hipMalloc(&s->ab_cf, sizeof(double) * s->ldh);
+    if (e == hipSuccess) e = hipMalloc(&s->ab_cal, sizeof(double) * s->ldh);
+    if (e == hipSuccess) e = hipMalloc(&s->ab_ml, sizeof(double) * ml.size());
+    if (e == hipSuccess) e = hipMalloc(&s->ab_err, sizeof(double) * s->nprob);
+    if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d0, d0.data(), sizeof(double) * d0.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->ab_cf, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->ab_cal, cal.data(), sizeof(double) * cal.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->ab_ml, ml.data(), sizeof(double) * ml.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->ab_err, 0, sizeof(double) * s->nprob);
+    return e == hipSuccess ? FRM_OK : check_hip(ctx, e, "mt1d AB setup");
 }
 
 extern "C" {
@@ -484,8 +669,10 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     *out = nullptr;
     FRM_REQUIRE(ctx, d->nprob >= 1 && d->capacity >= 1 && d->picard >= 1 && d->tau > 0,
                 "frm_mt1d_create: bad batch descriptor");
-    FRM_REQUIRE(ctx, d->kind == FRM_MT_L1 || d->kind == FRM_MT_PSI, "frm_mt1d_create: kind %d",
-                d->kind);
+    FRM_REQUIRE(ctx, d->kind == FRM_MT_L1 || d->kind == FRM_MT_PSI || d->kind == FRM_MT_AB,
+                "frm_mt1d_create: kind %d", d->kind);
+    FRM_REQUIRE(ctx, d->kind != FRM_MT_AB || (d->picard == 1 && d->precision == FRM_PREC_F64),
+                "frm_mt1d_create: kind AB takes picard 1 and FRM_PREC_F64");
     FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_F32C,
                 "frm_mt1d_create: precision %d", d->precision);
     FRM_HIP(ctx, hipSetDevice(ctx->device));
@@ -497,14 +684,17 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     s->precision = d->precision;
     s->picard = d->picard;
     s->tau = d->tau;
-    s->slots = std::max(1, d->capacity - 2);  // increments enter from step 3 on
+    // L1/psi: increments enter from step 3 on; AB: every increment
+    const bool is_ab = d->kind == FRM_MT_AB;
+    s->slots = is_ab ? d->capacity : std::max(1, d->capacity - 2);
     s->ldw = s->slots;
     std::vector<int> gp;
     int64_t uoff = 0, hoff = 0;
     for (int p = 0; p < d->nprob; ++p) {
         const frm_mt1d_problem& q = d->probs[p];
         if (q.N < 2 || q.N > kMtMaxN || !(q.alpha > 0 && q.alpha < 1) || !(q.L > 0) ||
-            q.form < 0 || q.form > 2 || q.top < 0 || q.top > 2) {
+            q.form < 0 || q.form > 2 || q.top < 0 || q.top > 2 ||
+            (is_ab && (q.src < 0 || q.src > 2))) {
             mt_free(s);
             return set_error(ctx, FRM_E_INVALID, "frm_mt1d_create: problem %d invalid (N=%d)", p,
                              q.N);
@@ -514,9 +704,10 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         P.form = q.form;
         P.top = q.top;
         P.bottom_cond = q.bottom_cond;
+        P.src = q.src;
         P.uoff = uoff;
         P.hoff = hoff;
-        P.dL = q.L / q.N;
+        P.dL = is_ab ? 1.0 / q.N : q.L / q.N;
         P.Dg = std::pow(P.dL, -2) / 1.0;  // Gamma(2 - gamma) = Gamma(1) = 1
         P.alpha = q.alpha;
         P.beta = q.beta;
@@ -583,6 +774,13 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         mt_free(s);
         return check_hip(ctx, e, "mt1d setup copies");
     }
+    if (is_ab) {
+        const int rc = ab_setup(ctx, s, d);
+        if (rc) {
+            mt_free(s);
+            return rc;
+        }
+    }
     *out = s;
     return FRM_OK;
 }
@@ -601,6 +799,10 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->tstep = 1;
     s->depth = 0;
+    if (s->ab_err) {
+        FRM_HIP(ctx, hipMemsetAsync(s->ab_err, 0, sizeof(double) * s->nprob, ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return FRM_OK;
 }
 
@@ -643,6 +845,33 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
                                    chunk, s->part);
             FRM_HIP(ctx, hipGetLastError());
         }
+        if (s->kind == FRM_MT_AB) {
+            AbSweepArgs b;
+            b.pr = s->probs;
+            b.nprob = s->nprob;
+            b.U = s->U;
+            b.part = s->part;
+            b.nch = nch;
+            b.ldh = s->ldh;
+            b.d0 = s->d0;
+            b.cf = s->ab_cf;
+            b.cal = s->ab_cal;
+            b.ml = s->ab_ml;
+            b.ldml = s->capacity;
+            b.k = k;
+            b.tau = s->tau;
+            b.slot = (double*)s->hist + (size_t)depth * s->ldh;
+            b.err = s->ab_err;
+            {
+                LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
+                hipLaunchKernelGGL(k_ab1d_sweep<kMtCM>, dim3((unsigned)((s->nprob + 3) / 4)), dim3(256),
+                                   0, ctx->stream, b);
+                FRM_HIP(ctx, hipGetLastError());
+            }
+            s->depth = depth + 1;
+            s->tstep = k + 1;
+            continue;
+        }
         MtSweepArgs a;
         a.pr = s->probs;
         a.nprob = s->nprob;
@@ -677,6 +906,20 @@ int frm_mt1d_get_state(frm_mt1d* s, double* h_U) {
         return set_error(ctx, FRM_E_STATE,
                          "frm_mt1d: a psi weight series did not return a finite value (the "
                          "reference falls back to adaptive quadrature there)");
+    return FRM_OK;
+}
+
+int frm_mt1d_ab_error(frm_mt1d* s, double* out) {
+    if (!s || !out) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_mt1d_ab_error: null");
+    frm_ctx* ctx = s->ctx;
+    FRM_REQUIRE(ctx, s->kind == FRM_MT_AB, "frm_mt1d_ab_error: batch kind %d is not AB", s->kind);
+    std::vector<double> e(s->nprob);
+    FRM_HIP(ctx, hipMemcpyAsync(e.data(), s->ab_err, sizeof(double) * s->nprob, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const int steps = s->tstep - 1;
+    for (int p = 0; p < s->nprob; ++p)
+        out[p] = steps > 0 ? e[p] / ((s->hprobs[p].N + 1.0) * steps) : 0.0;
     return FRM_OK;
 }
 

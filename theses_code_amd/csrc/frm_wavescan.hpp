@@ -52,11 +52,12 @@ __device__ inline void scan_affine_down(double& P, double& Q, int lane) {
 //   backward: u_e  = al_e * u_{e+1} + bt_e,           u_n = bt_{n-1} / (1 - al_{n-1})
 // (fr_d_t_g_2d.cpp:3494-3553 and 2548-2550 with f_e = Al[i+1]/B(i)).
 // On entry v[] holds om for the lane's chunk; on exit it holds u.
-// Returns u_n (the value at i = N) to every lane.
+// Returns u_n (the value at i = N) to every lane; with un_fixed the caller
+// gives u_n (a Dirichlet end) instead.
 template <int CM>
 __device__ inline double line_solve(const double (&a)[CM], const double (&f)[CM],
                                     const double (&al)[CM], double (&v)[CM], int e0, int n,
-                                    double b0, int lane) {
+                                    double b0, int lane, const double* un_fixed = nullptr) {
     // ---- forward: compose chunk maps x -> p x + q, p = f a, q = -f om
     double P = 1.0, Q = 0.0;
 #pragma unroll
@@ -83,7 +84,7 @@ __device__ inline double line_solve(const double (&a)[CM], const double (&f)[CM]
 #pragma unroll
     for (int k = 0; k < CM; ++k)
         if (e0 + k == n - 1) un_local = v[k] / (1.0 - al[k]);
-    const double un = __shfl(un_local, last);
+    const double un = un_fixed ? *un_fixed : __shfl(un_local, last);
     // ---- backward: compose y -> al y + bt from the top of the chunk down
     P = 1.0;
     Q = 0.0;

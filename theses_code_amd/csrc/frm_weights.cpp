@@ -8,6 +8,7 @@
 // after ten consecutive terms below eps/l!, so its value depends on the
 // operation order; the arithmetic below follows the reference's order so
 // that the GPU path sees the reference's weights.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -75,6 +76,52 @@ double cody_gamma(double x) {
 }
 
 double sgn(double x) { return x > 0.0 ? 1.0 : -1.0; }
+
+// Pair sums ordered by decreasing magnitude, then added neighbour by
+// neighbour, level by level; an odd leftover joins the last new sum (a
+// single value is therefore doubled, as in sum_sort_pyramid,
+// MF_int_series.cpp:200-217).
+double pyramid_sum(std::vector<double> v) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end(), [](double x, double y) { return std::fabs(x) > std::fabs(y); });
+    size_t lo = 0, hi = 0;
+    do {
+        lo = hi;
+        hi = v.size();
+        const size_t odd = (hi - lo) % 2;
+        for (size_t i = lo; i < hi - odd; i += 2) v.push_back(v[i] + v[i + 1]);
+        if (odd) v.back() += v[hi - 1];
+    } while (v.size() - hi > 1);
+    return v.back();
+}
+
+// Integrand of the integral representation of E_{a,b}(z), z < 0.
+double ml_kernel(double a, double b, double z, double r) {
+    const double k1 = (1.0 / (kPi * a)) * std::pow(r, (1 - b) / a) * std::exp(-std::pow(r, 1.0 / a));
+    const double k2 = r * std::sin(kPi * (1 - b)) - z * std::sin(kPi * (1 - b + a));
+    const double k3 = r * r - 2 * r * z * std::cos(kPi * a) + z * z;
+    return k1 * k2 / k3;
+}
+
+// Endpoint average plus sd equally spaced interior points, with sd doubled
+// until two estimates differ by less than eps (or sd reaches 1000).
+double ml_quadrature(double a, double b, double z, double r0, double r1, double eps) {
+    int sd = 1;
+    double prev = 0.0;
+    for (;;) {
+        const double ends = 0.5 * (r1 - r0) * (ml_kernel(a, b, z, r0) + ml_kernel(a, b, z, r1));
+        double v = 0.0;
+        for (int i = 1; i <= sd; i++) v += ml_kernel(a, b, z, r0 + (i / ((double)(sd + 1))) * (r1 - r0));
+        v *= (r1 - r0);
+        if (std::isfinite(ends)) v += ends;
+        v /= (double)(sd + 1);
+        if (sd >= 1000) return v;
+        if (sd == 1) prev = ends;
+        if (std::fabs(v - prev) < eps) return v;
+        sd *= 2;
+        prev = v;
+    }
+}
 
 double spectral(double a, double v) {  // Fv of the integral representation
     const double p = -a / (1.0 - a);
@@ -249,6 +296,42 @@ void ab_exact_weights(double a, double tau, int D, double* out) {
         }
         out[d - 1] = s;
     }
+}
+
+}  // namespace frm
+
+namespace frm {
+
+double ml_eval(double a, double b, double z, double eps) {
+    int k0 = (int)(std::log(eps * (1 - z)) / std::log(z));
+    const int kmin = 1 + (int)((1 - b) / a);
+    if (z >= 1.0 || z < 0.0) k0 = (int)(-2.0 * std::log(eps));
+    if (k0 < kmin) k0 = kmin;
+    if (z > 0) {
+        double res = 0.0, zz = 1.0;
+        for (int i = 0; i < k0; i++) {
+            const double v = zz / cody_gamma(b + a * i);
+            if (!std::isfinite(v)) break;
+            res += v;
+            zz *= z;
+        }
+        return res;
+    }
+    if (std::fabs(z) > kPi * a && b < 1 + a && a > 0 && a < 1) {
+        const double r1 = std::max({1.0, 2.0 * std::fabs(z), std::pow(-std::log(kPi * eps / 6.0), a)});
+        return ml_quadrature(a, b, z, 0.0, r1, eps);
+    }
+    std::vector<double> pairs;
+    double zz = 1.0;
+    for (int i = 0; i < k0; i += 2) {
+        const double v1 = zz / cody_gamma(b + a * i);
+        zz *= z;
+        const double v2 = zz / cody_gamma(b + a * (i + 1));
+        zz *= z;
+        if (!std::isfinite(v1 + v2)) break;
+        pairs.push_back(v1 + v2);
+    }
+    return pyramid_sum(pairs);
 }
 
 }  // namespace frm

@@ -21,6 +21,12 @@ class AbIntegrator {
     std::vector<double> logi_, lgk_, prodlog_, prodsgn_;
 };
 
+// E_{a,b}(z) by the method of mittag_leffler (MF_int_series.cpp:247-302):
+// the power series for z > 0; for z <= 0 the integral representation when
+// |z| > pi a (0 < a < 1, b < 1 + a), else the series summed in pairs, the
+// pair sums ordered by decreasing magnitude and added level by level.
+double ml_eval(double a, double b, double z, double eps);
+
 // Accurate FP64 integrals I(d) = int_{(d-1)tau}^{d tau} E_a(-a/(1-a) u^a) du,
 // d = 1..D (series for d = 1, spectral representation otherwise).
 void ab_exact_weights(double a, double tau, int D, double* out);

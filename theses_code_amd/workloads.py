@@ -130,3 +130,23 @@ def ab_diffusion_case(n_interior: int, seed: int, D: float = 1e-3):
     C0 = 0.5 + 0.1 * C0
     zero = np.zeros((ng, ng))
     return sN, M, coefs_advection(zero, zero, D, h), C0
+
+
+# Batch of 1D Atangana-Baleanu test problems (solve_and_check form): 1024
+# problems, N uniform in [64, 1024] intervals, orders from 16 values in
+# [0.3, 0.9], manufactured solutions F_func uniform in {0, 1, 2}, tau = 1/2000,
+# 2000 steps (T = 1); MF_int_choice weights with 50 series terms.
+AB1D_WORKLOAD = dict(name="ab1d_batch1024", nprob=1024, Nmin=64, Nmax=1024, tau=1.0 / 2000,
+                     steps=2000, maxl=50, maxkiter=1000, eps=1e-10, seed=SEED_BASE + 5)
+
+
+def ab1d_case(nprob: int = AB1D_WORKLOAD["nprob"], seed: int = AB1D_WORKLOAD["seed"]):
+    rng = np.random.default_rng(seed)
+    alphas = np.linspace(0.3, 0.9, 16)
+    probs = []
+    for _ in range(nprob):
+        probs.append(dict(N=int(rng.integers(AB1D_WORKLOAD["Nmin"], AB1D_WORKLOAD["Nmax"] + 1)),
+                          alpha=float(alphas[rng.integers(0, 16)]), src=int(rng.integers(0, 3)),
+                          ab_maxl=AB1D_WORKLOAD["maxl"], ab_maxkiter=AB1D_WORKLOAD["maxkiter"],
+                          ab_eps=AB1D_WORKLOAD["eps"]))
+    return probs
