@@ -55,18 +55,39 @@ __global__ __launch_bounds__(256, 2) void k_block_far(const float* __restrict__ 
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // loader map: nodes 4*ng .. 4*ng+3 of slots 2*sp, 2*sp+1 of the chunk
+    // loader map: nodes 4*ng .. 4*ng+3 of slots 2*sp, 2*sp+1 of the chunk.
+    // N0 is a multiple of the chunk length (block_far checks it), so every
+    // chunk is whole and the loads need no guard: a guarded vector load
+    // compiles to a select between global and private addresses and a flat
+    // load, which serialises the prefetch.
     const int ng = tid & 15, sp = tid >> 4;
     float4 x0, x1;
     auto load = [&](int s0) {
-        const int s = s0 + 2 * sp;
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        x0 = s < N0 ? *reinterpret_cast<const float4*>(hist + (int64_t)s * ld + i0 + 4 * ng) : z;
-        x1 = s + 1 < N0 ? *reinterpret_cast<const float4*>(hist + (int64_t)(s + 1) * ld + i0 + 4 * ng)
-                        : z;
+        const float* q = hist + (int64_t)(s0 + 2 * sp) * ld + i0 + 4 * ng;
+        x0 = *reinterpret_cast<const float4*>(q);
+        x1 = *reinterpret_cast<const float4*>(q + ld);
     };
+    // weight fragments of a chunk (from the table, L2-resident)
+    bf16x8 an[2][NS];
+    auto aload = [&](int s0) {
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+            const bf16x8* at = atab + ((int64_t)((N0 - s0) / kKc - 1) * 8 + wave * 2 + rr) * 3 * 64 + lane;
+#pragma unroll
+            for (int p = 0; p < NS; ++p) an[rr][p] = at[p * 64];
+        }
+    };
+    // software pipeline: every global load of chunk c + 1 is issued after
+    // the barrier of chunk c and consumed at the top of the next iteration,
+    // so the products of chunk c wait on LDS only
     load(0);
+    aload(0);
     for (int s0 = 0; s0 < N0; s0 += kKc) {
+        bf16x8 ap[2][NS];
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+            for (int p = 0; p < NS; ++p) ap[rr][p] = an[rr][p];
         // ---- store the loaded chunk: BF16 parts, node-major, slot pairs packed
         {
             const float a[4] = {x0.x, x0.y, x0.z, x0.w}, b[4] = {x1.x, x1.y, x1.z, x1.w};
@@ -87,7 +108,10 @@ __global__ __launch_bounds__(256, 2) void k_block_far(const float* __restrict__ 
             }
         }
         __syncthreads();
-        if (s0 + kKc < N0) load(s0 + kKc);  // next chunk in flight during the products
+        if (s0 + kKc < N0) {
+            load(s0 + kKc);
+            aload(s0 + kKc);
+        }
         bf16x8 bp[NS][4];
 #pragma unroll
         for (int p = 0; p < NS; ++p)
@@ -96,22 +120,17 @@ __global__ __launch_bounds__(256, 2) void k_block_far(const float* __restrict__ 
                 bp[p][ct] = *reinterpret_cast<const bf16x8*>(&sB[p][ct * 16 + lc][kg * 8]);
 #pragma unroll
         for (int rr = 0; rr < 2; ++rr) {
-            const int rt = wave * 2 + rr;
-            const bf16x8* at = atab + ((int64_t)((N0 - s0) / kKc - 1) * 8 + rt) * 3 * 64 + lane;
-            bf16x8 ap[NS];
-#pragma unroll
-            for (int p = 0; p < NS; ++p) ap[p] = at[p * 64];
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
                 f32x4 c = acc[rr][ct];
                 if constexpr (NP == 6) {  // small terms first
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1], bp[1][ct], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[2][ct], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[2], bp[0][ct], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[1][ct], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[2][ct], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][2], bp[0][ct], c, 0, 0, 0);
                 }
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[1][ct], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[1], bp[0][ct], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[0], bp[0][ct], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[1][ct], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[0][ct], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[0][ct], c, 0, 0, 0);
                 acc[rr][ct] = c;
             }
         }
