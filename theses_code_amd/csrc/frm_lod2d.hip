@@ -62,6 +62,7 @@ struct frm_lod2d {
     int atab_chunks = 0;
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
     float* F = nullptr;        // [128][ld]
+    double* G = nullptr;       // [16][ld] mid part of the current 8-step sub-block
 };
 
 namespace frm {
@@ -130,17 +131,39 @@ __global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
     double* Crow = p.C + (int64_t)(line + 1) * p.ld;
     const int64_t qbase = (int64_t)line * p.NX;
     if (live) {
-        // om in element order: C_n (copied to Cprev) and the history sum
+        // om in element order: C_n (copied to Cprev) and the history sum.
+        // Every load is issued before the first store (clamped, unguarded
+        // indices; the pointers do not alias), so the row costs one memory
+        // latency instead of one per element group.
+        const double* __restrict__ cr = Crow + 1;
+        const double* __restrict__ pt = p.part + qbase;
+        double* __restrict__ cpv = p.Cprev + qbase;
+        // the first partial-sum chunk is loaded with the state (0 + p0 = p0
+        // exactly, so the sum order is unchanged); further chunks follow
+        const bool any = p.nch > 0;
+        double u[CM], ts[CM];
+#pragma unroll
+        for (int r = 0; r < CM; ++r) {
+            const int e = min(r * kWave + lane, n - 1);
+            u[r] = cr[e];
+            ts[r] = any ? pt[e] : 0.0;
+        }
+        for (int c = 1; c < p.nch; ++c) {
+            const double* __restrict__ pc = pt + (int64_t)c * 2 * p.ldh;
+            double x[CM];
+#pragma unroll
+            for (int r = 0; r < CM; ++r) x[r] = pc[min(r * kWave + lane, n - 1)];
+#pragma unroll
+            for (int r = 0; r < CM; ++r) ts[r] += x[r];
+        }
+#pragma unroll
         for (int r = 0; r < CM; ++r) {
             const int e = r * kWave + lane;
             if (e < n) {
-                const double u = Crow[e + 1];
-                p.Cprev[qbase + e] = u;
-                double ts = 0.0;
-                for (int c = 0; c < p.nch; ++c) ts += p.part[(int64_t)c * 2 * p.ldh + qbase + e];
-                double om = p.negsk * u;
+                cpv[e] = u[r];
+                double om = p.negsk * u[r];
                 om += p.S_half;
-                om += p.sg * ts;
+                om += p.sg * ts[r];
                 buf[e + e / CM] = om;
             }
         }
@@ -216,19 +239,36 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
     const bool tlive = tline < p.lines;
     constexpr int kRowsPerPass = kColW * 64 / kColW;  // 64
     if (tlive) {
+        // all loads first (clamped indices), then the LDS writes
         const int i = tline + 1;
+        const double* __restrict__ cc = p.C + i;
+        const double* __restrict__ cpv = p.Cprev + (i - 1);
+        const bool any = p.nch > 0;
+        const double* __restrict__ p0 = p.part + p.ldh + (i - 1);
+        double u[CM], cp[CM], ts[CM];
+#pragma unroll
+        for (int r = 0; r < CM; ++r) {
+            const int e = min(r * kRowsPerPass + tr, n - 1);
+            u[r] = cc[(int64_t)(e + 1) * ld];
+            cp[r] = cpv[(int64_t)e * p.NX];  // node (i, j = e+1)
+            ts[r] = any ? p0[(int64_t)e * p.NX] : 0.0;
+        }
+        for (int c = 1; c < p.nch; ++c) {
+            const double* __restrict__ pc = p.part + ((int64_t)c * 2 + 1) * p.ldh + (i - 1);
+            double x[CM];
+#pragma unroll
+            for (int r = 0; r < CM; ++r) x[r] = pc[(int64_t)min(r * kRowsPerPass + tr, n - 1) * p.NX];
+#pragma unroll
+            for (int r = 0; r < CM; ++r) ts[r] += x[r];
+        }
+#pragma unroll
         for (int r = 0; r < CM; ++r) {
             const int e = r * kRowsPerPass + tr;
             if (e < n) {
-                const int64_t q = (int64_t)e * p.NX + (i - 1);  // node (i, j = e+1)
-                const double u = p.C[(int64_t)(e + 1) * ld + i];
-                const double cp = p.Cprev[q];
-                double ts = 0.0;
-                for (int c = 0; c < p.nch; ++c) ts += p.part[((int64_t)c * 2 + 1) * p.ldh + q];
-                ts += p.kch * (cp - u);
-                double om = p.negsk * u;
+                const double t = ts[r] + p.kch * (cp[r] - u[r]);
+                double om = p.negsk * u[r];
                 om += p.S_half;
-                om += p.sg * ts;
+                om += p.sg * t;
                 lds[tc][e + e / CM] = om;
             }
         }
@@ -255,13 +295,19 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
     __syncthreads();
     if (tlive) {
         const int i = tline + 1;
+        const double* __restrict__ cpv = p.Cprev + (i - 1);
+        double cp[CM];
+#pragma unroll
+        for (int r = 0; r < CM; ++r) cp[r] = cpv[(int64_t)min(r * kRowsPerPass + tr, n - 1) * p.NX];
+        double* __restrict__ cc = p.C + i;
+#pragma unroll
         for (int r = 0; r < CM; ++r) {
             const int e = r * kRowsPerPass + tr;
             if (e < n) {
                 const int64_t q = (int64_t)e * p.NX + (i - 1);
                 const double u = lds[tc][e + e / CM];
-                p.C[(int64_t)(e + 1) * ld + i] = u;
-                const double d = u - p.Cprev[q];
+                cc[(int64_t)(e + 1) * ld] = u;
+                const double d = u - cp[r];
                 if (p.slotf)
                     p.slotf[q] = (float)d;
                 else
@@ -400,6 +446,7 @@ static void lod_free(frm_lod2d* s) {
     hipFree(s->soe_c);
     hipFree(s->atab);
     hipFree(s->F);
+    hipFree(s->G);
     delete s;
 }
 
@@ -567,6 +614,7 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
         if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * kBlockSteps * s->hist->ld);
+        if (e == hipSuccess) e = hipMalloc(&s->G, sizeof(double) * 2 * kSubSteps * s->hist->ld);
         if (e == hipSuccess)
             e = hipMemcpy(s->atab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -648,10 +696,19 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                 if (block_far(ctx->stream, h32, ld, N0, s->atab, s->atab_chunks, s->mfma_nprod, s->F))
                     return check_hip(ctx, hipGetLastError(), "block_far");
             }
+            // the settled part of the block, every 8 steps, in FP64
+            const int N1 = n / kSubSteps * kSubSteps;
+            if (N1 > N0 && n == N1) {
+                LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * ((N1 - N0) * 4.0 + 2 * kSubSteps * 8));
+                if (block_mid(ctx->stream, h32, ld, N0, N1, s->w, s->capacity, s->G))
+                    return check_hip(ctx, hipGetLastError(), "block_mid");
+            }
             nch = 1;
             LaunchTimer t(ctx, FRM_K_MEMTERM,
-                          (double)ld * ((n - N0) * 4.0 + (N0 > 0 ? 2 * 4 : 0) + 2 * 8));
-            if (block_near(ctx->stream, h32, ld, N0, n, s->w, s->capacity, s->F, N0 > 0, s->hist->part))
+                          (double)ld * ((n - N1) * 4.0 + (N0 > 0 ? 2 * 4 : 0) + (N1 > N0 ? 2 * 8 : 0) +
+                                        2 * 8));
+            if (block_near(ctx->stream, h32, ld, N0, N1, n, s->w, s->capacity, s->F, N0 > 0, s->G,
+                           N1 > N0, s->hist->part))
                 return check_hip(ctx, hipGetLastError(), "block_near");
         } else {
             int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);

@@ -150,23 +150,72 @@ __global__ __launch_bounds__(256, 2) void k_block_far(const float* __restrict__ 
     }
 }
 
-// part[t*ld + i] = F[t*64 + n - N0][i] + sum_{N0 <= s < n} w_t(n - s) Delta_s[i],
+// Mid part of a 64-step block, refreshed every kSubSteps steps: for the
+// sub-block starting at N1 (N0 < N1 < N0 + 64),
+//   G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i],  j < 8,
+// in FP64 (two nodes per thread, 8-byte loads of the FP32 slots; the
+// weights are wavefront-uniform).
+__global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ hist, int64_t ld,
+                                                   int N0, int N1, const double* __restrict__ w,
+                                                   int ldw, double* __restrict__ G) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (i >= ld) return;
+    double g[2][kSubSteps][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int j = 0; j < kSubSteps; ++j) g[t][j][0] = g[t][j][1] = 0.0;
+    for (int s = N0; s < N1; ++s) {
+        const float2 x = *reinterpret_cast<const float2*>(hist + (int64_t)s * ld + i);
+        const int l0 = N1 - s;  // lag of step N1 + j is l0 + j
+#pragma unroll
+        for (int j = 0; j < kSubSteps; ++j) {
+            const bool ok = l0 + j <= ldw;  // lags past the capacity belong to no step
+            const double a = ok ? w[l0 + j - 1] : 0.0, b = ok ? w[ldw + l0 + j - 1] : 0.0;
+            g[0][j][0] += a * (double)x.x;
+            g[0][j][1] += a * (double)x.y;
+            g[1][j][0] += b * (double)x.x;
+            g[1][j][1] += b * (double)x.y;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int j = 0; j < kSubSteps; ++j)
+            *reinterpret_cast<double2*>(G + (int64_t)(t * kSubSteps + j) * ld + i) =
+                make_double2(g[t][j][0], g[t][j][1]);
+}
+
+// part[t*ld + i] = F[t*64 + n - N0][i] + G[t*8 + n - N1][i]
+//                  + sum_{N1 <= s < n} w_t(n - s) Delta_s[i],
 // four nodes per thread (16-byte loads of the FP32 slots), FP64 sums.
 __global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hist, int64_t ld,
-                                                    int N0, int n, const double* __restrict__ w,
-                                                    int ldw, const float* __restrict__ F,
-                                                    int use_F, double* __restrict__ part) {
+                                                    int N0, int N1, int n,
+                                                    const double* __restrict__ w, int ldw,
+                                                    const float* __restrict__ F, int use_F,
+                                                    const double* __restrict__ G, int use_G,
+                                                    double* __restrict__ part) {
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i >= ld) return;
-    const int j = n - N0;
     double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
     if (use_F) {
+        const int j = n - N0;
         const float4 f0 = *reinterpret_cast<const float4*>(F + (int64_t)j * ld + i);
         const float4 f1 = *reinterpret_cast<const float4*>(F + (int64_t)(kBlockSteps + j) * ld + i);
         s0[0] = f0.x, s0[1] = f0.y, s0[2] = f0.z, s0[3] = f0.w;
         s1[0] = f1.x, s1[1] = f1.y, s1[2] = f1.z, s1[3] = f1.w;
     }
-    for (int s = N0; s < n; ++s) {
+    if (use_G) {
+        const int j = n - N1;
+        const double* g0 = G + (int64_t)j * ld + i;
+        const double* g1 = G + (int64_t)(kSubSteps + j) * ld + i;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            s0[q] += g0[q];
+            s1[q] += g1[q];
+        }
+    }
+    for (int s = N1; s < n; ++s) {
         const float4 x = *reinterpret_cast<const float4*>(hist + (int64_t)s * ld + i);
         const double a = w[n - s - 1], b = w[ldw + n - s - 1];
         s0[0] += a * (double)x.x, s1[0] += b * (double)x.x;
@@ -221,12 +270,23 @@ std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks) {
     return out;
 }
 
-int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int n, const double* w,
-               int ldw, const float* F, int use_F, double* part) {
-    if (ld % 4 || n < N0 || n - N0 >= kBlockSteps) return -1;
+int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, const double* w,
+              int ldw, double* G) {
+    if (ld % 2 || N1 <= N0 || N1 - N0 >= kBlockSteps || (N1 - N0) % kSubSteps) return -1;
+    const int64_t threads = ld / 2;
+    const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
+    hipLaunchKernelGGL(k_block_mid, grid, block, 0, st, hist, ld, N0, N1, w, ldw, G);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, int n,
+               const double* w, int ldw, const float* F, int use_F, const double* G, int use_G,
+               double* part) {
+    if (ld % 4 || n < N1 || N1 < N0 || n - N1 >= kSubSteps || n - N0 >= kBlockSteps) return -1;
     const int64_t threads = ld / 4;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
-    hipLaunchKernelGGL(k_block_near, grid, block, 0, st, hist, ld, N0, n, w, ldw, F, use_F, part);
+    hipLaunchKernelGGL(k_block_near, grid, block, 0, st, hist, ld, N0, N1, n, w, ldw, F, use_F, G,
+                       use_G, part);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

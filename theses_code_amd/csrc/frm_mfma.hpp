@@ -9,6 +9,7 @@ namespace frm {
 
 constexpr int kBlockSteps = 64;  // steps that share one far-part product
 constexpr int kFarNodes = 64;    // nodes per far-part workgroup (ld is a multiple)
+constexpr int kSubSteps = 8;     // steps that share one mid-part sum
 
 constexpr int kFarChunk = 32;    // history slots per product k-chunk
 
@@ -21,8 +22,14 @@ int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const void*
 // Host: weight-fragment table for chunks d = 32 .. 32*chunks from the BF16
 // parts of the weights, wparts[(p*2 + t)*ldw + lag - 1], p < 3.
 std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks);
-// part[t*ld + i] = F[t*64 + n - N0][i] (if use_F) + sum_{N0 <= s < n} w_t(n - s) Delta_s[i]
-int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int n, const double* w,
-               int ldw, const float* F, int use_F, double* part);
+// G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i], j < 8, FP64;
+// N1 - N0 a positive multiple of 8 below 64.
+int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, const double* w,
+              int ldw, double* G);
+// part[t*ld + i] = F[t*64 + n - N0][i] (if use_F) + G[t*8 + n - N1][i] (if use_G)
+//                  + sum_{N1 <= s < n} w_t(n - s) Delta_s[i]
+int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, int n,
+               const double* w, int ldw, const float* F, int use_F, const double* G, int use_G,
+               double* part);
 
 }  // namespace frm
