@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="c3: problems timed through the one-solver-per-problem plain path")
     ap.add_argument("--soe-f32", action="store_true", help="c2/c4: FP32 mode states")
+    ap.add_argument("--sweep-w", type=int, default=0,
+                    help="wavefronts per sweep line for the chosen forms (0: autotuned)")
     ap.add_argument("--breakdown-plain", action="store_true",
                     help="c2/c4: also time plain FP64 per kernel (a second full run)")
     ap.add_argument("--forms", default="soe,mfma",
@@ -293,6 +295,7 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
             work[name] = by
     out = s.b_C
     fit = s.fit_error
+    work["sweep_shape"] = s.sweep_shape
     s.close()
     return dt, out, kt, fit, work
 
@@ -323,8 +326,9 @@ def run_lod_forms(args, cfg_name):
         kw = dict(kernel=N.KERNEL_AB, ab_exact=True)
     K = args.steps if args.steps != 5000 else cfg["steps"]
     forms = args.forms.split(",")
-    fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=args.soe_f32),
-           "mfma": dict(precision=N.PREC_MFMA_BF16)}
+    fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=args.soe_f32,
+                       sweep_w=args.sweep_w),
+           "mfma": dict(precision=N.PREC_MFMA_BF16, sweep_w=args.sweep_w)}
     tot_nodes, t_plain, kt_plain = 0, 0.0, {}
     res = {f: dict(seconds=0.0, errs=[], rms=[], fits=[], kernels_ms={}, work={}) for f in forms}
     for r in runs:
@@ -341,8 +345,9 @@ def run_lod_forms(args, cfg_name):
             kt_plain[k] = kt_plain.get(k, 0.0) + v
         msg = f"# set alpha={r['alpha']:.3f} grid={sN - 1}^2 plain {dp:.3f}s"
         for f in forms:
-            ds, us, _, fit, _ = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"],
-                                         cfg["sigma"], K, **kw, **fkw[f])
+            ds, us, _, fit, w0 = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"],
+                                          cfg["sigma"], K, **kw, **fkw[f])
+            res[f].setdefault("sweep_shapes", []).append(w0["sweep_shape"])
             # per-kernel breakdown from a second, event-timed run
             _, _, ks, _, wk = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"],
                                        K, timing=True, **kw, **fkw[f])
@@ -355,7 +360,8 @@ def run_lod_forms(args, cfg_name):
             for k, v in ks.items():
                 q["kernels_ms"][k] = q["kernels_ms"].get(k, 0.0) + v
             for k, v in wk.items():
-                q["work"][k] = q["work"].get(k, 0.0) + v
+                if k != "sweep_shape":
+                    q["work"][k] = q["work"].get(k, 0.0) + v
             msg += f" {f} {ds:.3f}s err {q['errs'][-1]:.2e}"
         print(msg, file=sys.stderr, flush=True)
     out_forms = {}
@@ -363,7 +369,8 @@ def run_lod_forms(args, cfg_name):
         q = res[f]
         o = {"form": FORMS[f], "value": tot_nodes * K / q["seconds"], "seconds": q["seconds"],
              "speedup_vs_plain_fp64": t_plain / q["seconds"], "kernels_ms": q["kernels_ms"],
-             "max_rel_err_vs_fp64": max(q["errs"]), "rms_rel_err_vs_fp64": max(q["rms"])}
+             "max_rel_err_vs_fp64": max(q["errs"]), "rms_rel_err_vs_fp64": max(q["rms"]),
+             "sweep_shapes_rows_cols": q["sweep_shapes"]}
         if f == "soe":
             o["terms"] = cfg["soe_terms"]
             o["states"] = "fp32" if args.soe_f32 else "fp64"

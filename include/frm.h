@@ -165,6 +165,8 @@ typedef struct frm_lod2d_desc {
     int32_t mfma_parts;  /* FRM_PREC_MFMA_BF16: BF16 parts per operand, 2 (three
                             products, ~2^-16) or 3 (six products, exact FP32
                             increments, ~2^-24); 0: 2 for Caputo, 3 for AB */
+    int32_t sweep_w;     /* wavefronts per sweep line (1, 2, 4); 0: chosen at
+                            creation by timing each (the autotuner) */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);
@@ -185,6 +187,9 @@ int frm_lod2d_fit_error(const frm_lod2d* s, double* max_rel);
  * column current weight, out[2] = half-step correction, then the row and
  * column history weights for lags 1..n (2 + 1 + 2n values). */
 int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out);
+/* Launch shapes of the sweeps in use: out[0..3] = elements per lane and
+ * wavefronts per line of the row sweep, then of the column sweep. */
+int frm_lod2d_sweep_shape(const frm_lod2d* s, int32_t* out);
 
 /* ------------------------- batched 1D H_solver (mt_minimal / mt_psi_caputo)
  * A batch of independent 1D problems stepped together, each one

@@ -68,6 +68,38 @@ def test_lod2d_matches_oracle_advection(ctx, sN, M, alpha, steps):
     g.close()
 
 
+# Every sweep launch shape (wavefronts per line W = 1, 2, 4 with their chunk
+# lengths) against the oracle, on lines from 1 to 4 wavefronts' worth of
+# lanes, including rows of 2499 unknowns (beyond one wavefront's 2048).
+@pytest.mark.parametrize("W", [1, 2, 4])
+@pytest.mark.parametrize("sN,M", [(300, 61), (67, 530), (2500, 21)])
+def test_lod2d_sweep_shapes_match_oracle(ctx, W, sN, M):
+    if W == 1 and sN > 2049:
+        pytest.skip("one wavefront holds at most 2048 unknowns per line")
+    rng = np.random.default_rng(sN + 7 * M)
+    coefs, C0 = _adv_case(rng, sN, M)
+    steps = 9
+    o = O.TGSolver(sN, M, 0.7, 1e-3, 1.0, 0.0, 0.05, *coefs, C0)
+    g = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0, S=0.05,
+                   sweep_w=W)
+    (_, wr), (_, wc) = g.sweep_shape
+    assert wr == W and wc == W
+    o.step(steps)
+    g.calc_step(steps)
+    assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    g.close()
+
+
+def test_lod2d_autotuned_shape_is_valid(ctx):
+    rng = np.random.default_rng(3)
+    coefs, C0 = _adv_case(rng, 600, 40)
+    g = S.TGSolver(ctx, 600, 40, 0.7, 1e-3, 1.0, coefs, C0, capacity=2)
+    (cr, wr), (cc, wc) = g.sweep_shape
+    assert wr in (1, 2, 4) and wc in (1, 2, 4)
+    assert 64 * wr * cr >= 599 and 64 * wc * cc >= 39
+    g.close()
+
+
 def test_lod2d_capacity_error(ctx):
     rng = np.random.default_rng(1)
     coefs, C0 = _adv_case(rng, 10, 10)

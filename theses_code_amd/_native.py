@@ -43,7 +43,7 @@ class Lod2dDesc(ct.Structure):
                 ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P),
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
                 ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
-                ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32)]
+                ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32), ("sweep_w", ct.c_int32)]
 
 
 class MT1DProblem(ct.Structure):
@@ -94,6 +94,7 @@ SIGNATURES = {
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
     "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
     "frm_lod2d_fit_error": (_I, [_V, _P]),
+    "frm_lod2d_sweep_shape": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_weights": (_I, [_V, ct.c_int32, _P]),
     "frm_mt1d_create": (_I, [_V, ct.POINTER(MT1DDesc), ct.POINTER(_V)]),
     "frm_mt1d_destroy": (_I, [_V]),

@@ -31,6 +31,13 @@
 #include "frm_soe.hpp"
 #include "frm_weights.hpp"
 
+namespace frm {
+// Launch shape of a sweep: chunk length CM and wavefronts per line W.
+struct SweepShape {
+    int cm = 0, w = 1;
+};
+}  // namespace frm
+
 struct frm_lod2d {
     frm_ctx* ctx = nullptr;
     int sN = 0, M = 0, ld = 0;
@@ -46,7 +53,8 @@ struct frm_lod2d {
     double* w = nullptr;       // [2][capacity] lag-indexed history weights
     double* rcoef = nullptr;   // rows:    a, f, al interleaved per line (coef_at)
     double* ccoef = nullptr;   // columns: the same
-    int cm_rows = 0, cm_cols = 0;
+    frm::SweepShape rsh, csh;  // launch shapes of the row / column sweeps
+    std::vector<double> rplain, cplain;  // [3][lines][n] coefficients, for re-layout
     int precision = FRM_PREC_F64;
     std::vector<double> hw;    // host copy: kr0, kc0, kch, then [2][capacity] weights
     // sum-of-exponentials form
@@ -81,34 +89,35 @@ static double caputo_kb(double alpha, double tau, double d, double l) {
 }
 
 // ------------------------------------------------------------- kernels
+// A line of n unknowns is split into NL = 64 W chunks of CM elements, one
+// per lane of the W wavefronts that share the line (W = 1, 2 or 4: the
+// number of lanes that run the parallel scan levels before each lane's
+// sequential Thomas replay of CM elements).
+//
 // Coefficient layout: for line L and array c (0: a, 1: f, 2: al), element
-// e = lane*CM + k of the line (lane's chunk, position k) is stored at
-//   ((L*3 + c)*CM + k)*64 + lane,
-// so that a wavefront's k-th load of a coefficient is 64 consecutive doubles.
+// e = g*CM + k of the line (chunk g = 64 wl + lane, position k) is stored at
+//   ((L*3 + c)*CM + k)*NL + g,
+// so that the k-th load of a coefficient is NL consecutive doubles.
 // The state, the partial sums and Cprev are staged through LDS: read in
 // element order (consecutive lanes, consecutive nodes), written to LDS at
 // e + e/CM (chunk-major with one pad double per chunk, so that the chunk
-// reads lane*(CM+1) + k spread over the banks), read back per chunk.
-template <int CM>
-__device__ inline int64_t coef_at(int line, int c, int k, int lane) {
-    return ((int64_t)(line * 3 + c) * CM + k) * kWave + lane;
-}
-
-template <int CM>
-__device__ inline void load_coefs(const double* coef, int line, int lane, int e0, int n,
-                                  double (&a)[CM], double (&f)[CM], double (&al)[CM]) {
+// reads g*(CM+1) + k spread over the banks), read back per chunk.
+template <int CM, int W>
+__device__ inline void load_coefs(const double* coef, int line, int g, double (&a)[CM],
+                                  double (&f)[CM], double (&al)[CM]) {
+    constexpr int NL = kWave * W;
 #pragma unroll
     for (int k = 0; k < CM; ++k) {
-        a[k] = coef[coef_at<CM>(line, 0, k, lane)];
-        f[k] = coef[coef_at<CM>(line, 1, k, lane)];
-        al[k] = coef[coef_at<CM>(line, 2, k, lane)];
+        a[k] = coef[((int64_t)(line * 3 + 0) * CM + k) * NL + g];
+        f[k] = coef[((int64_t)(line * 3 + 1) * CM + k) * NL + g];
+        al[k] = coef[((int64_t)(line * 3 + 2) * CM + k) * NL + g];
     }
 }
 
 struct RowArgs {
     double* C;
     double* Cprev;
-    const double* coef;  // interleaved, see coef_at
+    const double* coef;  // interleaved, see load_coefs
     const double* part;  // partial sums [nch][2][ldh], target 0
     int64_t ldh;
     int nch;
@@ -116,35 +125,47 @@ struct RowArgs {
     double negsk, S_half, sg, b0, al1;
 };
 
-constexpr int kRowWaves = 4;  // row lines per workgroup
-
-template <int CM>
+// Rows: 4 wavefronts per workgroup, 4/W lines.
+template <int CM, int W>
 __global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
     constexpr int S = CM + 1;
-    __shared__ double lds[kRowWaves][kWave * S];
+    constexpr int LPB = 4 / W;
+    constexpr int NL = kWave * W;
+    __shared__ double lds[LPB][NL * S];
+    __shared__ LineShare<W> share[LPB];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int line = blockIdx.x * kRowWaves + wv;
-    const bool live = line < p.lines;  // wave-uniform; dead waves still reach the barriers
+    const int li = wv / W, wl = wv % W;
+    const int line = blockIdx.x * LPB + li;
+    const bool live = line < p.lines;  // uniform per line; dead lines still reach the barriers
     const int n = p.n;
-    const int e0 = lane * CM;
-    double* buf = lds[wv];
-    double* Crow = p.C + (int64_t)(line + 1) * p.ld;
+    const int g = wl * kWave + lane;
+    const int e0 = g * CM;
+    double* buf = lds[li];
     const int64_t qbase = (int64_t)line * p.NX;
+    double* Crow = p.C + (int64_t)(line + 1) * p.ld;
+    // The coefficients do not depend on the staging: their loads go out
+    // first and overlap the state loads (up to 16 elements per lane; at 32
+    // the registers would spill).
+    constexpr bool kEarly = CM <= 16;
+    double a[CM], f[CM], al[CM], v[CM];
+#pragma unroll
+    for (int k = 0; k < CM; ++k) a[k] = f[k] = al[k] = v[k] = 0.0;
+    if (kEarly && live) load_coefs<CM, W>(p.coef, line, g, a, f, al);
     if (live) {
         // om in element order: C_n (copied to Cprev) and the history sum.
         // Every load is issued before the first store (clamped, unguarded
         // indices; the pointers do not alias), so the row costs one memory
-        // latency instead of one per element group.
+        // latency instead of one per element group.  The first partial-sum
+        // chunk is loaded with the state (0 + p0 = p0 exactly, so the sum
+        // order is unchanged); further chunks follow.
         const double* __restrict__ cr = Crow + 1;
         const double* __restrict__ pt = p.part + qbase;
         double* __restrict__ cpv = p.Cprev + qbase;
-        // the first partial-sum chunk is loaded with the state (0 + p0 = p0
-        // exactly, so the sum order is unchanged); further chunks follow
         const bool any = p.nch > 0;
         double u[CM], ts[CM];
 #pragma unroll
         for (int r = 0; r < CM; ++r) {
-            const int e = min(r * kWave + lane, n - 1);
+            const int e = min(r * NL + g, n - 1);
             u[r] = cr[e];
             ts[r] = any ? pt[e] : 0.0;
         }
@@ -152,13 +173,13 @@ __global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
             const double* __restrict__ pc = pt + (int64_t)c * 2 * p.ldh;
             double x[CM];
 #pragma unroll
-            for (int r = 0; r < CM; ++r) x[r] = pc[min(r * kWave + lane, n - 1)];
+            for (int r = 0; r < CM; ++r) x[r] = pc[min(r * NL + g, n - 1)];
 #pragma unroll
             for (int r = 0; r < CM; ++r) ts[r] += x[r];
         }
 #pragma unroll
         for (int r = 0; r < CM; ++r) {
-            const int e = r * kWave + lane;
+            const int e = r * NL + g;
             if (e < n) {
                 cpv[e] = u[r];
                 double om = p.negsk * u[r];
@@ -169,25 +190,25 @@ __global__ __launch_bounds__(256) void k_lod_rows(RowArgs p) {
         }
     }
     __syncthreads();
-    double a[CM], f[CM], al[CM], v[CM];
-    double un = 0.0;
     if (live) {
-        load_coefs<CM>(p.coef, line, lane, e0, n, a, f, al);
+        if (!kEarly) load_coefs<CM, W>(p.coef, line, g, a, f, al);
 #pragma unroll
-        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? buf[lane * S + k] : 0.0;
-        un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? buf[g * S + k] : 0.0;
+    }
+    const double un = line_solve_mw<CM, W>(a, f, al, v, e0, n, p.b0, lane, wl, &share[li], live);
+    if (live) {
 #pragma unroll
         for (int k = 0; k < CM; ++k)
-            if (e0 + k < n) buf[lane * S + k] = v[k];
+            if (e0 + k < n) buf[g * S + k] = v[k];
     }
     __syncthreads();
     if (live) {
         for (int r = 0; r < CM; ++r) {
-            const int e = r * kWave + lane;
+            const int e = r * NL + g;
             if (e < n) Crow[e + 1] = buf[e + e / CM];
         }
-        if (lane == 0) Crow[0] = p.al1 * v[0] + p.b0;
-        if (lane == 63) {
+        if (g == 0) {
+            Crow[0] = p.al1 * v[0] + p.b0;
             Crow[n + 1] = un;
             Crow[n + 2] = un;
         }
@@ -199,45 +220,55 @@ struct ColArgs {
     const double* Cprev;
     double* slot;        // history slot for Delta_n (compact node order)
     float* slotf;        // FP32 history slot instead of slot when non-null
-    const double* coef;  // interleaved, see coef_at
+    const double* coef;  // interleaved, see load_coefs
     const double* part;  // target 1 partials at part + ldh
     int64_t ldh;
     int nch;
     int ld, lines, n, NX;
-    int groups;          // column groups of kColW lines
+    int groups;          // column groups of col_width lines
     double negsk, S_half, sg, kch, b0, al1;
 };
 
-// Column lines per workgroup: a row segment of W columns is one contiguous
-// W*8-byte piece of the state, Cprev and partial sums.  W = 8 up to 1024
-// unknowns per line; 4 above, where a lane's registers hold 32 elements.
-template <int CM>
-constexpr int col_width() {
-    return CM >= 32 ? 4 : 8;
+// Column lines per workgroup: a row segment of col_width columns is one
+// contiguous piece of the state, Cprev and partial sums; workgroups stay at
+// 512 threads and within the LDS the staging needs.
+template <int CM, int W>
+__host__ __device__ constexpr int col_width() {
+    return (CM * W >= 32 ? 4 : 8) / W > 0 ? (CM * W >= 32 ? 4 : 8) / W : 1;
 }
 
-template <int CM>
-__global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
-    constexpr int kColW = col_width<CM>();
+template <int CM, int W>
+__global__ __launch_bounds__((col_width<CM, W>() * kWave * W)) void k_lod_cols(ColArgs p) {
+    constexpr int kColW = col_width<CM, W>();
     constexpr int S = CM + 1;
-    __shared__ double lds[kColW][kWave * S];
+    constexpr int NL = kWave * W;
+    __shared__ double lds[kColW][NL * S];
+    __shared__ LineShare<W> share[kColW];
     // Workgroups are dealt to the 8 XCDs round-robin (b -> XCD b % 8); the
     // group order below gives each XCD a contiguous run of column groups so
     // that neighbouring groups, which share cache lines, share an L2.
     const int per = gridDim.x / 8;
-    const int g = (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (g >= p.groups) return;  // whole workgroup, before any barrier
+    const int grp = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (grp >= p.groups) return;  // whole workgroup, before any barrier
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int line = g * kColW + wv;
+    const int li = wv / W, wl = wv % W;
+    const int line = grp * kColW + li;
     const bool live = line < p.lines;
     const int n = p.n;
-    const int e0 = lane * CM;
+    const int g = wl * kWave + lane;
+    const int e0 = g * CM;
     const int64_t ld = p.ld;
-    // staging map: column tc of the group, rows tr + 64 r
+    // staging map: column tc of the group, rows tr + NL r
     const int tc = threadIdx.x % kColW, tr = threadIdx.x / kColW;
-    const int tline = g * kColW + tc;
+    const int tline = grp * kColW + tc;
     const bool tlive = tline < p.lines;
-    constexpr int kRowsPerPass = kColW * 64 / kColW;  // 64
+    // coefficient loads first (they overlap the staging loads); the staging
+    // thread keeps its C_n values of Cprev for the increment push
+    constexpr bool kEarly = CM <= 8;
+    double a[CM], f[CM], al[CM], v[CM], cp[CM];
+#pragma unroll
+    for (int k = 0; k < CM; ++k) a[k] = f[k] = al[k] = v[k] = cp[k] = 0.0;
+    if (kEarly && live) load_coefs<CM, W>(p.coef, line, g, a, f, al);
     if (tlive) {
         // all loads first (clamped indices), then the LDS writes
         const int i = tline + 1;
@@ -245,10 +276,10 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
         const double* __restrict__ cpv = p.Cprev + (i - 1);
         const bool any = p.nch > 0;
         const double* __restrict__ p0 = p.part + p.ldh + (i - 1);
-        double u[CM], cp[CM], ts[CM];
+        double u[CM], ts[CM];
 #pragma unroll
         for (int r = 0; r < CM; ++r) {
-            const int e = min(r * kRowsPerPass + tr, n - 1);
+            const int e = min(r * NL + tr, n - 1);
             u[r] = cc[(int64_t)(e + 1) * ld];
             cp[r] = cpv[(int64_t)e * p.NX];  // node (i, j = e+1)
             ts[r] = any ? p0[(int64_t)e * p.NX] : 0.0;
@@ -257,13 +288,13 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
             const double* __restrict__ pc = p.part + ((int64_t)c * 2 + 1) * p.ldh + (i - 1);
             double x[CM];
 #pragma unroll
-            for (int r = 0; r < CM; ++r) x[r] = pc[(int64_t)min(r * kRowsPerPass + tr, n - 1) * p.NX];
+            for (int r = 0; r < CM; ++r) x[r] = pc[(int64_t)min(r * NL + tr, n - 1) * p.NX];
 #pragma unroll
             for (int r = 0; r < CM; ++r) ts[r] += x[r];
         }
 #pragma unroll
         for (int r = 0; r < CM; ++r) {
-            const int e = r * kRowsPerPass + tr;
+            const int e = r * NL + tr;
             if (e < n) {
                 const double t = ts[r] + p.kch * (cp[r] - u[r]);
                 double om = p.negsk * u[r];
@@ -274,20 +305,19 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
         }
     }
     __syncthreads();
-    double a[CM], f[CM], al[CM], v[CM];
-    double un = 0.0;
     if (live) {
-        double* buf = lds[wv];
-        load_coefs<CM>(p.coef, line, lane, e0, n, a, f, al);
+        if (!kEarly) load_coefs<CM, W>(p.coef, line, g, a, f, al);
 #pragma unroll
-        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? buf[lane * S + k] : 0.0;
-        un = line_solve<CM>(a, f, al, v, e0, n, p.b0, lane);
+        for (int k = 0; k < CM; ++k) v[k] = e0 + k < n ? lds[li][g * S + k] : 0.0;
+    }
+    const double un = line_solve_mw<CM, W>(a, f, al, v, e0, n, p.b0, lane, wl, &share[li], live);
+    if (live) {
 #pragma unroll
         for (int k = 0; k < CM; ++k)
-            if (e0 + k < n) buf[lane * S + k] = v[k];
+            if (e0 + k < n) lds[li][g * S + k] = v[k];
         double* Ccol = p.C + (line + 1);
-        if (lane == 0) Ccol[0] = p.al1 * v[0] + p.b0;
-        if (lane == 63) {
+        if (g == 0) {
+            Ccol[0] = p.al1 * v[0] + p.b0;
             Ccol[(int64_t)(n + 1) * ld] = un;
             Ccol[(int64_t)(n + 2) * ld] = un;
         }
@@ -295,14 +325,10 @@ __global__ __launch_bounds__(col_width<CM>() * 64) void k_lod_cols(ColArgs p) {
     __syncthreads();
     if (tlive) {
         const int i = tline + 1;
-        const double* __restrict__ cpv = p.Cprev + (i - 1);
-        double cp[CM];
-#pragma unroll
-        for (int r = 0; r < CM; ++r) cp[r] = cpv[(int64_t)min(r * kRowsPerPass + tr, n - 1) * p.NX];
         double* __restrict__ cc = p.C + i;
 #pragma unroll
         for (int r = 0; r < CM; ++r) {
-            const int e = r * kRowsPerPass + tr;
+            const int e = r * NL + tr;
             if (e < n) {
                 const int64_t q = (int64_t)e * p.NX + (i - 1);
                 const double u = lds[tc][e + e / CM];
@@ -387,50 +413,175 @@ static float bf16_round(double x) {
     return f;
 }
 
-// [3][lines][n] (a, f, al per line) -> the interleaved layout of coef_at
+// [3][lines][n] (a, f, al per line) -> the interleaved layout of load_coefs
 static std::vector<double> interleave_coefs(const std::vector<double>& plain, int lines, int n,
-                                            int cm) {
-    std::vector<double> out((size_t)lines * 3 * cm * kWave, 0.0);
+                                            int cm, int nl) {
+    std::vector<double> out((size_t)lines * 3 * cm * nl, 0.0);
     for (int L = 0; L < lines; ++L)
         for (int c = 0; c < 3; ++c)
             for (int e = 0; e < n; ++e) {
-                const int lane = e / cm, k = e % cm;
-                out[(((size_t)L * 3 + c) * cm + k) * kWave + lane] =
-                    plain[((size_t)c * lines + L) * n + e];
+                const int g = e / cm, k = e % cm;
+                out[(((size_t)L * 3 + c) * cm + k) * nl + g] = plain[((size_t)c * lines + L) * n + e];
             }
     return out;
 }
 
-static int pick_cm(int n) {
-    if (n <= 64 * 4) return 4;
-    if (n <= 64 * 8) return 8;
-    if (n <= 64 * 16) return 16;
-    if (n <= 64 * 32) return 32;
+// smallest chunk length that fits n unknowns on 64 W lanes (0: none)
+static int pick_cm(int n, int W) {
+    for (int cm : {4, 8, 16, 32})
+        if (n <= kWave * W * cm) return cm;
     return 0;
 }
 
-template <typename Args>
-static hipError_t launch_rows(int cm, dim3 grid, hipStream_t st, const Args& a) {
-    switch (cm) {
-        case 4: hipLaunchKernelGGL(k_lod_rows<4>, grid, dim3(256), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(k_lod_rows<8>, grid, dim3(256), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(k_lod_rows<16>, grid, dim3(256), 0, st, a); break;
-        case 32: hipLaunchKernelGGL(k_lod_rows<32>, grid, dim3(256), 0, st, a); break;
-        default: return hipErrorInvalidValue;
+
+#define FRM_SWEEP_CASES(X) X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(4, 2) X(8, 2) X(16, 2) X(32, 2) \
+    X(4, 4) X(8, 4) X(16, 4) X(32, 4)
+
+static hipError_t launch_rows(SweepShape sh, hipStream_t st, const RowArgs& a) {
+    const int lpb = 4 / sh.w;
+    const dim3 grid((unsigned)((a.lines + lpb - 1) / lpb));
+#define X(CM, W)                                                                     \
+    if (sh.cm == CM && sh.w == W) {                                                  \
+        hipLaunchKernelGGL((k_lod_rows<CM, W>), grid, dim3(256), 0, st, a);          \
+        return hipGetLastError();                                                    \
     }
-    return hipGetLastError();
+    FRM_SWEEP_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
-template <typename Args>
-static hipError_t launch_cols(int cm, dim3 grid, hipStream_t st, const Args& a) {
-    switch (cm) {
-        case 4: hipLaunchKernelGGL(k_lod_cols<4>, grid, dim3(col_width<4>() * 64), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(k_lod_cols<8>, grid, dim3(col_width<8>() * 64), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(k_lod_cols<16>, grid, dim3(col_width<16>() * 64), 0, st, a); break;
-        case 32: hipLaunchKernelGGL(k_lod_cols<32>, grid, dim3(col_width<32>() * 64), 0, st, a); break;
-        default: return hipErrorInvalidValue;
+static int col_groups(SweepShape sh, int lines) {
+    int cw = 0;
+#define X(CM, W) \
+    if (sh.cm == CM && sh.w == W) cw = col_width<CM, W>();
+    FRM_SWEEP_CASES(X)
+#undef X
+    return cw ? (lines + cw - 1) / cw : 0;
+}
+
+static hipError_t launch_cols(SweepShape sh, hipStream_t st, const ColArgs& a) {
+    const dim3 grid((unsigned)((a.groups + 7) / 8 * 8));
+#define X(CM, W)                                                                                 \
+    if (sh.cm == CM && sh.w == W) {                                                              \
+        hipLaunchKernelGGL((k_lod_cols<CM, W>), grid, dim3(col_width<CM, W>() * kWave * W), 0, \
+                           st, a);                                                               \
+        return hipGetLastError();                                                                \
     }
-    return hipGetLastError();
+    FRM_SWEEP_CASES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+// Kernel arguments of one step's sweeps (n = step index, for the push slot).
+static RowArgs row_args(frm_lod2d* s, int nch) {
+    RowArgs a;
+    a.C = s->C;
+    a.Cprev = s->Cprev;
+    a.coef = s->rcoef;
+    a.part = s->hist->part;
+    a.ldh = s->hist->ld;
+    a.nch = nch;
+    a.ld = s->ld;
+    a.lines = s->M - 1;
+    a.n = s->NX;
+    a.NX = s->NX;
+    a.negsk = -s->sg * s->kr0;
+    a.S_half = s->S * 0.5;
+    a.sg = s->sg;
+    a.b0 = s->C1;
+    a.al1 = 0.0;
+    return a;
+}
+
+static ColArgs col_args(frm_lod2d* s, int nch, int n) {
+    const bool soe = s->precision == FRM_PREC_SOE;
+    const bool mf = s->precision == FRM_PREC_MFMA_BF16;
+    ColArgs a;
+    a.C = s->C;
+    a.Cprev = s->Cprev;
+    a.slot = (double*)s->hist->data +
+             (int64_t)(soe ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
+    a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
+    a.coef = s->ccoef;
+    a.part = s->hist->part;
+    a.ldh = s->hist->ld;
+    a.nch = nch;
+    a.ld = s->ld;
+    a.lines = s->sN - 1;
+    a.n = s->NY;
+    a.NX = s->NX;
+    a.negsk = -s->sg * s->kc0;
+    a.S_half = s->S * 0.5;
+    a.sg = s->sg;
+    a.kch = s->kch;
+    a.b0 = 0.0;
+    a.al1 = 1.0;
+    a.groups = col_groups(s->csh, a.lines);
+    return a;
+}
+
+static hipError_t upload_coefs(frm_lod2d* s, bool rows, SweepShape sh) {
+    const int lines = rows ? s->M - 1 : s->sN - 1, n = rows ? s->NX : s->NY;
+    const std::vector<double> lay =
+        interleave_coefs(rows ? s->rplain : s->cplain, lines, n, sh.cm, kWave * sh.w);
+    return hipMemcpy(rows ? s->rcoef : s->ccoef, lay.data(), lay.size() * sizeof(double),
+                     hipMemcpyHostToDevice);
+}
+
+// Autotuner of the sweep launch shape: for every wavefronts-per-line count
+// W in {1, 2, 4} that fits the line (or only `force`), lay the coefficients
+// out for it and time the row and the column sweep on the solver's own
+// buffers (an empty history, zero state: a step's full memory traffic and
+// arithmetic), then keep the fastest shape of each.  Runs at creation,
+// before the initial state is set; the state and the history slot it
+// writes are overwritten by frm_lod2d_set_state and the first step.
+static int tune_sweeps(frm_lod2d* s, int force) {
+    frm_ctx* ctx = s->ctx;
+    hipEvent_t e0, e1;
+    FRM_HIP(ctx, hipEventCreate(&e0));
+    FRM_HIP(ctx, hipEventCreate(&e1));
+    constexpr int kWarm = 2, kReps = 5;
+    int rc = FRM_OK;
+    for (int pass = 0; pass < 2 && rc == FRM_OK; ++pass) {
+        const bool rows = pass == 0;
+        const int n = rows ? s->NX : s->NY;
+        float best = 1e30f;
+        SweepShape pick;
+        for (int W : {1, 2, 4}) {
+            if ((force && W != force) || !pick_cm(n, W)) continue;
+            const SweepShape sh{pick_cm(n, W), W};
+            hipError_t e = upload_coefs(s, rows, sh);
+            if (rows) s->rsh = sh;
+            else s->csh = sh;
+            float ms = 0.f;
+            for (int it = 0; it < kWarm + kReps && e == hipSuccess; ++it) {
+                if (it == kWarm) e = hipEventRecord(e0, ctx->stream);
+                if (e != hipSuccess) break;
+                e = rows ? launch_rows(sh, ctx->stream, row_args(s, 0))
+                         : launch_cols(sh, ctx->stream, col_args(s, 0, 0));
+            }
+            if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                rc = check_hip(ctx, e, "sweep autotuning");
+                break;
+            }
+            if (ms < best) {
+                best = ms;
+                pick = sh;
+            }
+        }
+        if (rc == FRM_OK) {
+            if (rows) s->rsh = pick;
+            else s->csh = pick;
+            const hipError_t e = upload_coefs(s, rows, pick);
+            if (e != hipSuccess) rc = check_hip(ctx, e, "sweep coefficients");
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
 }
 
 static void lod_free(frm_lod2d* s) {
@@ -476,8 +627,12 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: mfma_parts %d (2 or 3)", d->mfma_parts);
     FRM_REQUIRE(ctx, d->kernel == FRM_KERNEL_CAPUTO || d->kernel == FRM_KERNEL_AB,
                 "frm_lod2d_create: kernel %d", d->kernel);
-    const int cmr = pick_cm(d->sN - 1), cmc = pick_cm(d->M - 1);
-    FRM_REQUIRE(ctx, cmr && cmc, "frm_lod2d_create: lines longer than %d unknowns", 64 * 32);
+    FRM_REQUIRE(ctx, pick_cm(d->sN - 1, 4) && pick_cm(d->M - 1, 4),
+                "frm_lod2d_create: lines longer than %d unknowns", 4 * 64 * 32);
+    FRM_REQUIRE(ctx, d->sweep_w == 0 || d->sweep_w == 1 || d->sweep_w == 2 || d->sweep_w == 4,
+                "frm_lod2d_create: sweep_w %d (0, 1, 2 or 4)", d->sweep_w);
+    FRM_REQUIRE(ctx, d->sweep_w == 0 || (pick_cm(d->sN - 1, d->sweep_w) && pick_cm(d->M - 1, d->sweep_w)),
+                "frm_lod2d_create: lines too long for sweep_w %d", d->sweep_w);
     FRM_HIP(ctx, hipSetDevice(ctx->device));
 
     frm_lod2d* s = new frm_lod2d();
@@ -494,8 +649,6 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     s->C1 = d->C1;
     s->S = d->S;
     s->sg = d->sigma / d->tau;
-    s->cm_rows = cmr;
-    s->cm_cols = cmc;
     // Caputo: the kernel's integrals over the LOD sub-intervals, as
     // TG_solver::kb (rows target t_{n+1/2}, columns t_{n+1}).
     // Atangana-Baleanu: the kernel is bounded, so a half-interval current
@@ -536,10 +689,17 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     hipError_t e = hipMalloc(&s->C, nn * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->Cprev, nodes * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&s->w, (size_t)2 * d->capacity * sizeof(double));
+    // coefficient buffers sized for the longest padded line of any shape
+    auto padded = [](int n) {
+        int mx = 0;
+        for (int W : {1, 2, 4})
+            if (pick_cm(n, W)) mx = std::max(mx, pick_cm(n, W) * kWave * W);
+        return mx;
+    };
     if (e == hipSuccess)
-        e = hipMalloc(&s->rcoef, (size_t)3 * (d->M - 1) * cmr * kWave * sizeof(double));
+        e = hipMalloc(&s->rcoef, (size_t)3 * (d->M - 1) * padded(s->NX) * sizeof(double));
     if (e == hipSuccess)
-        e = hipMalloc(&s->ccoef, (size_t)3 * (d->sN - 1) * cmc * kWave * sizeof(double));
+        e = hipMalloc(&s->ccoef, (size_t)3 * (d->sN - 1) * padded(s->NY) * sizeof(double));
     if (e != hipSuccess) {
         lod_free(s);
         return check_hip(ctx, e, "hipMalloc(lod2d)");
@@ -570,8 +730,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         line_coefs(s->NY, d->colA + base, d->colB + base, s->ld, s->sg * s->kc0, 1.0, oa,
                    oa + lcols * s->NY, oa + 2 * lcols * s->NY);
     }
-    rc3 = interleave_coefs(rc3, (int)lrows, s->NX, cmr);
-    cc3 = interleave_coefs(cc3, (int)lcols, s->NY, cmc);
+    s->rplain = std::move(rc3);
+    s->cplain = std::move(cc3);
     s->hw.assign({s->kr0, s->kc0, s->kch});
     s->hw.insert(s->hw.end(), w.begin(), w.end());
     if (d->precision == FRM_PREC_SOE) {
@@ -623,13 +783,15 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         }
     }
     e = hipMemcpy(s->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(s->rcoef, rc3.data(), rc3.size() * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(s->ccoef, cc3.data(), cc3.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->C, 0, nn * sizeof(double));
     if (e != hipSuccess) {
         lod_free(s);
         return check_hip(ctx, e, "hipMemcpy(lod2d setup)");
+    }
+    rc = tune_sweeps(s, d->sweep_w);
+    if (rc) {
+        lod_free(s);
+        return rc;
     }
     *out = s;
     return FRM_OK;
@@ -667,8 +829,6 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     if (s->tstep + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
                          s->tstep + nsteps, s->capacity);
-    const double S_half = s->S * 0.5;
-    const int wpb = 256 / kWave;
     for (int it = 0; it < nsteps; ++it) {
         const int n = s->tstep;
         int nch = 0;
@@ -715,50 +875,12 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             if (rc) return rc;
         }
         {
-            RowArgs a;
-            a.C = s->C;
-            a.Cprev = s->Cprev;
-            a.coef = s->rcoef;
-            a.part = s->hist->part;
-            a.ldh = s->hist->ld;
-            a.nch = nch;
-            a.ld = s->ld;
-            a.lines = s->M - 1;
-            a.n = s->NX;
-            a.NX = s->NX;
-            a.negsk = -s->sg * s->kr0;
-            a.S_half = S_half;
-            a.sg = s->sg;
-            a.b0 = s->C1;
-            a.al1 = 0.0;
             LaunchTimer t(ctx, FRM_K_ROWS, 0.0);
-            FRM_HIP(ctx, launch_rows(s->cm_rows, dim3((a.lines + wpb - 1) / wpb), ctx->stream, a));
+            FRM_HIP(ctx, launch_rows(s->rsh, ctx->stream, row_args(s, nch)));
         }
         {
-            ColArgs a;
-            a.C = s->C;
-            a.Cprev = s->Cprev;
-            a.slot = (double*)s->hist->data +
-                     (int64_t)(soe ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
-            a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
-            a.coef = s->ccoef;
-            a.part = s->hist->part;
-            a.ldh = s->hist->ld;
-            a.nch = nch;
-            a.ld = s->ld;
-            a.lines = s->sN - 1;
-            a.n = s->NY;
-            a.NX = s->NX;
-            a.negsk = -s->sg * s->kc0;
-            a.S_half = S_half;
-            a.sg = s->sg;
-            a.kch = s->kch;
-            a.b0 = 0.0;
-            a.al1 = 1.0;
-            const int cw = s->cm_cols >= 32 ? col_width<32>() : col_width<16>();
-            a.groups = (a.lines + cw - 1) / cw;
             LaunchTimer t(ctx, FRM_K_COLS, 0.0);
-            FRM_HIP(ctx, launch_cols(s->cm_cols, dim3((a.groups + 7) / 8 * 8), ctx->stream, a));
+            FRM_HIP(ctx, launch_cols(s->csh, ctx->stream, col_args(s, nch, n)));
         }
         if (!soe) s->hist->depth += 1;
         s->tstep = n + 1;
@@ -797,6 +919,15 @@ int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out) {
         out[3 + k] = s->hw[3 + k];
         out[3 + n + k] = s->hw[3 + s->capacity + k];
     }
+    return FRM_OK;
+}
+
+int frm_lod2d_sweep_shape(const frm_lod2d* s, int32_t* out) {
+    if (!s || !out) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_sweep_shape: null");
+    out[0] = s->rsh.cm;
+    out[1] = s->rsh.w;
+    out[2] = s->csh.cm;
+    out[3] = s->csh.w;
     return FRM_OK;
 }
 

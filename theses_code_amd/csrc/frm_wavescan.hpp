@@ -47,6 +47,134 @@ __device__ inline void scan_affine_down(double& P, double& Q, int lane) {
     Q = lane == 63 ? 0.0 : Qe;
 }
 
+// Both scans with the wavefront's total map as well (lane 63's inclusive
+// forward composition, lane 0's inclusive backward composition), for lines
+// that span several wavefronts.
+__device__ inline void scan_affine_up_total(double& P, double& Q, int lane, double& Pt, double& Qt) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Po = __shfl_up(P, d);
+        const double Qo = __shfl_up(Q, d);
+        if (lane >= d) {
+            Q = P * Qo + Q;
+            P = P * Po;
+        }
+    }
+    Pt = __shfl(P, 63);
+    Qt = __shfl(Q, 63);
+    const double Pe = __shfl_up(P, 1);
+    const double Qe = __shfl_up(Q, 1);
+    P = lane == 0 ? 1.0 : Pe;
+    Q = lane == 0 ? 0.0 : Qe;
+}
+
+__device__ inline void scan_affine_down_total(double& P, double& Q, int lane, double& Pt,
+                                              double& Qt) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Po = __shfl_down(P, d);
+        const double Qo = __shfl_down(Q, d);
+        if (lane + d < 64) {
+            Q = P * Qo + Q;
+            P = P * Po;
+        }
+    }
+    Pt = __shfl(P, 0);
+    Qt = __shfl(Q, 0);
+    const double Pe = __shfl_down(P, 1);
+    const double Qe = __shfl_down(Q, 1);
+    P = lane == 63 ? 1.0 : Pe;
+    Q = lane == 63 ? 0.0 : Qe;
+}
+
+// Scratch in LDS for one line that spans W wavefronts.
+template <int W>
+struct LineShare {
+    double fP[W], fQ[W];  // forward wave totals
+    double bP[W], bQ[W];  // backward wave totals
+    double un;
+};
+
+// line_solve for a line spread over W wavefronts of one workgroup: lane
+// `lane` of wavefront `wl` (0..W-1) owns elements e0 = (64 wl + lane) CM ..
+// + CM - 1.  The wavefront totals meet in `sh`; every wavefront of the
+// workgroup must call this (it contains workgroup barriers), `active`
+// false for wavefronts without a line.
+template <int CM, int W>
+__device__ inline double line_solve_mw(const double (&a)[CM], const double (&f)[CM],
+                                       const double (&al)[CM], double (&v)[CM], int e0, int n,
+                                       double b0, int lane, int wl, LineShare<W>* sh, bool active,
+                                       const double* un_fixed = nullptr) {
+    // ---- forward
+    double P = 1.0, Q = 0.0, Pt = 1.0, Qt = 0.0;
+#pragma unroll
+    for (int k = 0; k < CM; ++k) {
+        if (e0 + k < n) {
+            const double p = f[k] * a[k];
+            const double q = -f[k] * v[k];
+            Q = p * Q + q;
+            P = p * P;
+        }
+    }
+    if (active) {
+        scan_affine_up_total(P, Q, lane, Pt, Qt);
+        if (lane == 0) {
+            sh->fP[wl] = Pt;
+            sh->fQ[wl] = Qt;
+        }
+    }
+    __syncthreads();
+    double x = 0.0;
+    if (active) {
+        double y = b0;  // value entering this wavefront's part of the line
+        for (int w = 0; w < wl; ++w) y = sh->fP[w] * y + sh->fQ[w];
+        x = P * y + Q;
+#pragma unroll
+        for (int k = 0; k < CM; ++k) {
+            if (e0 + k < n) {
+                x = f[k] * (a[k] * x - v[k]);
+                v[k] = x;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+            if (e0 + k == n - 1) sh->un = un_fixed ? *un_fixed : v[k] / (1.0 - al[k]);
+    }
+    __syncthreads();
+    const double un = active ? sh->un : 0.0;
+    // ---- backward
+    P = 1.0;
+    Q = 0.0;
+#pragma unroll
+    for (int k = CM - 1; k >= 0; --k) {
+        if (e0 + k < n) {
+            Q = al[k] * Q + v[k];
+            P = al[k] * P;
+        }
+    }
+    if (active) {
+        scan_affine_down_total(P, Q, lane, Pt, Qt);
+        if (lane == 0) {
+            sh->bP[wl] = Pt;
+            sh->bQ[wl] = Qt;
+        }
+    }
+    __syncthreads();
+    if (active) {
+        double y = un;  // value just above this wavefront's part of the line
+        for (int w = W - 1; w > wl; --w) y = sh->bP[w] * y + sh->bQ[w];
+        y = P * y + Q;
+#pragma unroll
+        for (int k = CM - 1; k >= 0; --k) {
+            if (e0 + k < n) {
+                y = al[k] * y + v[k];
+                v[k] = y;
+            }
+        }
+    }
+    return un;
+}
+
 // Solve one line whose Thomas alpha coefficients are known.
 //   forward : bt_e = f_e * (a_e * bt_{e-1} - om_e),  bt_{-1} = b0
 //   backward: u_e  = al_e * u_{e+1} + bt_e,           u_n = bt_{n-1} / (1 - al_{n-1})
