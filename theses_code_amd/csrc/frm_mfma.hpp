@@ -8,7 +8,6 @@
 namespace frm {
 
 constexpr int kBlockSteps = 64;  // steps that share one far-part product
-constexpr int kFarNodes = 64;    // nodes per far-part workgroup (ld is a multiple)
 constexpr int kSubSteps = 8;     // steps that share one mid-part sum
 
 constexpr int kFarChunk = 32;    // history slots per product k-chunk

@@ -127,18 +127,19 @@ __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f32c(
         const int64_t ldv = ldh / 2;
         const mt_fv2* q = reinterpret_cast<const mt_fv2*>(hist + (int64_t)s0 * ldh + j);
         const float* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
+        // 8 slots in flight per lane: the same 64 bytes as the FP64 kernel's 4
         int s = s0;
-        for (; s + 4 <= s1; s += 4) {
-            mt_fv2 x[4];
+        for (; s + 8 <= s1; s += 8) {
+            mt_fv2 x[8];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
+            for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
                 kahan_add(s0x, c0x, wp[-u] * x[u].x);
                 kahan_add(s1x, c1x, wp[-u] * x[u].y);
             }
-            q += 4 * ldv;
-            wp -= 4;
+            q += 8 * ldv;
+            wp -= 8;
         }
         for (; s < s1; ++s) {
             const mt_fv2 x = __builtin_nontemporal_load(q);
