@@ -1,0 +1,98 @@
+"""The product's host-side FP64 generators (include/frm_host.h) against the
+reference's golden vectors, the oracle restatements (themselves pinned to
+the goldens) and closed forms.  CPU only: these calls never touch a GPU."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from theses_code_amd import _native as N
+
+
+def test_gamma_matches_math():
+    for x in [0.05, 0.3, 0.5, 1.0, 1.7, 2.5, 4.2, 7.9, 11.5]:
+        assert N.host_gamma(x) == pytest.approx(math.gamma(x), rel=2e-15)
+    for x in [12.5, 20.0, 40.0, 100.0]:
+        assert N.host_gamma(x) == pytest.approx(math.gamma(x), rel=5e-14)
+
+
+def test_mittag_leffler_matches_reference(golden):
+    _, z = golden("mf")
+    got = np.array([N.host_mittag_leffler(*row) for row in z["ml_in"]])
+    np.testing.assert_allclose(got, z["ml_out"], rtol=1e-14, atol=1e-300)
+
+
+def test_mittag_leffler_closed_forms():
+    for zz in [0.3, 0.9, 2.0, -0.5, -2.0, -4.0]:
+        assert N.host_mittag_leffler(1.0, 1.0, zz, 1e-15) == pytest.approx(math.exp(zz), rel=1e-12)
+        want = math.cosh(math.sqrt(zz)) if zz > 0 else math.cos(math.sqrt(-zz))
+        assert N.host_mittag_leffler(2.0, 1.0, zz, 1e-15) == pytest.approx(want, rel=1e-11)
+
+
+def test_ab_weights_match_reference(golden):
+    m, z = golden("mf")
+    for q, (a, tau, maxl, maxk, eps, D) in enumerate(m["weights"]):
+        np.testing.assert_allclose(N.host_ab_weights(a, tau, maxl, maxk, eps, 0, D), z[f"w{q}"],
+                                   rtol=1e-14)
+
+
+@pytest.mark.parametrize("a", [0.3, 0.6, 0.9])
+def test_ab_integral_matches_oracle_both_branches(a):
+    # small tj: the double series; large tj: the integral representation
+    for tj, ts1, ts in [(0.01, 0.01, 0.0), (0.05, 0.02, 0.01), (0.8, 0.5, 0.4), (2.0, 1.5, 0.0),
+                        (5.0, 0.1, 0.0)]:
+        got = N.host_ab_integral(a, 50, 1000, 1e-10, tj, ts1, ts)
+        assert got == pytest.approx(O.ab_int(tj, ts1, ts, a, 50, 1000, 1e-10), rel=1e-14, abs=1e-300)
+
+
+def test_ab_exact_weights_against_quadrature():
+    # I(d) = int_{(d-1)tau}^{d tau} E_a(-a/(1-a) u^a) du by Gauss-Legendre with
+    # the substitution u = (d-1) tau + tau v^2 (the integrand's u^a cusp at 0)
+    alpha, tau, D = 0.7, 2e-3, 40
+    got = N.host_ab_weights(alpha, tau, 0, 0, 0.0, 1, D)
+    lam = alpha / (1 - alpha)
+    xg, wg = np.polynomial.legendre.leggauss(120)
+    v = 0.5 * (xg + 1)
+    for d in [1, 2, 3, 10, 26, 40]:
+        u = (d - 1) * tau + tau * v * v
+        f = np.array([O.mittag_leffler(alpha, 1.0, -lam * ui ** alpha, 1e-15) for ui in u])
+        want = 0.5 * np.sum(wg * f * 2 * tau * v)
+        assert got[d - 1] == pytest.approx(want, rel=1e-10)
+
+
+def test_caputo_lod_weights_match_oracle_kb():
+    alpha, tau, D = 0.7, 1e-3, 60
+    kr, kc, kh, wr, wc = N.host_caputo_lod_weights(alpha, tau, D)
+    n = D  # lag k is s = n - k
+    assert kr == pytest.approx(O.tg_kb(0, 1.0, alpha, tau, n, n, 0.5, 0, 0.5), rel=1e-11)
+    assert kc == pytest.approx(O.tg_kb(0, 1.0, alpha, tau, n, n, 1.0, 0.5, 0.5), rel=1e-11)
+    assert kh == pytest.approx(O.tg_kb(0, 1.0, alpha, tau, n, n, 1.0, 0, 0.5), rel=1e-11)
+    for k in [1, 2, 7, 30, 60]:
+        assert wr[k - 1] == pytest.approx(O.tg_kb(0, 1.0, alpha, tau, n, n - k, 0.5, 0, 1), rel=1e-11)
+        assert wc[k - 1] == pytest.approx(O.tg_kb(0, 1.0, alpha, tau, n, n - k, 1.0, 0, 1), rel=1e-11)
+
+
+def test_caputo_lod_weights_far_lags_keep_precision():
+    # at lag 1e6 the direct difference d^(1-a) - (d-1)^(1-a) loses ~6 digits;
+    # the closed form's leading behaviour (1-a) d^(-a) is the check
+    alpha, tau, D = 0.6, 1e-3, 1_000_000
+    _, _, _, _, wc = N.host_caputo_lod_weights(alpha, tau, D)
+    c = tau ** (1 - alpha) / (1 - alpha) / math.gamma(1 - alpha)
+    d = D + 1.0
+    lead = c * (1 - alpha) * d ** (-alpha) * (1 + alpha / (2 * d))
+    assert wc[-1] == pytest.approx(lead, rel=1e-11)
+
+
+@pytest.mark.parametrize("L,bound", [(32, 1e-8), (64, 1e-10)])
+def test_soe_fit_reports_its_true_error(L, bound):
+    alpha, tau, K, k0 = 0.6, 1e-3, 4000, 2
+    _, _, _, wr, wc = N.host_caputo_lod_weights(alpha, tau, K)
+    w = np.stack([wr, wc])
+    q, c, max_rel = N.host_soe_fit(w, k0, L, tau)
+    assert np.all((q > 0) & (q < 1))
+    k = np.arange(k0, K + 1)
+    approx = c @ (q[:, None] ** (k - k0)[None, :])
+    err = np.max(np.abs(approx - w[:, k0 - 1:]) / np.abs(w[:, k0 - 1:]))
+    assert err == pytest.approx(max_rel, rel=1e-3, abs=1e-15)
+    assert max_rel < bound

@@ -66,6 +66,13 @@ TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
 
 # name -> (restype, argtypes); every symbol include/frm.h declares
 SIGNATURES = {
+    # host-side generators (include/frm_host.h)
+    "frm_host_gamma": (_I, [_D, _P]),
+    "frm_host_mittag_leffler": (_I, [_D, _D, _D, _D, _P]),
+    "frm_host_ab_integral": (_I, [_D, ct.c_int32, ct.c_int32, _D, _D, _D, _D, _P]),
+    "frm_host_ab_weights": (_I, [_D, _D, ct.c_int32, ct.c_int32, _D, ct.c_int32, ct.c_int32, _P]),
+    "frm_host_caputo_lod_weights": (_I, [_D, _D, ct.c_int32, _P]),
+    "frm_host_soe_fit": (_I, [_P, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, _D, _P, _P, _P]),
     "frm_ctx_create": (_I, [_I, ct.POINTER(_V)]),
     "frm_ctx_destroy": (_I, [_V]),
     "frm_sync": (_I, [_V]),
@@ -223,3 +230,43 @@ class DeviceArray:
         if self.ptr:
             self.ctx.free(self.ptr)
             self.ptr = 0
+
+
+# ---------------------------------------------------------------- host generators
+def host_gamma(x: float) -> float:
+    out = _D()
+    _check(lib().frm_host_gamma(x, ct.byref(out)))
+    return out.value
+
+
+def host_mittag_leffler(a: float, b: float, z: float, eps: float) -> float:
+    out = _D()
+    _check(lib().frm_host_mittag_leffler(a, b, z, eps, ct.byref(out)))
+    return out.value
+
+
+def host_ab_integral(a, maxl, maxkiter, eps, tj, ts1, ts) -> float:
+    out = _D()
+    _check(lib().frm_host_ab_integral(a, maxl, maxkiter, eps, tj, ts1, ts, ct.byref(out)))
+    return out.value
+
+
+def host_ab_weights(a, tau, maxl, maxkiter, eps, exact, D) -> np.ndarray:
+    out = np.empty(D)
+    _check(lib().frm_host_ab_weights(a, tau, maxl, maxkiter, eps, int(exact), D, _dp(out)))
+    return out
+
+
+def host_caputo_lod_weights(alpha, tau, D):
+    out = np.empty(3 + 2 * D)
+    _check(lib().frm_host_caputo_lod_weights(alpha, tau, D, _dp(out)))
+    return out[0], out[1], out[2], out[3:3 + D], out[3 + D:]
+
+
+def host_soe_fit(w: np.ndarray, k0: int, L: int, tau: float):
+    """w: [ntargets][K] lag-indexed weights; returns (q[L], c[ntargets][L], max_rel)."""
+    w = np.ascontiguousarray(np.atleast_2d(w), dtype=np.float64)
+    nt, K = w.shape
+    q, c, mr = np.empty(L), np.empty(nt * L), _D()
+    _check(lib().frm_host_soe_fit(_dp(w), nt, K, k0, L, tau, _dp(q), _dp(c), ct.byref(mr)))
+    return q, c.reshape(nt, L), mr.value

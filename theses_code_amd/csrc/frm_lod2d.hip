@@ -75,19 +75,6 @@ struct frm_lod2d {
 
 namespace frm {
 
-// ------------------------------------------------------------- weights
-// kb(s, ja, sa, l) for g(t) = t depends on d = n + ja - s - sa only:
-//   Da tau^(1-a)/(1-a) * (d^(1-a) - (d-l)^(1-a)),   Da = 1/Gamma(1-a)
-// (the closed form of _kb_row, fr_d_t_g_2d.cpp:2937-2963).  The difference
-// of powers is evaluated as -d^(1-a) expm1((1-a) log1p(-l/d)), which keeps
-// full relative precision at large lags where the direct difference cancels.
-static double caputo_kb(double alpha, double tau, double d, double l) {
-    const double c = tau == 0 ? 0 : std::pow(tau, 1.0 - alpha) / (1.0 - alpha) / std::tgamma(1.0 - alpha);
-    if (d <= 0) return 0.0;
-    if (l >= d) return c * std::pow(d, 1.0 - alpha);
-    return -c * std::pow(d, 1.0 - alpha) * std::expm1((1.0 - alpha) * std::log1p(-l / d));
-}
-
 // ------------------------------------------------------------- kernels
 // A line of n unknowns is split into NL = 64 W chunks of CM elements, one
 // per lane of the W wavefronts that share the line (W = 1, 2 or 4: the

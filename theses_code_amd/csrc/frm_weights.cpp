@@ -302,6 +302,18 @@ void ab_exact_weights(double a, double tau, int D, double* out) {
 
 namespace frm {
 
+double gamma_cody(double x) { return cody_gamma(x); }
+
+// The difference of powers is evaluated as -d^(1-a) expm1((1-a) log1p(-l/d)),
+// which keeps full relative precision at large lags where the direct
+// difference cancels (the closed form of _kb_row, fr_d_t_g_2d.cpp:2937-2963).
+double caputo_kb(double alpha, double tau, double d, double l) {
+    const double c = tau == 0 ? 0 : std::pow(tau, 1.0 - alpha) / (1.0 - alpha) / std::tgamma(1.0 - alpha);
+    if (d <= 0) return 0.0;
+    if (l >= d) return c * std::pow(d, 1.0 - alpha);
+    return -c * std::pow(d, 1.0 - alpha) * std::expm1((1.0 - alpha) * std::log1p(-l / d));
+}
+
 double ml_eval(double a, double b, double z, double eps) {
     int k0 = (int)(std::log(eps * (1 - z)) / std::log(z));
     const int kmin = 1 + (int)((1 - b) / a);

@@ -27,6 +27,14 @@ class AbIntegrator {
 // pair sums ordered by decreasing magnitude and added level by level.
 double ml_eval(double a, double b, double z, double eps);
 
+// kb(s, ja, sa, l) of TG_solver for g(t) = t as a function of
+// d = n + ja - s - sa: Da tau^(1-a)/(1-a) (d^(1-a) - (d-l)^(1-a)), Da =
+// 1/Gamma(1-a), with the difference of powers in a cancellation-free form.
+double caputo_kb(double alpha, double tau, double d, double l);
+
+// Gamma(x), W. J. Cody's rational approximation (Stirling above 12).
+double gamma_cody(double x);
+
 // Accurate FP64 integrals I(d) = int_{(d-1)tau}^{d tau} E_a(-a/(1-a) u^a) du,
 // d = 1..D (series for d = 1, spectral representation otherwise).
 void ab_exact_weights(double a, double tau, int D, double* out);
