@@ -161,6 +161,27 @@ def run_mt1d(args, cfg_name):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
                         "traffic": None, "kernel": "k_mt_memterm_f64"}
+    if cfg_name == "c0" and args.cpu_steps > 0:
+        # the reference's own CPU case: the oracle's H_solver restatement on
+        # the whole run (all steps), the field compared with the GPU's
+        from oracle import oracle as O
+        p = probs[0]
+        o = O.MTSolver(p["N"], p.get("L", 1.0), p["alpha"], cfg["tau"], U0[0], form=p["form"],
+                       D=p["D"], top=p["top"], top_val=p["top_val"],
+                       bottom_cond=p["bottom_cond"], Hbottom=p["Hbottom"])
+        t0 = time.perf_counter()
+        o.step(K)
+        dcpu = time.perf_counter() - t0
+        ref = o.state()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        line["cpu_baseline"] = {"value": nodes * K / dcpu, "unit": "node-steps/s",
+                                "cores": threads, "kind": "port", "seconds": dcpu,
+                                "sample": f"the whole run ({K} steps) by oracle/frm_oracle.c "
+                                          "(H_solver::calc_step restatement; OpenMP over the "
+                                          "history sum once it exceeds 2e5 terms, as "
+                                          "mt_minimal.cpp:909)"}
+        line["parity"] = {"vs": "oracle H_solver restatement, whole run",
+                          "max_rel_err": float(np.max(np.abs(U64 - ref)) / np.max(np.abs(ref)))}
     if N.PREC_F32C in results:
         dt32, U32, kt32 = results[N.PREC_F32C]
         d = U32 - U64

@@ -780,6 +780,18 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         lod_free(s);
         return rc;
     }
+    // leave no trace of the tuning runs: zero state, Cprev and history slots
+    {
+        const size_t hb = (s->precision == FRM_PREC_MFMA_BF16 ? sizeof(float) : sizeof(double)) *
+                          (size_t)std::min(s->hist->capacity, kSoeLag0) * s->hist->ld;
+        e = hipMemset(s->C, 0, nn * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(s->Cprev, 0, nodes * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(s->hist->data, 0, hb);
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMemset(lod2d after tuning)");
+        }
+    }
     *out = s;
     return FRM_OK;
 }

@@ -29,8 +29,15 @@
 namespace frm {
 
 constexpr int kGroup = 128;       // nodes per memory-term wavefront
-constexpr int kMtCM = 16;         // unknowns per lane in the sweep (N - 1 <= 1024)
+constexpr int kMtCM = 16;         // largest unknowns per lane in the sweep (N - 1 <= 1024)
 constexpr int kMtMaxN = 64 * kMtCM + 1;
+
+// unknowns per lane for a batch whose largest problem has maxN intervals
+static int mt_pick_cm(int maxN) {
+    if (maxN - 1 <= 64 * 4) return 4;
+    if (maxN - 1 <= 64 * 8) return 8;
+    return 16;
+}
 
 struct MtProbDev {
     int N, form, top, bottom_cond, src;
@@ -865,8 +872,12 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             b.err = s->ab_err;
             {
                 LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
-                hipLaunchKernelGGL(k_ab1d_sweep<kMtCM>, dim3((unsigned)((s->nprob + 3) / 4)), dim3(256),
-                                   0, ctx->stream, b);
+                const dim3 g((unsigned)((s->nprob + 3) / 4));
+                switch (mt_pick_cm(s->maxN)) {
+                    case 4: hipLaunchKernelGGL(k_ab1d_sweep<4>, g, dim3(256), 0, ctx->stream, b); break;
+                    case 8: hipLaunchKernelGGL(k_ab1d_sweep<8>, g, dim3(256), 0, ctx->stream, b); break;
+                    default: hipLaunchKernelGGL(k_ab1d_sweep<16>, g, dim3(256), 0, ctx->stream, b);
+                }
                 FRM_HIP(ctx, hipGetLastError());
             }
             s->depth = depth + 1;
@@ -886,8 +897,12 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         a.picard = s->picard;
         {
             LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
-            hipLaunchKernelGGL(k_mt_sweep<kMtCM>, dim3((unsigned)((s->nprob + 3) / 4)), dim3(256), 0,
-                               ctx->stream, a);
+            const dim3 g((unsigned)((s->nprob + 3) / 4));
+            switch (mt_pick_cm(s->maxN)) {
+                case 4: hipLaunchKernelGGL(k_mt_sweep<4>, g, dim3(256), 0, ctx->stream, a); break;
+                case 8: hipLaunchKernelGGL(k_mt_sweep<8>, g, dim3(256), 0, ctx->stream, a); break;
+                default: hipLaunchKernelGGL(k_mt_sweep<16>, g, dim3(256), 0, ctx->stream, a);
+            }
             FRM_HIP(ctx, hipGetLastError());
         }
         if (k >= 3) s->depth = depth + 1;
