@@ -429,7 +429,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the history sums shard by node with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the history sums shard by node with no exchange)", file=sys.stderr)
         sys.exit(2)
     if args.config in ("c0", "c3"):
         run_mt1d(args, args.config)

@@ -11,9 +11,11 @@
 // Here one step is three launches on one stream:
 //   1. memterm: both half-step history sums in one pass over the increments
 //      (rows use kb(s,1/2,0,1), columns kb(s,1,0,1); both depend only on the
-//      lag n-s for g(t) = t), split along time into FP64 partials;
-//   2. rows: one wavefront per row -- RHS assembly, Thomas forward and back
-//      substitution (chunked over lanes, wavefront-scan seeds), boundary
+//      lag n-s for g(t) = t), split along time into FP64 partials -- or, for
+//      the chosen forms, the sum-of-exponentials recurrences or the blocked
+//      Toeplitz parts (frm_mfma.hip);
+//   2. rows: one to four wavefronts per row -- RHS assembly, Thomas forward
+//      and back substitution (chunked over lanes, scan seeds), boundary
 //      nodes, and a copy of C_n for step 3;
 //   3. cols: the same per column, plus the push of Delta_n = C_{n+1} - C_n
 //      into the history store.
@@ -51,7 +53,7 @@ struct frm_lod2d {
     double* C = nullptr;       // state, reference layout
     double* Cprev = nullptr;   // C_n at interior nodes, compact node order
     double* w = nullptr;       // [2][capacity] lag-indexed history weights
-    double* rcoef = nullptr;   // rows:    a, f, al interleaved per line (coef_at)
+    double* rcoef = nullptr;   // rows:    a, f, al interleaved per line (load_coefs)
     double* ccoef = nullptr;   // columns: the same
     frm::SweepShape rsh, csh;  // launch shapes of the row / column sweeps
     std::vector<double> rplain, cplain;  // [3][lines][n] coefficients, for re-layout

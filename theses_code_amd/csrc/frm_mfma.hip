@@ -8,10 +8,11 @@
 // dense product  F[t*64 + j][i] = sum_{s < N0} W[t*64 + j][s] Delta_s[i]  with
 // M = 128, K = N0, N = nodes -- read once per 64 steps instead of every step.
 // The reference forms every step's sum from scratch (fr_d_t_g_2d.cpp:
-// 3340-3437).  W and Delta are split into BF16 hi and lo parts and the three
-// products hi*hi + hi*lo + lo*hi accumulate in FP32 on
-// v_mfma_f32_16x16x32_bf16; the near part (at most 63 slots) is summed per
-// step in FP64.
+// 3340-3437).  W and Delta are split into two or three BF16 parts whose
+// cross products accumulate in FP32 on v_mfma_f32_16x16x32_bf16.  The rest
+// of the block is summed in FP64: every 8 steps the settled slots of the
+// block (k_block_mid), every step the F row, the mid row and at most 7
+// recent slots (k_block_near).
 #include <cstring>
 
 #include "frm_mfma.hpp"
