@@ -63,28 +63,40 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(case, cfg, depth, nsteps):
+def cpu_baseline(case, cfg, depth, nsteps, ab=None, steps_1thread=1):
+    """The oracle (a port of the reference's CPU path) on a window of LOD steps
+    at history depth `depth` (prefilled), with all OpenMP threads and then
+    with one."""
     from oracle import oracle as O
 
     sN, M, coefs, C0 = case
-    o = O.TGSolver(sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], 0.0, 0.0, *coefs, C0)
+    o = O.TGSolver(sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], 0.0, 0.0, *coefs, C0, ab=ab)
     o.prefill(depth, seed=12345)
-    o.step(1)  # first step at this depth also touches every snapshot once
+    o.step(1)  # first step at this depth also touches every snapshot (and makes the weights)
+    threads = O.get_threads()
     t0 = time.perf_counter()
     o.step(nsteps)
     dt = time.perf_counter() - t0
+    v1 = None
+    if steps_1thread > 0:
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        o.step(steps_1thread)
+        v1 = (sN - 1) * (M - 1) * steps_1thread / (time.perf_counter() - t0)
+        O.set_threads(threads)
     nodes = (sN - 1) * (M - 1)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     del o
     return {
         "value": nodes * nsteps / dt,
         "unit": "node-steps/s",
         "cores": threads,
+        "value_1thread": v1,
         "kind": "port",
-        "sample": (f"{nsteps} LOD steps of the {sN - 1}x{M - 1} workload at history depth "
-                   f"{depth + 1}..{depth + nsteps} (synthetic prefilled history; per-step cost is "
-                   f"linear in depth, so depth T/2 is the run average); oracle/frm_oracle.c, "
-                   f"OpenMP over the nodes of each line as fr_d_t_g_2d.cpp:3341"),
+        "sample": (f"{nsteps} LOD steps ({steps_1thread} on one thread) of the {sN - 1}x{M - 1} "
+                   f"workload at history depth {depth + 2}.. (synthetic prefilled history; "
+                   f"per-step cost is linear in depth, so depth T/2 is the run average); "
+                   f"oracle/frm_oracle.c, OpenMP over the nodes of each line as "
+                   f"fr_d_t_g_2d.cpp:3341"),
         "seconds": dt,
     }
 
@@ -161,6 +173,44 @@ def run_mt1d(args, cfg_name):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
                         "traffic": None, "kernel": "k_mt_memterm_f64"}
+    if cfg_name == "c3" and args.cpu_steps > 0:
+        # the oracle (H_solver restatement with the psi weights rebuilt every
+        # step, as the reference does) on whole runs of the first problems,
+        # all threads, then one problem on one thread; also parity of the
+        # batched FP64 GPU run on those problems
+        from oracle import oracle as O
+        offs = np.concatenate([[0], np.cumsum([p["N"] + 2 for p in probs])])
+
+        def oracle_run(i):
+            p = probs[i]
+            o = O.MTSolver(p["N"], p.get("L", 1.0), p["alpha"], cfg["tau"], U0[i], psi=kind,
+                           beta=p["beta"], form=p["form"], k0=p["k0"], b=p["b"], top=p["top"],
+                           top_val=p["top_val"], bottom_cond=p["bottom_cond"], picard=picard)
+            o.step(K)
+            return o.state()
+
+        threads = O.get_threads()
+        t0 = time.perf_counter()
+        par = 0.0
+        for i in range(2):
+            ref = oracle_run(i)
+            ub = U64[offs[i]:offs[i + 1]]
+            par = max(par, float(np.max(np.abs(ub - ref)) / np.max(np.abs(ref))))
+        dcpu = time.perf_counter() - t0
+        n2 = sum(p["N"] + 1 for p in probs[:2])
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        oracle_run(2)
+        d1 = time.perf_counter() - t0
+        O.set_threads(threads)
+        line["cpu_baseline"] = {"value": n2 * K / dcpu, "unit": "node-steps/s", "cores": threads,
+                                "value_1thread": (probs[2]["N"] + 1) * K / d1, "kind": "port",
+                                "seconds": dcpu,
+                                "sample": f"whole {K}-step runs of problems 0 and 1 (problem 2 on "
+                                          "one thread) by oracle/frm_oracle.c, psi weights rebuilt "
+                                          "every step as mt_psi_caputo.cpp:1461-1491"}
+        line["parity"] = {"vs": "oracle H_solver restatement, whole runs of problems 0 and 1",
+                          "max_rel_err": par}
     if cfg_name == "c0" and args.cpu_steps > 0:
         # the reference's own CPU case: the oracle's H_solver restatement on
         # the whole run (all steps), the field compared with the GPU's
@@ -421,6 +471,14 @@ def run_lod_forms(args, cfg_name):
             "plain_fp64": {"value": tot_nodes * K / t_plain, "seconds": t_plain,
                            "kernels_ms": kt_plain or None},
             "speedup_vs_plain_fp64": t_plain / out_forms[best]["seconds"]}
+    if args.cpu_steps > 0:
+        # the oracle on a window at depth T/2 of the first set (the AB sets
+        # with the reference's MF_int_choice weights)
+        r0 = runs[0]
+        c = dict(alpha=r0["alpha"], tau=cfg["tau"], sigma=cfg["sigma"])
+        line["cpu_baseline"] = cpu_baseline(r0["case"], c, depth=K // 2, nsteps=2 if cfg_name == "c2" else 1,
+                                            ab=(50, 1000, 1e-10) if cfg_name == "c4" else None,
+                                            steps_1thread=1)
     print(json.dumps(line))
     ctx.close()
 

@@ -1152,3 +1152,15 @@ double orc_ab_int(double tj, double ts1, double ts, double a, int maxl, int maxk
     mf_cache_free(&c);
     return v;
 }
+
+/* ===================================================================== */
+/* Thread count of the OpenMP loops (timing of the CPU baselines).        */
+/* ===================================================================== */
+#ifdef _OPENMP
+#include <omp.h>
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
+#else
+void orc_set_threads(int n) { (void)n; }
+int orc_get_threads(void) { return 1; }
+#endif

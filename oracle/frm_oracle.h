@@ -87,6 +87,9 @@ void orc_psi_weights(double alpha, double beta, double tau, int A, double* out);
 
 /* ---- W4/H5/S3: Atangana-Baleanu path of MF_int_series.cpp -------------- */
 double orc_mittag_leffler(double a, double b, double z, double eps);   /* :247-302 */
+/* OpenMP threads of the oracle's parallel loops (for the CPU baselines) */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* I(d) = MF_int_choice(d*tau, tau, 0, a, a, maxl, maxkiter, eps), d = 1..D */
 void orc_ab_weights(double a, double tau, int maxl, int maxkiter, double eps, int D, double* out);
 /* solve_and_check(a, T, n, m, simple=1, ...) (MF_int_series.cpp:875-958):

@@ -66,6 +66,9 @@ def lib():
             L.orc_ab_history.argtypes = [_D, _D, _I, _I, _D, _I, _P, _P]
             L.orc_ab_int.restype = _D
             L.orc_ab_int.argtypes = [_D, _D, _D, _D, _I, _I, _D]
+        if hasattr(L, "orc_set_threads"):
+            L.orc_set_threads.argtypes = [_I]
+            L.orc_get_threads.restype = _I
         _lib = L
     return _lib
 
@@ -189,6 +192,14 @@ def psi_weights(alpha, beta, tau, A) -> np.ndarray:
     out = np.empty(A)
     lib().orc_psi_weights(alpha, beta, tau, A, _p(out))
     return out
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().orc_get_threads()
 
 
 def mittag_leffler(a, b, z, eps) -> float:
