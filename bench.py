@@ -92,8 +92,9 @@ def cpu_baseline(case, cfg, depth, nsteps, ab=None, steps_1thread=1):
         "cores": threads,
         "value_1thread": v1,
         "kind": "port",
-        "sample": (f"{nsteps} LOD steps ({steps_1thread} on one thread) of the {sN - 1}x{M - 1} "
-                   f"workload at history depth {depth + 2}.. (synthetic prefilled history; "
+        "sample": (f"{nsteps} LOD step(s) ({steps_1thread} more on one thread) of the "
+                   f"{sN - 1}x{M - 1} workload from history depth {depth + 1} "
+                   f"(synthetic prefilled history; "
                    f"per-step cost is linear in depth, so depth T/2 is the run average); "
                    f"oracle/frm_oracle.c, OpenMP over the nodes of each line as "
                    f"fr_d_t_g_2d.cpp:3341"),
