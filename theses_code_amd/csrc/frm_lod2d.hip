@@ -759,7 +759,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 r -= (double)parts[p * w.size() + k];
             }
         }
-        s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk);
+        // d = 32 .. N0 + 32 for the largest block start N0 below the capacity
+        s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk) + 1;
         const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
         if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * kBlockSteps * s->hist->ld);

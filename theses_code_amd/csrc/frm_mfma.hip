@@ -26,6 +26,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kKc = 32;   // slots per MFMA k-chunk
 constexpr int kNt = 128;  // nodes per workgroup (8 column tiles)
 constexpr int kBpad = 40; // LDS row length of a node's 32 slots (bf16), padded
+static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block in pairs");
 
 // Workgroup: 4 wavefronts, 128 rows (2 targets x 64 steps) x 128 nodes.
 // Wavefront w owns row tiles 2w, 2w+1 (16 rows each) and all 8 column
@@ -38,21 +39,33 @@ constexpr int kBpad = 40; // LDS row length of a node's 32 slots (bf16), padded
 // ~2^-24 -- the FP32 accumulation's own level.
 //
 // The history chunk (32 slots x 128 nodes, FP32) is loaded as float4 (4
-// nodes of one slot per load, 4 slots per thread), one chunk ahead of the products
-// (register double buffer), split into BF16 parts and transposed to
-// node-major LDS rows [node][slot].  The weight operand of a chunk depends
-// only on d = N0 - s0 (Toeplitz), so its fragments come ready-made from a
-// table built once per solver (atab_build), in register order.
+// nodes of one slot per load, 4 slots per thread) with non-temporal loads
+// (the stream is read once; the weight table stays in L2), split into BF16
+// parts and transposed to node-major LDS rows [node][slot].  Two chunks are
+// in flight: the loop is unrolled by two over register buffers x (even
+// chunks) and y (odd chunks), each refilled right after its chunk's LDS
+// store, with the weight loads issued first so that waiting for them never
+// waits for the history behind them.
+//
+// The weight operand of a chunk depends only on d = N0 - s0 (Toeplitz): its
+// fragments come ready-made from a table built once per solver (atab_build).
+// Row tile rt + 2 of target t at lag base d equals row tile rt at d + 32
+// (rows j and j + 32, same k), i.e. the fragment rt held one chunk earlier;
+// so wavefront w owns row tiles r0 and r0 + 2 (r0 = 0, 1, 4, 5) and loads
+// one fragment per chunk, keeping the previous one.  The two-part form is
+// held to 168 registers (three workgroups per CU); the three-part form
+// would spill under that limit.
 template <int NP>
-__global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ hist, int64_t ld, int N0,
-                                                   const bf16x8* __restrict__ atab,
-                                                   float* __restrict__ F) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1 : 3, 8))) void k_block_far(
+    const float* __restrict__ hist, int64_t ld, int N0, const bf16x8* __restrict__ atab,
+    float* __restrict__ F) {
     constexpr int NS = NP == 6 ? 3 : 2;    // BF16 parts per operand
     constexpr int CT = kNt / 16;           // column tiles per workgroup
     constexpr int SPT = kKc * kNt / 4 / 256;  // slots per loader thread (4 nodes each)
     __shared__ __attribute__((aligned(16))) __bf16 sB[NS][kNt][kBpad];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kg = lane >> 4, lc = lane & 15;
+    const int r0 = (wave & 1) + 4 * (wave >> 1);  // row tiles r0, r0 + 2
     const int64_t i0 = (int64_t)blockIdx.x * kNt;
     f32x4 acc[2][CT];
 #pragma unroll
@@ -60,46 +73,45 @@ __global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ his
 #pragma unroll
         for (int b = 0; b < CT; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
     // loader map: nodes 4*ng .. 4*ng+3 of slots SPT*sq .. SPT*sq + SPT-1 of
-    // the chunk.  N0 is a multiple of the chunk length (block_far checks
+    // the chunk.  N0 is a multiple of two chunk lengths (block_far checks
     // it), so every chunk is whole and the loads need no guard: a guarded
     // vector load compiles to a select between global and private addresses
     // and a flat load, which serialises the prefetch.
     constexpr int NG = kNt / 4;
     const int ng = tid % NG, sq = tid / NG;
-    float4 x[SPT];
-    auto load = [&](int s0) {
+    float4 x[SPT], y[SPT];
+    auto load = [&](float4 (&d)[SPT], int s0) {
         const float* q = hist + (int64_t)(s0 + SPT * sq) * ld + i0 + 4 * ng;
 #pragma unroll
-        for (int u = 0; u < SPT; ++u) x[u] = *reinterpret_cast<const float4*>(q + (int64_t)u * ld);
-    };
-    // weight fragments of a chunk (from the table, L2-resident)
-    bf16x8 an[2][NS];
-    auto aload = [&](int s0) {
-#pragma unroll
-        for (int rr = 0; rr < 2; ++rr) {
-            const bf16x8* at = atab + ((int64_t)((N0 - s0) / kKc - 1) * 8 + wave * 2 + rr) * 3 * 64 + lane;
-#pragma unroll
-            for (int p = 0; p < NS; ++p) an[rr][p] = at[p * 64];
+        for (int u = 0; u < SPT; ++u) {
+            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(q + (int64_t)u * ld));
+            d[u] = make_float4(v.x, v.y, v.z, v.w);
         }
     };
-    // software pipeline: every global load of chunk c + 1 is issued after
-    // the barrier of chunk c and consumed at the top of the next iteration,
-    // so the products of chunk c wait on LDS only
-    load(0);
-    aload(0);
-    for (int s0 = 0; s0 < N0; s0 += kKc) {
+    // an: fragment of row tile r0 for the chunk at s0 (d = N0 - s0); acar:
+    // the one of the chunk before, which is row tile r0 + 2's for this chunk
+    bf16x8 an[NS], acar[NS];
+    auto afrag = [&](bf16x8 (&dst)[NS], int d) {
+        const bf16x8* at = atab + ((int64_t)(d / kKc - 1) * 8 + r0) * 3 * 64 + lane;
+#pragma unroll
+        for (int p = 0; p < NS; ++p) dst[p] = at[p * 64];
+    };
+    // one chunk: BF16 parts of d into LDS, then (after the barrier) the next
+    // loads, then the products
+    auto chunk = [&](float4 (&d)[SPT], int next_a, int next_h) {
         bf16x8 ap[2][NS];
 #pragma unroll
-        for (int rr = 0; rr < 2; ++rr)
-#pragma unroll
-            for (int p = 0; p < NS; ++p) ap[rr][p] = an[rr][p];
-        // ---- store the loaded chunk: BF16 parts, node-major, SPT slots packed
+        for (int p = 0; p < NS; ++p) {
+            ap[0][p] = an[p];
+            ap[1][p] = acar[p];
+            acar[p] = an[p];
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             float r[SPT];
 #pragma unroll
             for (int u = 0; u < SPT; ++u)
-                r[u] = q == 0 ? x[u].x : q == 1 ? x[u].y : q == 2 ? x[u].z : x[u].w;
+                r[u] = q == 0 ? d[u].x : q == 1 ? d[u].y : q == 2 ? d[u].z : d[u].w;
 #pragma unroll
             for (int p = 0; p < NS; ++p) {
                 bf16x4 v;
@@ -112,10 +124,10 @@ __global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ his
             }
         }
         __syncthreads();
-        if (s0 + kKc < N0) {
-            load(s0 + kKc);
-            aload(s0 + kKc);
-        }
+        if (next_a >= 0) afrag(an, N0 - next_a);
+        __builtin_amdgcn_sched_barrier(0);
+        if (next_h >= 0) load(d, next_h);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
             bf16x8 bp[NS];
@@ -137,10 +149,24 @@ __global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ his
             }
         }
         __syncthreads();
+    };
+    afrag(acar, N0 + kKc);  // row tile r0 + 2 of the first chunk
+    afrag(an, N0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(x, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(y, kKc);
+    __builtin_amdgcn_sched_barrier(0);
+    int s0 = 0;
+    for (; s0 + 2 * kKc < N0; s0 += 2 * kKc) {
+        chunk(x, s0 + kKc, s0 + 2 * kKc);
+        chunk(y, s0 + 2 * kKc, s0 + 3 * kKc);
     }
+    chunk(x, s0 + kKc, -1);  // the last pair loads no history
+    chunk(y, -1, -1);
 #pragma unroll
     for (int rr = 0; rr < 2; ++rr) {
-        const int rt = wave * 2 + rr;
+        const int rt = r0 + 2 * rr;
         const int t = rt >> 2;
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct)
@@ -156,7 +182,10 @@ __global__ __launch_bounds__(256) void k_block_far(const float* __restrict__ his
 // sub-block starting at N1 (N0 < N1 < N0 + 64),
 //   G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i],  j < 8,
 // in FP64 (two nodes per thread, 8-byte loads of the FP32 slots; the
-// weights are wavefront-uniform).
+// weights are wavefront-uniform).  N1 - N0 is a multiple of 8: the slots go
+// in batches of 8 whose loads are all issued before the first product, and
+// a batch's 8 x 8 products touch only the 15 lags N1 - s0 - 7 .. N1 - s0 + 7.
+// Terms are added in increasing s, as a plain loop over s would add them.
 __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ hist, int64_t ld,
                                                    int N0, int N1, const double* __restrict__ w,
                                                    int ldw, double* __restrict__ G) {
@@ -167,17 +196,35 @@ __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ his
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int j = 0; j < kSubSteps; ++j) g[t][j][0] = g[t][j][1] = 0.0;
-    for (int s = N0; s < N1; ++s) {
-        const float2 x = *reinterpret_cast<const float2*>(hist + (int64_t)s * ld + i);
-        const int l0 = N1 - s;  // lag of step N1 + j is l0 + j
+    for (int s0 = N0; s0 < N1; s0 += kSubSteps) {
+        float2 x[kSubSteps];
 #pragma unroll
-        for (int j = 0; j < kSubSteps; ++j) {
-            const bool ok = l0 + j <= ldw;  // lags past the capacity belong to no step
-            const double a = ok ? w[l0 + j - 1] : 0.0, b = ok ? w[ldw + l0 + j - 1] : 0.0;
-            g[0][j][0] += a * (double)x.x;
-            g[0][j][1] += a * (double)x.y;
-            g[1][j][0] += b * (double)x.x;
-            g[1][j][1] += b * (double)x.y;
+        for (int u = 0; u < kSubSteps; ++u)
+            x[u] = *reinterpret_cast<const float2*>(hist + (int64_t)(s0 + u) * ld + i);
+        // wl[t][k] = w_t(L0 + k), L0 = N1 - s0 - 7 >= 1; lags past the
+        // capacity belong to no step and read as 0
+        const int L0 = N1 - s0 - (kSubSteps - 1);
+        double wl[2][2 * kSubSteps - 1];
+#pragma unroll
+        for (int k = 0; k < 2 * kSubSteps - 1; ++k) {
+            const bool ok = L0 + k <= ldw;
+            const int l = ok ? L0 + k : ldw;  // in bounds either way: unconditional loads
+            const double v0 = w[l - 1], v1 = w[ldw + l - 1];
+            wl[0][k] = ok ? v0 : 0.0;
+            wl[1][k] = ok ? v1 : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kSubSteps; ++u) {
+            const double x0 = (double)x[u].x, x1 = (double)x[u].y;
+#pragma unroll
+            for (int j = 0; j < kSubSteps; ++j) {
+                // lag of step N1 + j from slot s0 + u
+                const int k = kSubSteps - 1 - u + j;
+                g[0][j][0] += wl[0][k] * x0;
+                g[0][j][1] += wl[0][k] * x1;
+                g[1][j][0] += wl[1][k] * x0;
+                g[1][j][1] += wl[1][k] * x1;
+            }
         }
     }
 #pragma unroll
@@ -190,15 +237,19 @@ __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ his
 
 // part[t*ld + i] = F[t*64 + n - N0][i] + G[t*8 + n - N1][i]
 //                  + sum_{N1 <= s < n} w_t(n - s) Delta_s[i],
-// four nodes per thread (16-byte loads of the FP32 slots), FP64 sums.
-__global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hist, int64_t ld,
-                                                    int N0, int N1, int n,
-                                                    const double* __restrict__ w, int ldw,
-                                                    const float* __restrict__ F, int use_F,
-                                                    const double* __restrict__ G, int use_G,
-                                                    double* __restrict__ part) {
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-    if (i >= ld) return;
+// four nodes per thread (16-byte loads of the FP32 slots), FP64 sums.  The
+// slot count CNT = n - N1 (< 8) is a template parameter, so the CNT slot
+// loads, the F row and the G row are all in flight together.
+template <int CNT>
+__device__ __forceinline__ void near_part(const float* __restrict__ hist, int64_t ld, int64_t i,
+                                          int N0, int N1, int n, const double* __restrict__ w,
+                                          int ldw, const float* __restrict__ F, int use_F,
+                                          const double* __restrict__ G, int use_G,
+                                          double* __restrict__ part) {
+    float4 x[CNT > 0 ? CNT : 1];
+#pragma unroll
+    for (int u = 0; u < CNT; ++u)
+        x[u] = *reinterpret_cast<const float4*>(hist + (int64_t)(N1 + u) * ld + i);
     double s0[4] = {0, 0, 0, 0}, s1[4] = {0, 0, 0, 0};
     if (use_F) {
         const int j = n - N0;
@@ -209,32 +260,49 @@ __global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hi
     }
     if (use_G) {
         const int j = n - N1;
-        const double* g0 = G + (int64_t)j * ld + i;
-        const double* g1 = G + (int64_t)(kSubSteps + j) * ld + i;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            s0[q] += g0[q];
-            s1[q] += g1[q];
-        }
-    }
-    for (int s = N1; s < n; ++s) {
-        const float4 x = *reinterpret_cast<const float4*>(hist + (int64_t)s * ld + i);
-        const double a = w[n - s - 1], b = w[ldw + n - s - 1];
-        s0[0] += a * (double)x.x, s1[0] += b * (double)x.x;
-        s0[1] += a * (double)x.y, s1[1] += b * (double)x.y;
-        s0[2] += a * (double)x.z, s1[2] += b * (double)x.z;
-        s0[3] += a * (double)x.w, s1[3] += b * (double)x.w;
+        const double2* g0 = reinterpret_cast<const double2*>(G + (int64_t)j * ld + i);
+        const double2* g1 = reinterpret_cast<const double2*>(G + (int64_t)(kSubSteps + j) * ld + i);
+        const double2 a0 = g0[0], a1 = g0[1], b0 = g1[0], b1 = g1[1];
+        s0[0] += a0.x, s0[1] += a0.y, s0[2] += a1.x, s0[3] += a1.y;
+        s1[0] += b0.x, s1[1] += b0.y, s1[2] += b1.x, s1[3] += b1.y;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        part[i + q] = s0[q];
-        part[ld + i + q] = s1[q];
+    for (int u = 0; u < CNT; ++u) {
+        const int lag = CNT - u;  // slot N1 + u, step n = N1 + CNT
+        const double a = w[lag - 1], b = w[ldw + lag - 1];
+        s0[0] += a * (double)x[u].x, s1[0] += b * (double)x[u].x;
+        s0[1] += a * (double)x[u].y, s1[1] += b * (double)x[u].y;
+        s0[2] += a * (double)x[u].z, s1[2] += b * (double)x[u].z;
+        s0[3] += a * (double)x[u].w, s1[3] += b * (double)x[u].w;
     }
+    *reinterpret_cast<double2*>(part + i) = make_double2(s0[0], s0[1]);
+    *reinterpret_cast<double2*>(part + i + 2) = make_double2(s0[2], s0[3]);
+    *reinterpret_cast<double2*>(part + ld + i) = make_double2(s1[0], s1[1]);
+    *reinterpret_cast<double2*>(part + ld + i + 2) = make_double2(s1[2], s1[3]);
+}
+
+__global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hist, int64_t ld,
+                                                    int N0, int N1, int n,
+                                                    const double* __restrict__ w, int ldw,
+                                                    const float* __restrict__ F, int use_F,
+                                                    const double* __restrict__ G, int use_G,
+                                                    double* __restrict__ part) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i >= ld) return;
+#define FRM_NEAR_CASE(c) \
+    case c: near_part<c>(hist, ld, i, N0, N1, n, w, ldw, F, use_F, G, use_G, part); break;
+    switch (n - N1) {
+        FRM_NEAR_CASE(0) FRM_NEAR_CASE(1) FRM_NEAR_CASE(2) FRM_NEAR_CASE(3)
+        FRM_NEAR_CASE(4) FRM_NEAR_CASE(5) FRM_NEAR_CASE(6) FRM_NEAR_CASE(7)
+        default: break;
+    }
+#undef FRM_NEAR_CASE
 }
 
 int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const void* atab,
               int atab_chunks, int nprod, float* F) {
-    if (ld % kNt || N0 <= 0 || N0 % kBlockSteps || N0 / kKc > atab_chunks ||
+    // the table must reach d = N0 + 32 (the carried fragment of the first chunk)
+    if (ld % kNt || N0 <= 0 || N0 % kBlockSteps || N0 / kKc + 1 > atab_chunks ||
         (nprod != 3 && nprod != 6))
         return -1;
     const dim3 grid((unsigned)(ld / kNt)), block(256);
