@@ -43,11 +43,13 @@ def parse():
     ap.add_argument("--parity-steps", type=int, default=12)
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
-    ap.add_argument("--config", default="c1", choices=["c1", "c0", "c2", "c3", "c4", "ab1d"],
+    ap.add_argument("--config", default="c1",
+                    choices=["c1", "c0", "c2", "c3", "c4", "ab1d", "ab1d_series"],
                     help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
-                         "c2: 2D LOD 1024x1024 plain FP64 vs series history; c3: 4096 "
+                         "c2: 2D LOD 1024x1024 plain FP64 vs the chosen forms; c3: 4096 "
                          "psi-Caputo problems, FP64 vs compensated FP32; c4: 16 AB sets, plain "
-                         "FP64 vs 64-term series")
+                         "FP64 vs the chosen forms; ab1d: 1024 AB test problems, full history; "
+                         "ab1d_series: 1024 AB test problems, the reference's series history")
     ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
     ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -255,17 +257,26 @@ def run_mt1d(args, cfg_name):
     ctx.close()
 
 
-def run_ab1d(args):
-    """Batched 1D Atangana-Baleanu solves (solve_and_check form): the batch,
-    the one-solver-per-problem drop-in on a sample, and the oracle's CPU solve
-    on a sample; parity of the batch against the oracle on a few problems."""
+def run_ab1d(args, series=False):
+    """Batched 1D Atangana-Baleanu solves (solve_and_check form; `series`:
+    the series history of simple = 0, else the full history of simple = 1):
+    the batch, the one-solver-per-problem drop-in on a sample, and the
+    oracle's CPU solve on a sample, which is also the parity check."""
     from oracle import oracle as O
     from theses_code_amd import _native as N
     from theses_code_amd import workloads as Wl
     from theses_code_amd.solvers import MT1DBatch
 
-    cfg = Wl.AB1D_WORKLOAD
-    probs = Wl.ab1d_case(nprob=args.nprob or cfg["nprob"])
+    if series:
+        cfg = Wl.AB1D_SERIES_WORKLOAD
+        probs = Wl.ab1d_series_case(nprob=args.nprob or cfg["nprob"])
+        oracle_solve, kernel, n_cpu = O.ab_solve_series, "k_ab_series", 2
+        plain_cap = 16
+    else:
+        cfg = Wl.AB1D_WORKLOAD
+        probs = Wl.ab1d_case(nprob=args.nprob or cfg["nprob"])
+        oracle_solve, kernel, n_cpu = O.ab_solve, "k_mt_memterm_f64", 4
+        plain_cap = args.plain_sample
     K = args.steps if args.steps != 5000 else cfg["steps"]
     tau = cfg["tau"]
     nodes = sum(p["N"] + 1 for p in probs)
@@ -285,15 +296,8 @@ def run_ab1d(args):
     kt = {name: ctx.timing(k) for name, k in (("memterm", N.K_MEMTERM), ("sweep", N.K_SWEEP1D))}
     states = b.state()
     b.close()
-    # parity: the first problems against the oracle's solve_and_check
-    par = 0.0
-    for i in range(min(3, len(probs))):
-        p = probs[i]
-        _, Cref = O.ab_solve(p["alpha"], K * tau, K, p["N"], cfg["maxl"], cfg["maxkiter"],
-                             cfg["eps"], p["src"])
-        par = max(par, float(np.max(np.abs(states[i][:p["N"] + 1] - Cref)) / np.max(np.abs(Cref))))
     # the plain drop-in: one solver per problem, on a sample
-    P = min(args.plain_sample, len(probs))
+    P = min(plain_cap, len(probs))
     singles = [MT1DBatch(ctx, [probs[i]], tau, K, [U0[i]], kind=N.MT_AB) for i in range(P)]
     ctx.sync()
     t0 = time.perf_counter()
@@ -304,33 +308,34 @@ def run_ab1d(args):
     for s1 in singles:
         s1.close()
     sample_nodes = sum(p["N"] + 1 for p in probs[:P])
-    # CPU: the oracle's solve on a bounded sample of problems
-    cpu = None
-    if args.cpu_steps > 0:
-        t0 = time.perf_counter()
-        cn = 0
-        for p in probs[:4]:
-            O.ab_solve(p["alpha"], K * tau, K, p["N"], cfg["maxl"], cfg["maxkiter"], cfg["eps"],
-                       p["src"])
-            cn += p["N"] + 1
-        dcpu = time.perf_counter() - t0
-        cpu = {"value": cn * K / dcpu, "unit": "node-steps/s", "cores": 1, "kind": "port",
-               "sample": f"oracle solve_and_check restatement, first 4 problems, all {K} steps "
-                         "(weights included, as the reference computes them in the run)",
-               "seconds": dcpu}
+    # CPU: the oracle's whole solve of the first problems, one thread (the
+    # reference's solve_and_check has no parallel loop); parity against them
+    t0 = time.perf_counter()
+    cn, par = 0, 0.0
+    for i, p in enumerate(probs[:n_cpu]):
+        _, Cref = oracle_solve(p["alpha"], K * tau, K, p["N"], cfg["maxl"], cfg["maxkiter"],
+                               cfg["eps"], p["src"])
+        cn += p["N"] + 1
+        par = max(par, float(np.max(np.abs(states[i][:p["N"] + 1] - Cref)) / np.max(np.abs(Cref))))
+    dcpu = time.perf_counter() - t0
+    what = "series history (simple = 0)" if series else "full history (simple = 1)"
+    cpu = {"value": cn * K / dcpu, "unit": "node-steps/s", "cores": 1, "kind": "port",
+           "sample": f"oracle solve_and_check restatement, {what}, first {n_cpu} problems, all "
+                     f"{K} steps (weights included, as the reference computes them in the run)",
+           "seconds": dcpu}
     ms, nb = kt["memterm"][1], kt["memterm"][2]
     line = {"metric": METRIC, "value": nodes * K / dt, "unit": "node-steps/s", "n_gpus": 1,
             "steps": K, "warmup": 0, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": cfg["name"], "problems": len(probs), "nodes": nodes,
-                       "tau": tau, "parallelism": "1 GPU"},
+                       "tau": tau, "history": what, "parallelism": "1 GPU"},
             "setup_seconds": setup,
             "kernels_ms": {k: v[1] for k, v in kt.items()},
             "roofline": {"bound": "hbm", "achieved": nb / (ms * 1e-3) / 1e9 if ms else 0.0,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
-                         "traffic": None, "kernel": "k_mt_memterm_f64"},
-            "parity": {"vs": "oracle solve_and_check restatement", "problems": 3,
+                         "traffic": None, "kernel": kernel},
+            "parity": {"vs": f"oracle solve_and_check restatement, {what}", "problems": n_cpu,
                        "max_rel_err": par},
             "plain_fp64_per_problem": {"value": sample_nodes * K / dts, "seconds": dts,
                                        "problems_timed": P},
@@ -496,8 +501,8 @@ def main():
     if args.config in ("c2", "c4"):
         run_lod_forms(args, args.config)
         return
-    if args.config == "ab1d":
-        run_ab1d(args)
+    if args.config in ("ab1d", "ab1d_series"):
+        run_ab1d(args, series=args.config == "ab1d_series")
         return
 
     from theses_code_amd import _native as N

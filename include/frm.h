@@ -227,6 +227,7 @@ typedef struct frm_mt1d_problem {
     int32_t ab_maxkiter; /* inner series limit (0: 1000) */
     int32_t ab_exact;    /* 1: FP64-accurate kernel integrals */
     double ab_eps;       /* series tolerance of the current-step weight (0: 1e-10) */
+    int32_t ab_series;   /* 1: the series history of simple = 0 (see below) */
 } frm_mt1d_problem;
 
 typedef struct frm_mt1d_desc {
@@ -259,7 +260,19 @@ int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
  * at creation).  Fields used: N, alpha, src, ab_*; form/top/... ignored;
  * picard must be 1; precision FRM_PREC_F64.  frm_mt1d_ab_error returns per
  * problem the reference's error measure sum_steps sum_j (C_j - CC)^2 /
- * ((N+1) steps) over the steps taken since the last set_state. */
+ * ((N+1) steps) over the steps taken since the last set_state.
+ *
+ * ab_series = 1 takes the history as solve_and_check does with simple = 0
+ * (MF_int_series.cpp:917-922): per node, MF_intF_choice's accumulators
+ * (MF_intF's power series Lc1 while (a/(1-a)) t^a < 1, MF_intF_Fv's
+ * exponential series Lc2 with its log form on overflow, 569-853), fed with
+ * (C^{i-1}-C^{i-2})/tau; ab_maxl is the series length (and ab_eps the
+ * tolerance of its coefficients).  The node-independent coefficients of
+ * every step are formed on the host at creation (ab_maxl * 5 doubles per
+ * step and distinct (alpha, ab_maxl, ab_maxkiter, ab_eps)).  ab_exact must
+ * be 0.  When a series term is not finite the reference replaces the sum by
+ * a sorted pyramid sum (600-609); this path is not implemented and
+ * frm_mt1d_get_state then returns FRM_E_STATE. */
 int frm_mt1d_ab_error(frm_mt1d* s, double* out);
 
 #ifdef __cplusplus

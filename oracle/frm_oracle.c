@@ -1111,6 +1111,250 @@ void orc_ab_history(double a, double tau, int maxl, int maxkiter, double eps, in
 
 void orc_mf_cache_free(void* c) { mf_cache_free((mf_cache*)c); }
 
+/* ---- Series forms of the AB history: solve_and_check with simple = 0 ---- */
+
+/* What the reference keeps in function statics between calls: MF_intF's Kc
+ * (:576-590, refilled whenever n or maxl changes) and MF_intF_Fv's Kc and
+ * logKc (:732-777, emptied whenever n or maxl changes, refilled only by a
+ * call that is not append-only), plus MF_int_ksum's caches.  One state per
+ * solve: the reference process starts with empty statics. */
+typedef struct {
+    int n1, nk1;          /* old_n and Kc.size() of MF_intF */
+    int n2, nk2;          /* old_n and Kc.size() of MF_intF_Fv */
+    double *Kc1, *Kc2, *logKc2, *tmp;
+    int cap;
+    mf_cache kc;
+    int last_choice;      /* 1 series, 2 integral form (MF_intF_choice) */
+    long recounts;        /* times the non-finite branch of MF_intF ran */
+} mf_series_state;
+
+static void mf_series_init(mf_series_state* S, int maxl) {
+    memset(S, 0, sizeof(*S));
+    S->n1 = S->n2 = -1;
+    S->cap = maxl;
+    S->Kc1 = (double*)calloc(maxl, sizeof(double));
+    S->Kc2 = (double*)calloc(maxl, sizeof(double));
+    S->logKc2 = (double*)calloc(maxl, sizeof(double));
+    S->tmp = (double*)calloc(maxl, sizeof(double));
+}
+
+static void mf_series_free(mf_series_state* S) {
+    free(S->Kc1); free(S->Kc2); free(S->logKc2); free(S->tmp);
+    mf_cache_free(&S->kc);
+}
+
+/* MF_intF (:569-612) with oLc = Lc1 and is_Fvalue = 1.  With F_nosorting = 1
+ * (the reference's default) the `else` of the summation binds to the
+ * finiteness test: a non-finite term replaces the running sum by the sorted
+ * pyramid sum of all finite products, and the loop goes on adding. */
+static double mf_intF(mf_series_state* S, int n, double tau, double a, double b, int maxl,
+                      int maxkiter, double eps, double* lLc, int on, int mn, double Fvalue) {
+    double ret = 0.0;
+    int fill_kc = 0;
+    if ((maxl != S->nk1) || (n != S->n1)) {
+        S->nk1 = 0;
+        S->n1 = n;
+        fill_kc = 1;
+    }
+    if (fill_kc) {
+        mf_cache_setup(&S->kc, a, b, maxl, maxkiter);
+        for (int l = 0; l < maxl; l++) S->Kc1[l] = mf_ksum(&S->kc, l, n * tau, a, b, maxkiter, eps);
+        S->nk1 = maxl;
+    }
+    for (int t = on; t < n - mn; t++)
+        for (int l = 0; l < maxl; l++)
+            lLc[l] += (pow((t + 1) * tau, (l + 1.0)) - pow(t * tau, l + 1.0)) * Fvalue;
+    const double* Kc = S->Kc1;
+    for (int l = 0; l < maxl; l++) {
+        if ((isfinite(lLc[l])) && (isfinite(Kc[l])))
+            ret += lLc[l] * Kc[l];
+        else {
+            for (int q = 0; q < maxl; q++)
+                S->tmp[q] = ((isfinite(lLc[q])) && (isfinite(Kc[q]))) ? lLc[q] * Kc[q] : 0.0;
+            ret = mf_pyramid(S->tmp, maxl);
+            S->recounts++;
+        }
+    }
+    return ret / (1.0 - a);
+}
+
+/* the quadrature nodes of MF_intF_Fv / MF_intFv: maxl/3 equal steps up to
+ * mC2, maxl/3 up to mC1, then linearly growing steps (:744-753) */
+static void mf_fv_node(int l, int maxl, double wk1, double wk2, double wk3, double mC1, double mC2,
+                       double* wk, double* vk) {
+    if (l < (maxl / 3)) { *wk = wk1; *vk = l * *wk + 0.5 * *wk; }
+    else if (l < 2 * (maxl / 3)) { *wk = wk2; *vk = mC2 + (l - (maxl / 3)) * *wk + 0.5 * *wk; }
+    else { *wk = wk2 + (wk3 - wk2) * (double)(l - 2 * (maxl / 3)) / (maxl / 3); *vk = mC1 + (l - 2 * (maxl / 3)) * *wk + 0.5 * *wk; }
+}
+
+/* MF_intF_Fv (:720-836) with oLc = Lc2, ologLc = logLc2 and is_Fvalue = 1
+ * (F_nosorting = 1: the terms are added in order of l). */
+static double mf_intF_Fv(mf_series_state* S, int n, double tau, double a, int maxl, double eps,
+                         double* lLc, double* llogLc, int on, int mn, int only_append,
+                         double Fvalue) {
+    double ret = 0.0, wk, vk, C = 1.0, mC1, mC2, Fm, Fd, wk1, wk2, wk3;
+    double tj = n * tau;
+    int fill_kc = 0;
+    if ((maxl != S->nk2) || (n != S->n2)) {
+        S->nk2 = 0;
+        S->n2 = n;
+        fill_kc = 1;
+    }
+    do {
+        double ks = (tj - tau) * pow(C, 1.0 / a);
+        double ks1 = tj * pow(C, 1.0 / a);
+        double k2 = log(fabs(mf_Fv(a, C))) - (1.0 / a) * log(C) - pow(C, 1.0 / a) * tj;
+        double v = fabs(exp(ks + k2) - exp(ks1 + k2));
+        if (v > eps)
+            C *= 1.5;
+        else
+            break;
+    } while (1);
+    Fm = -a * cos(MF_PI * a) / (1.0 - a);
+    Fd = 5 * (sqrt(3.0) / 3) * sqrt((a * a / ((1.0 - a) * (1.0 - a))) - Fm * Fm);
+    mC1 = Fm + Fd;
+    mC2 = Fm - Fd;
+    if (mC2 < 0.0) mC2 = 0.0;
+    if (mC1 < 0.0) mC1 = 0.0;
+    if (mC2 == 0) mC2 = mC1 / 2.0;
+    if (mC1 == 0) { mC1 = C / 3.0; mC2 = 2.0 * C / 3.0; }
+    wk1 = mC2 / (maxl / 3);
+    wk2 = (mC1 - mC2) / (maxl / 3);
+    wk3 = 2.0 * (C - mC2) / (maxl / 3);
+    if (only_append == 0 && fill_kc) {
+        for (int l = 0; l < maxl; l++) {
+            mf_fv_node(l, maxl, wk1, wk2, wk3, mC1, mC2, &wk, &vk);
+            S->logKc2[l] = 0.0;
+            double kc = wk * pow(vk, -(1.0 / a)) * exp(-pow(vk, 1.0 / a) * tj) * mf_Fv(a, vk);
+            if (kc != 0.0)
+                S->Kc2[l] = kc;
+            else {
+                S->logKc2[l] = mf_sign(mf_Fv(a, vk));
+                S->Kc2[l] = log(wk) - (1.0 / a) * log(vk) - pow(vk, 1.0 / a) * tj + log(fabs(mf_Fv(a, vk)));
+            }
+        }
+        S->nk2 = maxl;
+    }
+    for (int t = on; t < n - mn; t++)
+        for (int l = 0; l < maxl; l++) {
+            mf_fv_node(l, maxl, wk1, wk2, wk3, mC1, mC2, &wk, &vk);
+            double sk = exp(pow(vk, 1.0 / a) * t * tau) - exp(pow(vk, 1.0 / a) * ((t + 1) * tau));
+            double olc = lLc[l];
+            if (llogLc[l] == 0) {
+                lLc[l] += sk * Fvalue;
+                if (!isfinite(lLc[l])) { /* keep log(Lc) on overflow */
+                    lLc[l] = log(fabs(olc));
+                    llogLc[l] = mf_sign(olc);
+                }
+            } else { /* the new sk is taken to dominate the old Lc */
+                sk = pow(vk, 1.0 / a) * t * tau + log(exp(pow(vk, 1.0 / a) * tau) - 1) + log(fabs(Fvalue));
+                lLc[l] = sk;
+                llogLc[l] = -mf_sign(Fvalue);
+            }
+        }
+    if (only_append == 1) return 0.0;
+    const double *Kc = S->Kc2, *logKc = S->logKc2;
+    for (int l = 0; l < maxl; l++) {
+        double v = 0.0;
+        if ((llogLc[l] == 0) && (logKc[l] == 0)) v = lLc[l] * Kc[l];
+        if ((llogLc[l] != 0) && (logKc[l] == 0)) v = llogLc[l] * mf_sign(Kc[l]) * exp(lLc[l] + log(fabs(Kc[l])));
+        if ((llogLc[l] == 0) && (logKc[l] != 0)) v = logKc[l] * mf_sign(lLc[l]) * exp(Kc[l] + log(fabs(lLc[l])));
+        if ((llogLc[l] != 0) && (logKc[l] != 0)) v = logKc[l] * llogLc[l] * exp(Kc[l] + lLc[l]);
+        if (!isfinite(v)) v = 0.0;
+        ret += v;
+    }
+    return ret / (1.0 - a);
+}
+
+/* MF_intF_choice (:839-853) as solve_and_check calls it (append_all = 1):
+ * the series of MF_intF while (a/(1-a)) tj^a < 1, keeping MF_intF_Fv's
+ * accumulators up to date meanwhile, then MF_intF_Fv alone. */
+static double mf_intF_choice(mf_series_state* S, int n, double tau, double a, double b, int maxl,
+                             int maxkiter, double eps, double* Lc1, double* Lc2, double* logLc2,
+                             int on, int mn, double Fvalue) {
+    double tj = n * tau;
+    double maxz = (a / (1 - a)) * pow(tj, a);
+    if ((maxz < 1.0) || (b != a)) {
+        mf_intF_Fv(S, n, tau, a, maxl, eps, Lc2, logLc2, on, mn, 1, Fvalue);
+        S->last_choice = 1;
+        return mf_intF(S, n, tau, a, b, maxl, maxkiter, eps, Lc1, on, mn, Fvalue);
+    }
+    S->last_choice = 2;
+    return mf_intF_Fv(S, n, tau, a, maxl, eps, Lc2, logLc2, on, mn, 0, Fvalue);
+}
+
+/* one node's series history as solve_and_check (simple = 0) forms it: for
+ * i = 2..nmax, out[i-2] = MF_intF_choice(i, ..., on = i-2, mn = 1,
+ * Fvalue = fvs[i-2]); choice[i-2] = the form used.  Returns the number of
+ * non-finite recounts of MF_intF. */
+long orc_ab_series_history(double a, double tau, int maxl, int maxkiter, double eps, int nmax,
+                           const double* fvs, double* out, double* choice) {
+    mf_series_state S;
+    mf_series_init(&S, maxl);
+    double* Lc = (double*)calloc(3 * (size_t)maxl, sizeof(double));
+    for (int i = 2; i <= nmax; ++i) {
+        out[i - 2] = mf_intF_choice(&S, i, tau, a, a, maxl, maxkiter, eps, Lc, Lc + maxl,
+                                    Lc + 2 * maxl, i - 2, 1, fvs[i - 2]);
+        if (choice) choice[i - 2] = S.last_choice;
+    }
+    long r = S.recounts;
+    free(Lc);
+    mf_series_free(&S);
+    return r;
+}
+
+/* solve_and_check with simple = 0 (MF_int_series.cpp:875-958): the series
+ * history of MF_intF_choice per node (Lc1, Lc2, logLc2), otherwise as
+ * orc_ab_solve.  Returns the mean squared error; C_out gets the final C. */
+double orc_ab_solve_series(double a, double T, int n, int m, int maxl, int maxkiter, double eps,
+                           int F, double* C_out) {
+    double h = 1.0 / m, tau = T / n;
+    double A = -(1.0 / (h * h)) - (1.0 / (2.0 * h));
+    double Cc = -(1.0 / (h * h)) + (1.0 / (2.0 * h));
+    double err = 0;
+    mf_cache c;
+    memset(&c, 0, sizeof(c));
+    mf_series_state S;
+    mf_series_init(&S, maxl);
+    double* C = (double*)calloc(m + 1, sizeof(double));
+    double* prevC = (double*)calloc(m + 1, sizeof(double));
+    double* Rp = (double*)calloc(m + 1, sizeof(double));
+    double* al = (double*)calloc(m + 1, sizeof(double));
+    double* bt = (double*)calloc(m + 1, sizeof(double));
+    double* Lc = (double*)calloc((size_t)(m + 1) * 3 * maxl, sizeof(double)); /* [j][3][maxl] */
+    double Ii = mf_int_choice(&c, tau, tau, 0, a, a, maxl, maxkiter, eps);
+    double B = (Ii / (1.0 - a)) + (2.0 / (h * h));
+    for (int i = 1; i <= n; i++) {
+        for (int j = 0; j <= m; j++) {
+            Rp[j] = (Ii / (1.0 - a)) * C[j] + mf_f(F, j * h, i * tau, a);
+            if (i >= 2) {
+                double prev_dtC = (C[j] - prevC[j]) / tau;
+                double* Lj = Lc + (size_t)j * 3 * maxl;
+                Rp[j] -= mf_intF_choice(&S, i, tau, a, a, maxl, maxkiter, eps, Lj, Lj + maxl,
+                                        Lj + 2 * maxl, i - 2, 1, prev_dtC);
+            }
+        }
+        for (int j = 0; j <= m; j++) prevC[j] = C[j];
+        al[0] = 0;
+        bt[0] = mf_CC(F, 0, i * tau);
+        for (int j = 1; j <= m; j++) {
+            al[j] = Cc / (B - A * al[j - 1]);
+            bt[j] = (Rp[j] - A * bt[j - 1]) / (B - A * al[j - 1]);
+        }
+        C[m] = mf_CC(F, m * h, i * tau);
+        for (int j = m - 1; j >= 0; j--) C[j] = bt[j] - al[j] * C[j + 1];
+        for (int j = 0; j <= m; j++) {
+            double d = fabs(C[j] - mf_CC(F, h * j, i * tau));
+            err += d * d;
+        }
+    }
+    if (C_out) memcpy(C_out, C, (m + 1) * sizeof(double));
+    free(C); free(prevC); free(Rp); free(al); free(bt); free(Lc);
+    mf_series_free(&S);
+    mf_cache_free(&c);
+    return err / ((m + 1.0) * n);
+}
+
 /* AB kernel in the LOD form: kb = MF_int_choice(d tau, l tau, 0) / (1 - a),
  * cached by (2d, l). */
 static double orc_tg_ab_weight(orc_tg* t, double d, double l) {

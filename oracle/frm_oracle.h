@@ -99,6 +99,14 @@ double orc_ab_solve(double a, double T, int n, int m, int maxl, int maxkiter, do
 /* per-node history of MF_intF_simple (mn = 1) for n = 2..nmax: out[n-2] */
 void orc_ab_history(double a, double tau, int maxl, int maxkiter, double eps, int nmax,
                     const double* fvs, double* out);
+/* one node's series history of solve_and_check with simple = 0 (MF_intF_choice
+ * with on = i-2, mn = 1) for i = 2..nmax: out[i-2], choice[i-2] (1 series, 2
+ * integral form; may be NULL).  Returns the count of non-finite recounts. */
+long orc_ab_series_history(double a, double tau, int maxl, int maxkiter, double eps, int nmax,
+                           const double* fvs, double* out, double* choice);
+/* solve_and_check(a, T, n, m, simple=0, ...): as orc_ab_solve with the series history */
+double orc_ab_solve_series(double a, double T, int n, int m, int maxl, int maxkiter, double eps,
+                           int F_func, double* C_out);
 /* MF_int_choice(tj, ts1, ts, a, a, maxl, maxkiter, eps) */
 double orc_ab_int(double tj, double ts1, double ts, double a, int maxl, int maxkiter, double eps);
 

@@ -66,6 +66,10 @@ def lib():
             L.orc_ab_history.argtypes = [_D, _D, _I, _I, _D, _I, _P, _P]
             L.orc_ab_int.restype = _D
             L.orc_ab_int.argtypes = [_D, _D, _D, _D, _I, _I, _D]
+            L.orc_ab_series_history.restype = ct.c_long
+            L.orc_ab_series_history.argtypes = [_D, _D, _I, _I, _D, _I, _P, _P, _P]
+            L.orc_ab_solve_series.restype = _D
+            L.orc_ab_solve_series.argtypes = [_D, _D, _I, _I, _I, _I, _D, _I, _P]
         if hasattr(L, "orc_set_threads"):
             L.orc_set_threads.argtypes = [_I]
             L.orc_get_threads.restype = _I
@@ -223,6 +227,22 @@ def ab_history(a, tau, maxl, maxkiter, eps, fvs) -> np.ndarray:
     out = np.empty(fvs.size - 1)
     lib().orc_ab_history(a, tau, maxl, maxkiter, eps, fvs.size, _p(fvs), _p(out))
     return out
+
+
+def ab_series_history(a, tau, maxl, maxkiter, eps, fvs):
+    """Series history of one node for i = 2..len(fvs)+1 (solve_and_check,
+    simple = 0): (values, choice per step, non-finite recount count)."""
+    fvs = np.ascontiguousarray(fvs, dtype=np.float64)
+    out, ch = np.empty(fvs.size), np.empty(fvs.size)
+    r = lib().orc_ab_series_history(a, tau, maxl, maxkiter, eps, fvs.size + 1, _p(fvs), _p(out),
+                                    _p(ch))
+    return out, ch, int(r)
+
+
+def ab_solve_series(a, T, n, m, maxl, maxkiter, eps, F_func=0):
+    C = np.empty(m + 1)
+    err = lib().orc_ab_solve_series(a, T, n, m, maxl, maxkiter, eps, F_func, _p(C))
+    return err, C
 
 
 def ab_int(tj, ts1, ts, a, maxl=50, maxkiter=1000, eps=1e-10) -> float:

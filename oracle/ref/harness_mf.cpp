@@ -13,6 +13,12 @@
 //      maxkiter, eps, F_func.  Output: the returned mean squared error.
 //   3: mittag_leffler(a, b, z, eps) (MF_int_series.cpp:247-302) for a list.
 //      Input: mode, count, then (a, b, z, eps) per entry.  Output: values.
+//   4: series history of one node as solve_and_check calls it with simple = 0
+//      (MF_int_series.cpp:917-922): for i = 2..nmax, MF_intF_choice(i, tau, a,
+//      a, maxl, maxkiter, eps, NULL, &Lc1, &Lc2, &logLc2, i-2, 1, 1, 1,
+//      Fvs[i-2]) in increasing i on one node's accumulators.  Input: mode, a,
+//      tau, maxl, maxkiter, eps, nmax, Fvs[nmax-1].  Output: the nmax-1 values
+//      and, per i, last_choice (1 series, 2 integral form).
 #include "harness_io.h"
 #define main ref_main
 #include "MF_int_series.cpp"
@@ -49,6 +55,20 @@ int main(int argc, char** argv) {
         double eps = in.d();
         F_func = in.i();
         out.d(solve_and_check(a, T, n, m, simple, maxl, maxkiter, eps));
+    } else if (mode == 4) {
+        double a = in.d(), tau = in.d();
+        int maxl = in.i(), maxkiter = in.i();
+        double eps = in.d();
+        int nmax = in.i();
+        std::vector<double> fvs(nmax - 1), choice;
+        in.arr(fvs.data(), nmax - 1);
+        std::vector<double> Lc1(maxl, 0.0), Lc2(maxl, 0.0), logLc2(maxl, 0.0);
+        for (int i = 2; i <= nmax; ++i) {
+            out.d(MF_intF_choice(i, tau, a, a, maxl, maxkiter, eps, NULL, &Lc1, &Lc2, &logLc2, i - 2,
+                                 1, 1, 1, fvs[i - 2]));
+            choice.push_back(last_choice);
+        }
+        for (double c : choice) out.d(c);
     } else if (mode == 3) {
         int cnt = in.i();
         for (int q = 0; q < cnt; ++q) {

@@ -133,6 +133,33 @@ def mf_all(name):
     save(name, meta, **arrays)
 
 
+def mf_series(name):
+    """solve_and_check with simple = 0: the series history of MF_intF_choice
+    per node (harness mode 4) and whole solves (mode 2), on the reference's
+    own test configurations (convection_diffusion_test: maxl = 1000;
+    convection_diffusion_test_time: 6000 steps, 100 intervals, maxl = 50)."""
+    arrays, meta = {}, dict(source="MF_int_series.cpp solve_and_check, simple = 0")
+    rng = np.random.default_rng(91)
+    # (a, tau, maxl, maxkiter, eps, nmax): series only, series then integral
+    # form (switch at i = 51 and i = 9), maxl = 1000
+    hcases = [(0.3, 0.02, 50, 1000, 1e-10, 60), (0.6, 0.01, 50, 1000, 1e-10, 80),
+              (0.9, 0.01, 50, 1000, 1e-10, 40), (0.5, 0.025, 1000, 1000, 1e-10, 41)]
+    for q, c in enumerate(hcases):
+        fvs = rng.normal(0, 1, c[-1] - 1)
+        arrays[f"fvs{q}"] = fvs
+        out = run("ref_mf", [4, *c, *fvs])
+        arrays[f"h{q}"] = out[: c[-1] - 1]
+        arrays[f"choice{q}"] = out[c[-1] - 1:]
+    meta["hist"] = hcases
+    # (a, T, n, m, simple, maxl, maxkiter, eps, F_func)
+    scases = [(0.9, 1.0, 6000, 100, 0, 50, 1000, 1e-10, 0), (0.5, 1.0, 6000, 100, 0, 50, 1000, 1e-10, 0),
+              (0.1, 1.0, 6000, 100, 0, 50, 1000, 1e-10, 0), (0.5, 1.0, 40, 40, 0, 1000, 1000, 1e-10, 1),
+              (0.7, 1.0, 80, 24, 0, 50, 1000, 1e-10, 2)]
+    arrays["solve_err"] = np.array([run("ref_mf", [2, *c])[0] for c in scases])
+    meta["solve"] = scases
+    save(name, meta, **arrays)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         sys.exit("the reference sources are needed to write golden vectors")
@@ -152,6 +179,7 @@ def main():
     psi_weights("psi_w", [(0.6, 0.8, 1e-3), (0.9, 1.4, 1e-3), (0.55, 1.0, 1e-2)],
                 [1, 2, 5, 17, 64, 200])
     mf_all("mf")
+    mf_series("mf_series")
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
-"""Batched 1D Atangana-Baleanu solve (solve_and_check of MF_int_series.cpp,
-simple = 1) on the GPU against the reference's own outputs (the `mf` golden:
-its error measure for three manufactured solutions) and the oracle's final
-fields.  Plain FP64: 1e-12 norm-wise relative on the field; the error
+"""Batched 1D Atangana-Baleanu solve (solve_and_check of MF_int_series.cpp)
+on the GPU against the reference's own outputs (the `mf` and `mf_series`
+goldens: its error measure for the three manufactured solutions, simple = 1
+full history and simple = 0 series history) and the oracle's final fields.  Plain FP64: 1e-12 norm-wise relative on the field; the error
 measure sum (C - CC)^2 is a difference-of-nearly-equal quantity (C - CC is
 ~1e-2..1e-4 of C), so it is held to 1e-9 relative."""
 import numpy as np
@@ -97,3 +97,65 @@ def test_ab1d_restart_and_capacity(ctx):
     assert np.array_equal(b.state()[0], first)
     assert b.ab_error()[0] == e1
     b.close()
+
+
+# ---------------------------------------------------------------- series history (simple = 0)
+def _series_prob(a, mm, maxl, F, maxk=1000, eps=1e-10):
+    return dict(N=mm, alpha=a, src=F, ab_maxl=maxl, ab_maxkiter=maxk, ab_eps=eps, ab_series=1)
+
+
+@pytest.mark.parametrize("q", range(5))
+def test_ab1d_series_matches_reference_golden(ctx, golden, q):
+    # the reference's own configurations: convection_diffusion_test_time
+    # (6000 steps, 100 intervals, maxl 50, orders 0.9 / 0.5 / 0.1) and
+    # convection_diffusion_test (maxl 1000), plus F_func 2
+    m, z = golden("mf_series")
+    a, T, n, mm, simple, maxl, maxk, eps, F = m["solve"][q]
+    print(f"series case {q}: a={a} n={n} m={mm} maxl={maxl} F={F}", flush=True)
+    b = _batch(ctx, [_series_prob(a, mm, maxl, F, maxk, eps)], T / n, n)
+    b.calc_step(n)
+    err = b.ab_error()[0]
+    C = b.state()[0][:mm + 1]
+    b.close()
+    _, Cref = O.ab_solve_series(a, T, n, mm, maxl, maxk, eps, F)
+    print(f"  err {err:.6e} golden {z['solve_err'][q]:.6e} field {relerr(C, Cref):.2e}")
+    assert relerr(C, Cref) <= TOL
+    assert err == pytest.approx(z["solve_err"][q], rel=TOL_ERR)
+
+
+# Series and full-history problems in one batch (the full-history sums skip
+# the series problems' nodes), orders on both sides of the series/integral
+# switch, against the oracle problem by problem.
+def test_ab1d_series_mixed_batch_matches_oracle(ctx):
+    tau, steps = 1.0 / 50, 50
+    spec = [(0.35, 30, 0, 1), (0.6, 64, 1, 1), (0.85, 129, 2, 0), (0.9, 40, 0, 1),
+            (0.5, 200, 2, 1), (0.72, 17, 1, 0)]
+    probs = [dict(N=mm, alpha=a, src=F, ab_maxl=50, ab_maxkiter=1000, ab_eps=1e-10, ab_series=sr)
+             for a, mm, F, sr in spec]
+    b = _batch(ctx, probs, tau, steps)
+    b.calc_step(23)
+    b.calc_step(steps - 23)
+    errs, states = b.ab_error(), b.state()
+    b.close()
+    for i, (a, mm, F, sr) in enumerate(spec):
+        solve = O.ab_solve_series if sr else O.ab_solve
+        e_ref, Cref = solve(a, steps * tau, steps, mm, 50, 1000, 1e-10, F)
+        assert relerr(states[i][:mm + 1], Cref) <= TOL, (i, relerr(states[i][:mm + 1], Cref))
+        assert errs[i] == pytest.approx(e_ref, rel=TOL_ERR), i
+
+
+def test_ab1d_series_restart_clears_accumulators(ctx):
+    prob = _series_prob(0.8, 20, 50, 2)
+    b = _batch(ctx, [prob], 1.0 / 30, 30)
+    b.calc_step(30)
+    first, e1 = b.state()[0].copy(), b.ab_error()[0]
+    b.set_state([np.zeros(22)])
+    b.calc_step(30)
+    assert np.array_equal(b.state()[0], first)
+    assert b.ab_error()[0] == e1
+    b.close()
+
+
+def test_ab1d_series_rejects_exact_weights(ctx):
+    with pytest.raises(N.FrmError):
+        _batch(ctx, [dict(_series_prob(0.5, 10, 50, 0), ab_exact=1)], 0.1, 10)

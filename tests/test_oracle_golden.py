@@ -145,6 +145,39 @@ def test_ab_weights_history_solve_match_reference(golden):
         assert err == pytest.approx(z["solve_err"][q], rel=1e-12)
 
 
+def test_ab_series_history_matches_reference(golden):
+    # MF_intF_choice on one node's accumulators as solve_and_check (simple = 0)
+    # calls it: series only, series then integral form, and maxl = 1000
+    m, z = golden("mf_series")
+    for q, (a, tau, maxl, maxk, eps, nmax) in enumerate(m["hist"]):
+        got, choice, recounts = O.ab_series_history(a, tau, maxl, maxk, eps, z[f"fvs{q}"])
+        np.testing.assert_array_equal(choice, z[f"choice{q}"])
+        np.testing.assert_allclose(got, z[f"h{q}"], rtol=1e-15, atol=0)
+        assert recounts == 0
+    assert set(np.concatenate([z[f"choice{q}"] for q in range(len(m["hist"]))])) == {1.0, 2.0}
+
+
+@pytest.mark.parametrize("q", [1, 3, 4])
+def test_ab_series_solve_matches_reference(golden, q):
+    # the reference's own configurations (convection_diffusion_test_time:
+    # 6000 steps, 100 intervals, maxl 50; convection_diffusion_test: maxl 1000)
+    m, z = golden("mf_series")
+    a, T, n, mm, simple, maxl, maxk, eps, F = m["solve"][q]
+    err, _ = O.ab_solve_series(a, T, n, mm, maxl, maxk, eps, F)
+    assert err == pytest.approx(z["solve_err"][q], rel=1e-14)
+
+
+def test_ab_series_first_step_is_the_kernel_integral():
+    # at i = 2 the history is one interval [0, tau] at tj = 2 tau: the series
+    # sum_l (tau^(l+1) - 0) Kc(l) F equals F I(2), I(2) the same series as
+    # MF_int_choice forms it for the full-history form
+    for a, tau in [(0.3, 0.02), (0.6, 0.01), (0.5, 1e-3)]:
+        F = 0.731
+        got, _, _ = O.ab_series_history(a, tau, 50, 1000, 1e-10, np.array([F]))
+        full = O.ab_history(a, tau, 50, 1000, 1e-10, np.array([F, 0.0]))
+        assert got[0] == pytest.approx(full[0], rel=1e-13)
+
+
 def test_mittag_leffler_matches_reference(golden):
     _, z = golden("mf")
     got = np.array([O.mittag_leffler(*row) for row in z["ml_in"]])

@@ -51,7 +51,7 @@ class MT1DProblem(ct.Structure):
                 ("bottom_cond", ct.c_int32), ("L", _D), ("alpha", _D), ("beta", _D), ("D", _D),
                 ("k", _D), ("k0", _D), ("b", _D), ("top_val", _D), ("Ev", _D), ("Hbottom", _D),
                 ("src", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
-                ("ab_exact", ct.c_int32), ("ab_eps", _D)]
+                ("ab_exact", ct.c_int32), ("ab_eps", _D), ("ab_series", ct.c_int32)]
 
 
 class MT1DDesc(ct.Structure):

@@ -43,6 +43,10 @@ struct MtProbDev {
     int N, form, top, bottom_cond, src;
     int64_t uoff, hoff;
     double dL, Dg, Da, alpha, beta, D, k, k0, b, top_val, Ev, Hbottom;
+    // kind AB with the series history: series length, offsets of the
+    // problem's per-step coefficient rows and phase choices
+    int ser, ser_maxl;
+    int64_t ser_off, ser_coff;
 };
 
 // ---------------------------------------------------------------- physics
@@ -473,7 +477,8 @@ struct AbSweepArgs {
     const MtProbDev* pr;
     int nprob;
     double* U;
-    const double* part;  // history sums sum_k I(lag+1) (C^{k+1}-C^k)/tau, [nch][ldh]
+    const double* part;  // history sums sum_k I(lag+1) (C^{k+1}-C^k)/tau, [nch][ldh];
+                         // series problems: MF_intF_choice's value in row 0
     int nch;
     int64_t ldh;
     const double* d0;    // I(1)/(1-alpha) per problem
@@ -513,10 +518,16 @@ __global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
         if (e < n) {
             const int j = e + 1;
             c0[k] = C[j];
-            double hs = 0.0;
-            for (int c = 0; c < a.nch; ++c) hs += a.part[(int64_t)c * a.ldh + P.hoff + e];
             double rp = Ii1a * c0[k] + ab_f(P.src, j * h, t, al, E);
-            if (a.k >= 2) rp -= hs / (1.0 - al);
+            if (a.k >= 2) {
+                if (P.ser) {  // MF_intF_choice's value, already over (1 - a)
+                    rp -= a.part[P.hoff + e];
+                } else {
+                    double hs = 0.0;
+                    for (int c = 0; c < a.nch; ++c) hs += a.part[(int64_t)c * a.ldh + P.hoff + e];
+                    rp -= hs / (1.0 - al);
+                }
+            }
             v[k] = rp;
             ca[k] = A;
             cf[k] = a.cf[P.hoff + e];
@@ -544,6 +555,86 @@ __global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) es += __shfl_xor(es, o);
     if (lane == 0) a.err[p] += es;  // the two ends equal CC exactly
+}
+
+// ---------------------------------------------------------------- AB series history
+struct AbSeriesArgs {
+    const MtProbDev* pr;
+    const int* gprob;       // 128-node group -> problem
+    int64_t ldh;
+    const double* tab;      // per-step coefficient rows (ab_series_tables)
+    const int* choice;      // per step: 1 series, 2 integral form
+    const double* F;        // (C^{i-1} - C^{i-2}) / tau per history node
+    double* lc1;            // [maxL][ldh] MF_intF's accumulators
+    double* lc2;            // [maxL][ldh] MF_intF_Fv's accumulators
+    signed char* lflag;     // [maxL][ldh] 0, or the sign of a log-form Lc2
+    double* part;           // out: the series value per history node
+    int i;                  // step, >= 2
+    int* bad;               // set when MF_intF meets a non-finite term
+};
+
+// One thread per node: MF_intF_choice(i, ..., on = i - 2, mn = 1, append_all
+// = 1, Fvalue = F) (MF_int_series.cpp:839-853) on the node's accumulators.
+// Both phases append to MF_intF_Fv's Lc2 (779-804: add sk F, or keep log|Lc2|
+// with its sign once the sum overflows, after which the newest term alone
+// is kept in log form); the series phase adds dp F to MF_intF's Lc1 and
+// returns sum Lc1 Kc (598-603), the integral phase returns sum of the
+// products of Lc2 and Kc with either in log form (806-822), terms in
+// increasing l, over (1 - a).  Products and sums are rounded separately, as
+// the reference's x86-64 build does.
+__global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
+#pragma clang fp contract(off)
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= a.ldh) return;
+    const int p = a.gprob[h / kGroup];
+    if (p < 0) return;
+    const MtProbDev& P = a.pr[p];
+    if (!P.ser || h - P.hoff >= P.N - 1) return;
+    const int maxl = P.ser_maxl;
+    const double* T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
+    const bool series = a.choice[P.ser_coff + a.i - 2] == 1;
+    const double F = a.F[h];
+    double ret = 0.0;
+    bool nonfinite = false;
+    for (int l = 0; l < maxl; ++l) {
+        const int64_t q = (int64_t)l * a.ldh + h;
+        double c2 = a.lc2[q];
+        int fl = a.lflag[q];
+        if (fl == 0) {
+            const double nv = c2 + T[3 * maxl + l] * F;
+            if (!isfinite(nv)) {
+                fl = c2 > 0.0 ? 1 : -1;
+                c2 = log(fabs(c2));
+            } else {
+                c2 = nv;
+            }
+        } else {
+            c2 = T[4 * maxl + l] + log(fabs(F));
+            fl = F > 0.0 ? -1 : 1;
+        }
+        a.lc2[q] = c2;
+        a.lflag[q] = (signed char)fl;
+        const double kc = T[l];
+        if (series) {
+            const double c1 = a.lc1[q] + T[2 * maxl + l] * F;
+            a.lc1[q] = c1;
+            if (isfinite(c1) && isfinite(kc))
+                ret += c1 * kc;
+            else
+                nonfinite = true;
+        } else {
+            const double lk = T[maxl + l];
+            double v = 0.0;
+            if (fl == 0 && lk == 0) v = c2 * kc;
+            if (fl != 0 && lk == 0) v = fl * (kc > 0.0 ? 1.0 : -1.0) * exp(c2 + log(fabs(kc)));
+            if (fl == 0 && lk != 0) v = lk * (c2 > 0.0 ? 1.0 : -1.0) * exp(kc + log(fabs(c2)));
+            if (fl != 0 && lk != 0) v = lk * fl * exp(kc + c2);
+            if (!isfinite(v)) v = 0.0;
+            ret += v;
+        }
+    }
+    if (nonfinite) atomicOr(a.bad, 2);
+    a.part[h] = ret / (1.0 - P.alpha);
 }
 
 }  // namespace frm
@@ -575,6 +666,14 @@ struct frm_mt1d {
     double* ab_cal = nullptr; // [ldh] -Al_j
     double* ab_ml = nullptr;  // [nprob][capacity] source factors
     double* ab_err = nullptr; // [nprob]
+    // kind AB, series history (ab_series)
+    int ser_maxL = 0, ser_count = 0;
+    int* gprob_mt = nullptr;  // group -> problem for the full-history sums (-1: series)
+    double* ser_tab = nullptr;
+    int* ser_choice = nullptr;
+    double* ser_lc = nullptr;        // [2][maxL][ldh]: Lc1, Lc2
+    signed char* ser_flag = nullptr; // [maxL][ldh]
+    std::vector<double> ser_bytes;   // per step i >= 2: algorithmic bytes of k_ab_series
 };
 
 static void mt_free(frm_mt1d* s) {
@@ -592,6 +691,11 @@ static void mt_free(frm_mt1d* s) {
     hipFree(s->ab_cal);
     hipFree(s->ab_ml);
     hipFree(s->ab_err);
+    hipFree(s->gprob_mt);
+    hipFree(s->ser_tab);
+    hipFree(s->ser_choice);
+    hipFree(s->ser_lc);
+    hipFree(s->ser_flag);
     delete s;
 }
 
@@ -614,20 +718,22 @@ static int ab_setup(frm_ctx* ctx, frm_mt1d* s, const frm_mt1d_desc* d) {
         const int maxl = q.ab_maxl > 0 ? q.ab_maxl : 1000;
         const int maxk = q.ab_maxkiter > 0 ? q.ab_maxkiter : 1000;
         const double eps = q.ab_eps > 0 ? q.ab_eps : 1e-10;
-        const auto key = std::make_tuple(a, maxl, maxk, eps, q.ab_exact ? 1 : 0);
+        const auto key = std::make_tuple(a, maxl, maxk, eps, q.ab_exact ? 1 : q.ab_series ? 2 : 0);
         auto it = icache.find(key);
         if (it == icache.end()) {
-            std::vector<double> I(cap + 1);  // I[d-1] = I(d)
+            std::vector<double> I(cap + 1, 0.0);  // I[d-1] = I(d)
             if (q.ab_exact) {
                 ab_exact_weights(a, tau, cap + 1, I.data());
             } else {
                 // the current weight takes the problem's eps, the history
                 // 1e-10 (MF_int_series.cpp:917 and the cached I of
-                // MF_intF_simple)
+                // MF_intF_simple); the series history needs I(1) only
                 AbIntegrator cur(a, maxl, maxk, eps);
-                AbIntegrator hist(a, maxl, maxk, 1e-10);
                 I[0] = cur.integral(tau, tau, 0.0);
-                for (int dd = 2; dd <= cap + 1; ++dd) I[dd - 1] = hist.integral(dd * tau, tau, 0.0);
+                if (!q.ab_series) {
+                    AbIntegrator hist(a, maxl, maxk, 1e-10);
+                    for (int dd = 2; dd <= cap + 1; ++dd) I[dd - 1] = hist.integral(dd * tau, tau, 0.0);
+                }
             }
             it = icache.emplace(key, std::move(I)).first;
         }
@@ -657,7 +763,73 @@ static int ab_setup(frm_ctx* ctx, frm_mt1d* s, const frm_mt1d_desc* d) {
         }
         std::copy(mt->second.begin(), mt->second.end(), ml.begin() + (size_t)p * cap);
     }
-    hipError_t e = hipMalloc(&s->ab_cf, sizeof(double) * s->ldh);
+    // series history: the node-independent coefficients of every step, per
+    // distinct (alpha, maxl, maxkiter, eps)
+    std::vector<double> stab;
+    std::vector<int> sch;
+    if (s->ser_count > 0 && cap >= 2) {
+        std::map<std::tuple<double, int, int, double>, std::pair<int64_t, int64_t>> scache;
+        for (int p = 0; p < s->nprob; ++p) {
+            MtProbDev& P = s->hprobs[p];
+            if (!P.ser) continue;
+            const frm_mt1d_problem& q = d->probs[p];
+            const int maxk = q.ab_maxkiter > 0 ? q.ab_maxkiter : 1000;
+            const double eps = q.ab_eps > 0 ? q.ab_eps : 1e-10;
+            const auto key = std::make_tuple(q.alpha, P.ser_maxl, maxk, eps);
+            auto it = scache.find(key);
+            if (it == scache.end()) {
+                const int64_t off = (int64_t)stab.size(), coff = (int64_t)sch.size();
+                const int64_t rows = (int64_t)(cap - 1) * kAbSeriesRows * P.ser_maxl;
+                if (off + rows > (int64_t(1) << 28))
+                    return set_error(ctx, FRM_E_INVALID,
+                                     "frm_mt1d_create: series coefficient tables of %.1f GB "
+                                     "(capacity x ab_maxl x %d doubles per distinct order)",
+                                     8.0 * (off + rows) / 1e9, kAbSeriesRows);
+                stab.resize(off + rows);
+                sch.resize(coff + cap - 1);
+                ab_series_tables(q.alpha, d->tau, P.ser_maxl, maxk, eps, cap, stab.data() + off,
+                                 sch.data() + coff);
+                it = scache.emplace(key, std::make_pair(off, coff)).first;
+            }
+            P.ser_off = it->second.first;
+            P.ser_coff = it->second.second;
+        }
+        // per node and series term: Lc2 and its flag read and written, and in
+        // the power-series phase Lc1 too; per node the increment and the value
+        s->ser_bytes.assign(cap - 1, 0.0);
+        for (int p = 0; p < s->nprob; ++p) {
+            const MtProbDev& P = s->hprobs[p];
+            if (!P.ser) continue;
+            for (int i = 2; i <= cap; ++i) {
+                const bool series = sch[P.ser_coff + i - 2] == 1;
+                s->ser_bytes[i - 2] += (double)(P.N - 1) * (P.ser_maxl * (series ? 34.0 : 18.0) + 16.0);
+            }
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (s->ser_count > 0) {
+        // the full-history sums skip the series problems' groups
+        std::vector<int> gm(s->ngroups);
+        e = hipMemcpy(gm.data(), s->gprob, sizeof(int) * gm.size(), hipMemcpyDeviceToHost);
+        for (int& g : gm)
+            if (g >= 0 && s->hprobs[g].ser) g = -1;
+        const size_t nl = (size_t)s->ser_maxL * s->ldh;
+        if (e == hipSuccess) e = hipMalloc(&s->gprob_mt, sizeof(int) * gm.size());
+        if (e == hipSuccess) e = hipMemcpy(s->gprob_mt, gm.data(), sizeof(int) * gm.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !stab.empty()) e = hipMalloc(&s->ser_tab, sizeof(double) * stab.size());
+        if (e == hipSuccess && !sch.empty()) e = hipMalloc(&s->ser_choice, sizeof(int) * sch.size());
+        if (e == hipSuccess && !stab.empty())
+            e = hipMemcpy(s->ser_tab, stab.data(), sizeof(double) * stab.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !sch.empty())
+            e = hipMemcpy(s->ser_choice, sch.data(), sizeof(int) * sch.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&s->ser_lc, sizeof(double) * 2 * nl);
+        if (e == hipSuccess) e = hipMalloc(&s->ser_flag, nl);
+        if (e == hipSuccess) e = hipMemset(s->ser_lc, 0, sizeof(double) * 2 * nl);
+        if (e == hipSuccess) e = hipMemset(s->ser_flag, 0, nl);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->probs, s->hprobs.data(), sizeof(MtProbDev) * s->nprob, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc(&s->ab_cf, sizeof(double) * s->ldh);
     if (e == hipSuccess) e = hipMalloc(&s->ab_cal, sizeof(double) * s->ldh);
     if (e == hipSuccess) e = hipMalloc(&s->ab_ml, sizeof(double) * ml.size());
     if (e == hipSuccess) e = hipMalloc(&s->ab_err, sizeof(double) * s->nprob);
@@ -700,9 +872,11 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     int64_t uoff = 0, hoff = 0;
     for (int p = 0; p < d->nprob; ++p) {
         const frm_mt1d_problem& q = d->probs[p];
+        const bool ser = is_ab && q.ab_series != 0;
+        const int ser_maxl = q.ab_maxl > 0 ? q.ab_maxl : 1000;
         if (q.N < 2 || q.N > kMtMaxN || !(q.alpha > 0 && q.alpha < 1) || !(q.L > 0) ||
             q.form < 0 || q.form > 2 || q.top < 0 || q.top > 2 ||
-            (is_ab && (q.src < 0 || q.src > 2))) {
+            (is_ab && (q.src < 0 || q.src > 2)) || (ser && (q.ab_exact || ser_maxl < 3))) {
             mt_free(s);
             return set_error(ctx, FRM_E_INVALID, "frm_mt1d_create: problem %d invalid (N=%d)", p,
                              q.N);
@@ -728,6 +902,12 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         P.top_val = q.top_val;
         P.Ev = q.Ev;
         P.Hbottom = q.Hbottom;
+        P.ser = ser ? 1 : 0;
+        P.ser_maxl = ser ? ser_maxl : 0;
+        if (ser) {
+            s->ser_count += 1;
+            s->ser_maxL = std::max(s->ser_maxL, ser_maxl);
+        }
         s->hprobs.push_back(P);
         const int ng = (q.N - 1 + kGroup - 1) / kGroup;
         for (int g = 0; g < ng; ++g) gp.push_back(p);
@@ -811,6 +991,12 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
         FRM_HIP(ctx, hipMemsetAsync(s->ab_err, 0, sizeof(double) * s->nprob, ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    if (s->ser_lc) {
+        const size_t nl = (size_t)s->ser_maxL * s->ldh;
+        FRM_HIP(ctx, hipMemsetAsync(s->ser_lc, 0, sizeof(double) * 2 * nl, ctx->stream));
+        FRM_HIP(ctx, hipMemsetAsync(s->ser_flag, 0, nl, ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return FRM_OK;
 }
 
@@ -835,7 +1021,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             FRM_HIP(ctx, hipGetLastError());
         }
         int nch = 0;
-        if (depth > 0) {
+        if (depth > 0 && s->ser_count < s->nprob) {
             const int64_t waves = s->ldh / (2 * kWave);
             nch = 1;
             if (waves < kMemtermWaves) nch = (int)((kMemtermWaves + waves - 1) / waves);
@@ -849,8 +1035,27 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
                                    chunk, s->part);
             else
                 hipLaunchKernelGGL(k_mt_memterm_f64, grid, dim3(kMemtermThreads), 0, ctx->stream,
-                                   (const double*)s->hist, s->ldh, depth, s->w, s->ldw, s->gprob,
-                                   chunk, s->part);
+                                   (const double*)s->hist, s->ldh, depth, s->w, s->ldw,
+                                   s->gprob_mt ? s->gprob_mt : s->gprob, chunk, s->part);
+            FRM_HIP(ctx, hipGetLastError());
+        }
+        if (s->kind == FRM_MT_AB && s->ser_count > 0 && k >= 2) {
+            AbSeriesArgs c;
+            c.pr = s->probs;
+            c.gprob = s->gprob;
+            c.ldh = s->ldh;
+            c.tab = s->ser_tab;
+            c.choice = s->ser_choice;
+            c.F = (const double*)s->hist + (size_t)(depth - 1) * s->ldh;  // step k - 1's increment
+            c.lc1 = s->ser_lc;
+            c.lc2 = s->ser_lc + (size_t)s->ser_maxL * s->ldh;
+            c.lflag = s->ser_flag;
+            c.part = s->part;
+            c.i = k;
+            c.bad = s->bad;
+            LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
+            hipLaunchKernelGGL(k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)), dim3(256), 0,
+                               ctx->stream, c);
             FRM_HIP(ctx, hipGetLastError());
         }
         if (s->kind == FRM_MT_AB) {
@@ -918,10 +1123,15 @@ int frm_mt1d_get_state(frm_mt1d* s, double* h_U) {
     FRM_HIP(ctx, hipMemcpyAsync(h_U, s->U, sizeof(double) * s->nU, hipMemcpyDeviceToHost, ctx->stream));
     FRM_HIP(ctx, hipMemcpyAsync(&bad, s->bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (bad)
+    if (bad & 1)
         return set_error(ctx, FRM_E_STATE,
                          "frm_mt1d: a psi weight series did not return a finite value (the "
                          "reference falls back to adaptive quadrature there)");
+    if (bad & 2)
+        return set_error(ctx, FRM_E_STATE,
+                         "frm_mt1d: a term of the AB power series was not finite (the reference "
+                         "recounts the sum by a sorted pyramid there, MF_int_series.cpp:600-609, "
+                         "which this path does not do)");
     return FRM_OK;
 }
 

@@ -129,6 +129,59 @@ double spectral(double a, double v) {  // Fv of the integral representation
 }
 }  // namespace
 
+// Upper limit C of the integral representation over [ts, ts1] at tj: grown
+// by 1.5 from 1 until the integrand's part beyond C falls to eps
+// (MF_intFv, MF_int_series.cpp:477-489; MF_intF_Fv uses ts = tj - tau,
+// ts1 = tj, :739-750).
+double fv_upper(double a, double tj, double ts1, double ts, double eps) {
+    double C = 1.0;
+    for (;;) {
+        const double ks = ts * std::pow(C, 1.0 / a), ks1 = ts1 * std::pow(C, 1.0 / a);
+        const double k2 = std::log(std::fabs(spectral(a, C))) - (1.0 / a) * std::log(C) -
+                          std::pow(C, 1.0 / a) * tj;
+        if (std::fabs(std::exp(ks + k2) - std::exp(ks1 + k2)) > eps)
+            C *= 1.5;
+        else
+            break;
+    }
+    return C;
+}
+
+// Quadrature nodes below C: maxl/3 equal steps up to mC2 = Fm - Fd, maxl/3 up
+// to mC1 = Fm + Fd (Fm the maximum of the spectral factor, Fd five times
+// its distance to the inflection points), then linearly growing steps
+// (MF_int_series.cpp:490-510, 744-753).
+FvNodes::FvNodes(double a, int maxl, double C) : maxl_(maxl) {
+    const double Fm = -a * std::cos(kPi * a) / (1.0 - a);
+    const double Fd = 5 * (std::sqrt(3.0) / 3) * std::sqrt((a * a / ((1.0 - a) * (1.0 - a))) - Fm * Fm);
+    mC1_ = Fm + Fd;
+    mC2_ = Fm - Fd;
+    if (mC2_ < 0.0) mC2_ = 0.0;
+    if (mC1_ < 0.0) mC1_ = 0.0;
+    if (mC2_ == 0) mC2_ = mC1_ / 2.0;
+    if (mC1_ == 0) {
+        mC1_ = C / 3.0;
+        mC2_ = 2.0 * C / 3.0;
+    }
+    wk1_ = mC2_ / (maxl / 3);
+    wk2_ = (mC1_ - mC2_) / (maxl / 3);
+    wk3_ = 2.0 * (C - mC2_) / (maxl / 3);
+}
+
+void FvNodes::node(int l, double& wk, double& vk) const {
+    const int third = maxl_ / 3;
+    if (l < third) {
+        wk = wk1_;
+        vk = l * wk + 0.5 * wk;
+    } else if (l < 2 * third) {
+        wk = wk2_;
+        vk = mC2_ + (l - third) * wk + 0.5 * wk;
+    } else {
+        wk = wk2_ + (wk3_ - wk2_) * (double)(l - 2 * third) / third;
+        vk = mC1_ + (l - 2 * third) * wk + 0.5 * wk;
+    }
+}
+
 AbIntegrator::AbIntegrator(double a, int maxl, int maxkiter, double eps)
     : a_(a), maxl_(maxl), maxkiter_(maxkiter), eps_(eps) {
     logi_.resize(maxl);
@@ -201,43 +254,13 @@ double AbIntegrator::integral(double tj, double ts1, double ts) {
         return ret;
     }
     // integral representation, quadrature over maxl nodes in three segments
-    const double eps = 1e-10;
+    const double C = fv_upper(a, tj, ts1, ts, 1e-10);
+    const FvNodes nd(a, maxl_, C);
     const int maxl = maxl_;
-    double C = 1.0;
-    for (;;) {
-        const double ks = ts * std::pow(C, 1.0 / a), ks1 = ts1 * std::pow(C, 1.0 / a);
-        const double k2 = std::log(std::fabs(spectral(a, C))) - (1.0 / a) * std::log(C) -
-                          std::pow(C, 1.0 / a) * tj;
-        if (std::fabs(std::exp(ks + k2) - std::exp(ks1 + k2)) > eps)
-            C *= 1.5;
-        else
-            break;
-    }
-    const double Fm = -a * std::cos(kPi * a) / (1.0 - a);
-    const double Fd = 5 * (std::sqrt(3.0) / 3) * std::sqrt((a * a / ((1.0 - a) * (1.0 - a))) - Fm * Fm);
-    double mC1 = Fm + Fd, mC2 = Fm - Fd;
-    if (mC2 < 0.0) mC2 = 0.0;
-    if (mC1 < 0.0) mC1 = 0.0;
-    if (mC2 == 0) mC2 = mC1 / 2.0;
-    if (mC1 == 0) {
-        mC1 = C / 3.0;
-        mC2 = 2.0 * C / 3.0;
-    }
-    const double wk1 = mC2 / (maxl / 3), wk2 = (mC1 - mC2) / (maxl / 3),
-                 wk3 = 2.0 * (C - mC2) / (maxl / 3);
     double ret = 0.0;
     for (int l = 0; l < maxl; l++) {
         double wk, vk;
-        if (l < (maxl / 3)) {
-            wk = wk1;
-            vk = l * wk + 0.5 * wk;
-        } else if (l < 2 * (maxl / 3)) {
-            wk = wk2;
-            vk = mC2 + (l - (maxl / 3)) * wk + 0.5 * wk;
-        } else {
-            wk = wk2 + (wk3 - wk2) * (double)(l - 2 * (maxl / 3)) / (maxl / 3);
-            vk = mC1 + (l - 2 * (maxl / 3)) * wk + 0.5 * wk;
-        }
+        nd.node(l, wk, vk);
         const double ks = ts * std::pow(vk, 1.0 / a), ks1 = ts1 * std::pow(vk, 1.0 / a);
         const double k2 = std::log(std::fabs(spectral(a, vk))) - (1.0 / a) * std::log(vk) -
                           std::pow(vk, 1.0 / a) * tj;
@@ -247,6 +270,54 @@ double AbIntegrator::integral(double tj, double ts1, double ts) {
         ret += lt;
     }
     return ret;
+}
+
+// Per-step coefficients of the series history of solve_and_check with
+// simple = 0: MF_intF_choice(i, tau, a, a, maxl, maxkiter, eps, ..., on =
+// i - 2, mn = 1, append_all = 1) (MF_int_series.cpp:839-853), which every
+// node of a problem calls with its own accumulators.  Everything that does
+// not depend on the node is formed here, with the reference's operations:
+//   series phase ((a/(1-a)) tj^a < 1): Kc = MF_int_ksum(l, tj) (MF_intF,
+//     :576-590) and dp = ((t+1) tau)^(l+1) - (t tau)^(l+1) (:598);
+//   integral phase: Kc = wk v^(-1/a) e^(-v^(1/a) tj) Fv(v), or its log with
+//     the flag sign(Fv) when that underflows to 0 (MF_intF_Fv, :755-777);
+//   both phases, MF_intF_Fv's accumulator (:779-804): sk = e^(v^(1/a) t tau)
+//     - e^(v^(1/a) (t+1) tau) and its log form v^(1/a) t tau +
+//     log(e^(v^(1/a) tau) - 1), to which the node adds log|F|,
+// with t = i - 2, tj = i tau and the node set v (wk) of step i.
+void ab_series_tables(double a, double tau, int maxl, int maxkiter, double eps, int cap,
+                      double* tab, int* choice) {
+    AbIntegrator ks(a, maxl, maxkiter, eps);
+    for (int i = 2; i <= cap; ++i) {
+        double* T = tab + (size_t)(i - 2) * kAbSeriesRows * maxl;
+        const double tj = i * tau;
+        const int t = i - 2;
+        const bool series = (a / (1 - a)) * std::pow(tj, a) < 1.0;
+        choice[i - 2] = series ? 1 : 2;
+        const double C = fv_upper(a, tj, tj, tj - tau, eps);
+        const FvNodes nd(a, maxl, C);
+        for (int l = 0; l < maxl; ++l) {
+            double wk, vk;
+            nd.node(l, wk, vk);
+            double kc = 0.0, lg = 0.0, dp = 0.0;
+            if (series) {
+                kc = ks.ksum(l, tj);
+                dp = std::pow((t + 1) * tau, (l + 1.0)) - std::pow(t * tau, l + 1.0);
+            } else {
+                kc = wk * std::pow(vk, -(1.0 / a)) * std::exp(-std::pow(vk, 1.0 / a) * tj) * spectral(a, vk);
+                if (kc == 0.0) {
+                    lg = sgn(spectral(a, vk));
+                    kc = std::log(wk) - (1.0 / a) * std::log(vk) - std::pow(vk, 1.0 / a) * tj +
+                         std::log(std::fabs(spectral(a, vk)));
+                }
+            }
+            T[0 * maxl + l] = kc;
+            T[1 * maxl + l] = lg;
+            T[2 * maxl + l] = dp;
+            T[3 * maxl + l] = std::exp(std::pow(vk, 1.0 / a) * t * tau) - std::exp(std::pow(vk, 1.0 / a) * ((t + 1) * tau));
+            T[4 * maxl + l] = std::pow(vk, 1.0 / a) * t * tau + std::log(std::exp(std::pow(vk, 1.0 / a) * tau) - 1);
+        }
+    }
 }
 
 }  // namespace frm

@@ -13,13 +13,39 @@ class AbIntegrator {
     // integral over [ts, ts1] of E_a(-a/(1-a) (tj - s)^a) ds
     double integral(double tj, double ts1, double ts);
 
-   private:
+    // coefficient of (ts1^(l+1) - ts^(l+1)) in the series form
+    // (MF_int_ksum, MF_int_series.cpp:332-435)
     double ksum(int l, double tj);
+
+   private:
     double a_;
     int maxl_, maxkiter_;
     double eps_;
     std::vector<double> logi_, lgk_, prodlog_, prodsgn_;
 };
+
+// Upper limit of the integral representation over [ts, ts1] at tj.
+double fv_upper(double a, double tj, double ts1, double ts, double eps);
+
+// The maxl quadrature nodes (weight wk, abscissa vk) below the upper limit C.
+class FvNodes {
+   public:
+    FvNodes(double a, int maxl, double C);
+    void node(int l, double& wk, double& vk) const;
+
+   private:
+    int maxl_;
+    double wk1_, wk2_, wk3_, mC1_, mC2_;
+};
+
+// Rows per step of ab_series_tables: Kc, its log flag, dp, sk, log sk.
+constexpr int kAbSeriesRows = 5;
+
+// Node-independent coefficients of the series history of solve_and_check
+// (simple = 0) for steps i = 2..cap: tab[(i-2)*kAbSeriesRows*maxl ...],
+// choice[i-2] = 1 (series) or 2 (integral form).
+void ab_series_tables(double a, double tau, int maxl, int maxkiter, double eps, int cap,
+                      double* tab, int* choice);
 
 // E_{a,b}(z) by the method of mittag_leffler (MF_int_series.cpp:247-302):
 // the power series for z > 0; for z <= 0 the integral representation when

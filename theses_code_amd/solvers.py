@@ -125,14 +125,15 @@ class MT1DBatch:
     problems' U arrays (N+2 values each, nodes 0..N and the ghost N+1)."""
 
     FIELDS = ("N", "form", "top", "bottom_cond", "L", "alpha", "beta", "D", "k", "k0", "b",
-              "top_val", "Ev", "Hbottom", "src", "ab_maxl", "ab_maxkiter", "ab_exact", "ab_eps")
+              "top_val", "Ev", "Hbottom", "src", "ab_maxl", "ab_maxkiter", "ab_exact", "ab_eps",
+              "ab_series")
 
     def __init__(self, ctx: N.Context, problems, tau: float, capacity: int, U0,
                  kind: int = N.MT_L1, precision: int = N.PREC_F64, picard: int = 1):
         self.ctx = ctx
         defaults = dict(form=N.FORM_CONST, top=N.TOP_AUTO, bottom_cond=1, L=1.0, beta=1.0, D=0.0,
                         k=0.0, k0=0.0, b=0.0, top_val=0.0, Ev=0.0, Hbottom=0.0, src=0,
-                        ab_maxl=0, ab_maxkiter=0, ab_exact=0, ab_eps=0.0)
+                        ab_maxl=0, ab_maxkiter=0, ab_exact=0, ab_eps=0.0, ab_series=0)
         arr = (N.MT1DProblem * len(problems))()
         for i, pr in enumerate(problems):
             q = {**defaults, **pr}

@@ -140,6 +140,25 @@ AB1D_WORKLOAD = dict(name="ab1d_batch1024", nprob=1024, Nmin=64, Nmax=1024, tau=
                      steps=2000, maxl=50, maxkiter=1000, eps=1e-10, seed=SEED_BASE + 5)
 
 
+# The series history of solve_and_check (simple = 0) at the shape of the
+# reference's convection_diffusion_test_time (MF_int_series.cpp:1128-1146:
+# 6000 steps on [0, 1], 100 intervals, 50 series terms, orders 0.9 .. 0.1),
+# batched: every problem one of those nine orders and one of the three
+# manufactured solutions.
+AB1D_SERIES_WORKLOAD = dict(name="ab1d_series_batch1024", nprob=1024, N=100, tau=1.0 / 6000,
+                            steps=6000, maxl=50, maxkiter=1000, eps=1e-10, seed=SEED_BASE + 6)
+
+
+def ab1d_series_case(nprob: int = AB1D_SERIES_WORKLOAD["nprob"],
+                     seed: int = AB1D_SERIES_WORKLOAD["seed"]):
+    rng = np.random.default_rng(seed)
+    alphas = np.round(np.arange(0.9, 0.05, -0.1), 1)
+    cfg = AB1D_SERIES_WORKLOAD
+    return [dict(N=cfg["N"], alpha=float(alphas[q % alphas.size]), src=int(rng.integers(0, 3)),
+                 ab_maxl=cfg["maxl"], ab_maxkiter=cfg["maxkiter"], ab_eps=cfg["eps"], ab_series=1)
+            for q in range(nprob)]
+
+
 def ab1d_case(nprob: int = AB1D_WORKLOAD["nprob"], seed: int = AB1D_WORKLOAD["seed"]):
     rng = np.random.default_rng(seed)
     alphas = np.linspace(0.3, 0.9, 16)
