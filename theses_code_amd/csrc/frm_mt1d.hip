@@ -591,15 +591,20 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
     const MtProbDev& P = a.pr[p];
     if (!P.ser || h - P.hoff >= P.N - 1) return;
     const int maxl = P.ser_maxl;
-    const double* T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
+    const double* __restrict__ T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
     const bool series = a.choice[P.ser_coff + a.i - 2] == 1;
     const double F = a.F[h];
+    // the accumulators are only this node's: no access aliases another, so
+    // the loads of several terms can be in flight together
+    double* __restrict__ lc1 = a.lc1 + h;
+    double* __restrict__ lc2 = a.lc2 + h;
+    signed char* __restrict__ lflag = a.lflag + h;
+    const int64_t ld = a.ldh;
     double ret = 0.0;
     bool nonfinite = false;
-    for (int l = 0; l < maxl; ++l) {
-        const int64_t q = (int64_t)l * a.ldh + h;
-        double c2 = a.lc2[q];
-        int fl = a.lflag[q];
+    // one term: the accumulators of term l as loaded (c2, fl, c1)
+    auto term = [&](int l, double c2, int fl, double c1) {
+        const int64_t q = (int64_t)l * ld;
         if (fl == 0) {
             const double nv = c2 + T[3 * maxl + l] * F;
             if (!isfinite(nv)) {
@@ -612,12 +617,12 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
             c2 = T[4 * maxl + l] + log(fabs(F));
             fl = F > 0.0 ? -1 : 1;
         }
-        a.lc2[q] = c2;
-        a.lflag[q] = (signed char)fl;
+        lc2[q] = c2;
+        lflag[q] = (signed char)fl;
         const double kc = T[l];
         if (series) {
-            const double c1 = a.lc1[q] + T[2 * maxl + l] * F;
-            a.lc1[q] = c1;
+            c1 = c1 + T[2 * maxl + l] * F;
+            lc1[q] = c1;
             if (isfinite(c1) && isfinite(kc))
                 ret += c1 * kc;
             else
@@ -632,6 +637,27 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
             if (!isfinite(v)) v = 0.0;
             ret += v;
         }
+    };
+    // the accumulators of 8 terms are loaded before any is updated (the
+    // terms are still added in increasing l)
+    constexpr int B = 8;
+    int l = 0;
+    for (; l + B <= maxl; l += B) {
+        double c2[B], c1[B];
+        int fl[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int64_t q = (int64_t)(l + u) * ld;
+            c2[u] = lc2[q];
+            fl[u] = lflag[q];
+            c1[u] = series ? lc1[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) term(l + u, c2[u], fl[u], c1[u]);
+    }
+    for (; l < maxl; ++l) {
+        const int64_t q = (int64_t)l * ld;
+        term(l, lc2[q], lflag[q], series ? lc1[q] : 0.0);
     }
     if (nonfinite) atomicOr(a.bad, 2);
     a.part[h] = ret / (1.0 - P.alpha);
