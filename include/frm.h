@@ -37,7 +37,9 @@ enum {
     FRM_PREC_F64 = 0,      /* plain FP64 history and FP64 accumulation */
     FRM_PREC_F32C = 1,     /* FP32 history, compensated (Kahan) FP32 sums */
     FRM_PREC_MFMA_BF16 = 2, /* blocked Toeplitz on MFMA, split hi+lo BF16, FP32 acc */
-    FRM_PREC_SOE = 3        /* sum-of-exponentials recurrences (series history) */
+    FRM_PREC_SOE = 3,       /* sum-of-exponentials recurrences (series history) */
+    FRM_PREC_BINOMIAL = 4   /* the reference's single-series expansion of the history
+                               (TG_solver dt_alg = 3, 2D LOD Caputo only) */
 };
 
 /* kernel-class ids for frm_timing_get */
@@ -167,6 +169,10 @@ typedef struct frm_lod2d_desc {
                             increments, ~2^-24); 0: 2 for Caputo, 3 for AB */
     int32_t sweep_w;     /* wavefronts per sweep line (1, 2, 4); 0: chosen at
                             creation by timing each (the autotuner) */
+    int32_t series_terms; /* FRM_PREC_BINOMIAL: moments per node and direction
+                             (the reference's dt_eps; 0: 25, max 1000) */
+    double series_tol;    /* FRM_PREC_BINOMIAL: int_err, where _kb_row2 stops
+                             taking coefficients (0: 1e-5, the reference's default) */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);

@@ -150,6 +150,13 @@ struct orc_tg {
     double *kb; /* [8][cap_kb]: the eight kb variants for s = 0..n */
     int cap_kb;
     double *Al, *Bt, *Om;
+    /* dt_alg 3: dt3 series terms; H1/H2 = Htmp1/Htmp2 [nn][dt3]; per full
+     * step i the coefficients v3 of _kb_row2 for [tau i, tau(i+1)] and how
+     * many of them row2_precalc holds at positions 1 and 5 of step i's block */
+    int dt3;
+    double *H1, *H2, *r2;
+    int *r2n;
+    int cap_r2;
 };
 
 static double* dup_arr(const double* src, size_t n) {
@@ -205,7 +212,109 @@ void orc_tg_destroy(orc_tg* t) {
     free(t->kb); free(t->Al); free(t->Bt); free(t->Om);
     free(t->ab_w);
     if (t->ab_cache) { orc_mf_cache_free(t->ab_cache); free(t->ab_cache); }
+    free(t->H1); free(t->H2); free(t->r2); free(t->r2n);
     free(t);
+}
+
+/* ---- dt_alg 3: single-series expansion of the history (fr_d_t_g_2d.cpp:3385-3428) */
+
+/* Coefficient k of _kb_row2 (fr_d_t_g_2d.cpp:2970-3038) for [t0, t1]: the
+ * integral of f'(x) x^k over [g(t0), g(t1)] over g(t1)^(k+1), f the inverse
+ * of g, by its series in (x - g(t1)) to int_err; v6 = (g(t0)/g(t1))^(k+1) is
+ * the caller's running power. */
+static double tg_row2_v3(const orc_tg* t, int k, double t0, double t1, double v6) {
+    const int func = t->func;
+    const double p = t->p, gt0 = tg_g(func, p, t0), gt1 = tg_g(func, p, t1);
+    if (k == 0) return (t1 - t0) / gt1;
+    double v3 = 0.0, v4, v5, v7 = 1;
+    int m = 0;
+    v4 = tg_inv_g_der(func, p, gt1, 1, tg_inv_g_der(func, p, gt1, 0, 0));
+    v5 = (1.0 / (k + 1.0)) * (1 - v6);
+    v3 += v4 * v5;
+    while (fabs(v4 * v5) > (v3 * t->int_err)) {
+        m++;
+        v4 = tg_inv_g_der(func, p, gt1, m + 1, v4);
+        v4 /= m;
+        v4 /= ((k + m + 1) / (gt1 * m));
+        if (m == 1)
+            v7 = (1 / (gt1 * m)) * (v6 * (gt0 - gt1));
+        else
+            v7 *= (gt0 - gt1) * ((m - 1.0) / m) * ((k + m) / (gt1 * (m - 1)));
+        v5 = -(v5 + v7);
+        v3 += v4 * v5;
+    }
+    return v3;
+}
+
+/* One call of _kb_row2 for full step i at target gtj, as far as row2_precalc
+ * is concerned: the do-while takes terms until dt_eps of them or until
+ * |v1 v2 v3| <= int_err (its `continue` goes to that test), reading cached
+ * coefficients and appending new ones; the cache only grows. */
+static void tg_row2_call(orc_tg* t, int i, int pos5, double gtj) {
+    const double a = t->alpha, t0 = t->tau * (i + 0.0), t1 = t->tau * (i + 0.0 + 1.0);
+    const double gt0 = tg_g(t->func, t->p, t0), gt1 = tg_g(t->func, t->p, t1);
+    double* v3s = t->r2 + (size_t)i * t->dt3;
+    int* cnt = t->r2n + 2 * (size_t)i + pos5;
+    double v1 = 1, v2 = pow(gtj, -a) * gt1, v6 = gt0 / gt1, v3;
+    int k = 0;
+    do {
+        if (k != 0) v1 *= -(-a - k + 1) / k;
+        if (k != 0) v2 *= gt1 / gtj;
+        if (k != 0) v6 *= gt0 / gt1;
+        if (k < *cnt) {
+            v3 = v3s[k];
+        } else {
+            v3 = tg_row2_v3(t, k, t0, t1, v6);
+            v3s[k] = v3;
+            *cnt = k + 1;
+        }
+        k++;
+        if (k > 10000) break; /* integr_max_niter */
+        if (k < t->dt3) continue;
+        break;
+    } while (fabs(v1 * v2 * v3) > t->int_err);
+}
+
+/* The _kb_row2 calls of kb()'s precalculation at step j (3112-3140) that feed
+ * Htmp: position 1 (ja 1/2, sa 0, l 1) for i = j-1, position 5 (ja 1, sa 0,
+ * l 1) for i = j-1 and i = j; interval [tau i, tau(i+1)] in both. */
+static void tg_dt3_precalc(orc_tg* t, int j) {
+    if (t->cap_r2 < j + 1) {
+        int c0 = t->cap_r2;
+        t->cap_r2 = 2 * (j + 1);
+        t->r2 = (double*)realloc(t->r2, (size_t)t->cap_r2 * t->dt3 * sizeof(double));
+        t->r2n = (int*)realloc(t->r2n, 2 * (size_t)t->cap_r2 * sizeof(int));
+        for (int q = 2 * c0; q < 2 * t->cap_r2; ++q) t->r2n[q] = 0;
+    }
+    for (int i = (j > 0 ? j - 1 : 0); i <= j; ++i) {
+        if (i == j - 1) tg_row2_call(t, i, 0, tg_g(t->func, t->p, t->tau * (j + 0.5)));
+        tg_row2_call(t, i, 1, tg_g(t->func, t->p, t->tau * (j + 1.0)));
+    }
+}
+
+/* _kb_row2_fixed_coefs (3041-3057): sum_k v1_k c_k, v1_0 = gtj^-a,
+ * v1_k = v1_{k-1} (a + k - 1)/(k gtj), stopping at a non-finite v1. */
+static double tg_fixed_coefs(double gtj, const double* c, int n, double alpha) {
+    double sum = 0.0, v1, i = 0;
+    int ii = 0;
+    v1 = pow(gtj, -alpha);
+    do {
+        sum += v1 * c[ii];
+        i += 1.0;
+        ii++;
+        v1 *= -(-alpha - i + 1.0) / (i * gtj);
+        if (!isfinite(v1)) break;
+    } while (ii != n);
+    return sum;
+}
+
+void orc_tg_set_dt3(orc_tg* t, int terms) {
+    size_t nn = (size_t)t->ld * (t->M + 2);
+    t->dt3 = terms;
+    free(t->H1);
+    free(t->H2);
+    t->H1 = (double*)calloc(nn * terms, sizeof(double));
+    t->H2 = (double*)calloc(nn * terms, sizeof(double));
 }
 
 /* weight variant index: bit0 ja==1, bit1 sa==1/2, bit2 l==1 (fr_d_t_g_2d.cpp:3077-3080) */
@@ -261,12 +370,32 @@ static void tg_line(orc_tg* t, int dir, int line, int n) {
         size_t id = NODE(i);
         double ts = 0;
         if (t->nold >= 2) {
-            if (t->nold >= 3)
+            if (t->nold >= 3 && !t->dt3)
                 for (int q = ((t->nold - 3) / 2) * 2; q >= 0; q -= 2) {
                     /* full-step increments, newest first: kb(s,1/2,0,1) or kb(s,1,0,1) */
                     double kv = dir == 0 ? TG_KB(t, 4, q / 2) : TG_KB(t, 5, q / 2);
                     ts += kv * (t->old[q + 2][id] - t->old[q][id]);
                 }
+            if (t->nold >= 3 && t->dt3) {
+                /* dt_alg 3: the newest full-step increment enters the node's
+                 * moments Htmp1 (rows) / Htmp2 (columns) with
+                 * g(tau(q/2+1))^(k+1) v3_k for the k < row2_precalc size, then
+                 * time_sum = Da _kb_row2_fixed_coefs(g(target), moments) */
+                int q = ((t->nold - 3) / 2) * 2;
+                double gtj = dir == 0 ? tg_g(t->func, t->p, (n + 0.5) * t->tau)
+                                      : tg_g(t->func, t->p, (n + 1.0) * t->tau);
+                double* H = (dir == 0 ? t->H1 : t->H2) + id * (size_t)t->dt3;
+                double diff = t->old[q + 2][id] - t->old[q][id];
+                const double* v3s = t->r2 + (size_t)(q / 2) * t->dt3;
+                int cnt = t->r2n[2 * (size_t)(q / 2) + (dir == 0 ? 0 : 1)];
+                double mult = tg_g(t->func, t->p, ((q / 2) + 1) * t->tau);
+                for (int k = 0; k < t->dt3; k++)
+                    if (cnt > k) {
+                        H[k] += mult * v3s[k] * diff;
+                        mult *= tg_g(t->func, t->p, ((q / 2) + 1) * t->tau);
+                    }
+                ts = (1.0 / orc_gamma(1.0 - t->alpha)) * tg_fixed_coefs(gtj, H, t->dt3, t->alpha);
+            }
             if (dir == 1) {
                 /* the column half-step adds kb(n,1,0,1/2)*(C_n - C_{n+1/2}) */
                 double kv = TG_KB(t, 1, (t->nold - 1) / 2);
@@ -293,6 +422,7 @@ static void tg_line(orc_tg* t, int dir, int line, int n) {
 void orc_tg_step(orc_tg* t) {
     int n = t->tstep;
     tg_fill_kb(t, n);
+    if (t->dt3) tg_dt3_precalc(t, n);
     for (int j = 1; j < t->M; ++j) tg_line(t, 0, j, n); /* rows, target n+1/2 */
     tg_push(t);                                          /* C_{n+1/2} */
     for (int i = 1; i < t->sN; ++i) tg_line(t, 1, i, n); /* columns, target n+1 */

@@ -56,6 +56,9 @@ void orc_tg_coefs_ref(int sN, int M, const double* V2, double dv, double dL, dou
  * 614-718): current I(1), history I(k+1)/2, no half-step correction. */
 enum { ORC_KERNEL_AB = 10 };
 void orc_tg_set_ab(orc_tg* t, int maxl, int maxkiter, double eps);
+/* dt_alg 3: the binomial-series history with `terms` moments per node and
+ * direction (fr_d_t_g_2d.cpp:3385-3428; int_alg 6), before the first step */
+void orc_tg_set_dt3(orc_tg* t, int terms);
 /* Grid spacings the reference constructor derives (fr_d_t_g_2d.cpp:2337-2388). */
 void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx);
 

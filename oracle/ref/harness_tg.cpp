@@ -1,7 +1,7 @@
 // Golden-vector harness for the 2D LOD solver of fr_d_t_g_2d.cpp.
 //
 // Drives the reference's own TG_solver_<func,fixed_d=1,kform=0> (CPU path,
-// dt_alg = 0 full-history summation) on injected state:
+// dt_alg = 0 full-history summation, or 3) on injected state:
 //   * V2 (the per-node squared-speed field) is overwritten after construction,
 //   * the initial field b_C is overwritten and pushed as oldC[0], exactly as
 //     TG_solver::initial() does (fr_d_t_g_2d.cpp:2294-2297) with noinit=1,
@@ -10,7 +10,10 @@
 // variants for s = 0..n at the final step n (fr_d_t_g_2d.cpp:3059-3239).
 //
 // Input layout (float64): func, sN, M, alpha, tau, sigma, d_value, C1, S,
-//   nsteps, every, int_alg, func_power, V2[(sN+2)(M+2)], C0[(sN+2)(M+2)]
+//   nsteps, every, int_alg, func_power, dt_alg, dt_eps, V2[(sN+2)(M+2)],
+//   C0[(sN+2)(M+2)].  dt_alg 3 is the binomial-series history (Htmp1/Htmp2,
+//   fr_d_t_g_2d.cpp:3385-3428) with dt_eps terms; update() then takes
+//   int_alg 6 unless 2 is given (2421-2424).
 #include "harness_io.h"
 #define main ref_main
 #include "fr_d_t_g_2d.cpp"
@@ -24,10 +27,13 @@ static void run(HarnessIn& in, HarnessOut& out, int sN_, int M_) {
     int nsteps = in.i(), every = in.i();
     int ialg = in.i();
     double fpow = in.d();
+    int dalg = in.i();
+    double deps = in.d();
     sN = sN_;
     _M = M_;
     int_alg = ialg;
-    dt_alg = 0;
+    dt_alg = dalg;
+    dt_eps = deps;
     int_err = 1e-6;
     _sigma = sigma;
     TG_solver_<func, 1, 0>* s = new TG_solver_<func, 1, 0>(

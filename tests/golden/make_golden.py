@@ -41,19 +41,24 @@ def save(name: str, meta: dict, **arrays):
 
 
 # ---------------------------------------------------------------- 2D LOD
-def tg_case(name, seed, func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, fpow=1.0):
+def tg_case(name, seed, func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, fpow=1.0,
+            dt_alg=0, dt_eps=1e-8):
     rng = np.random.default_rng(seed)
     nn = (sN + 2) * (M + 2)
     V2 = rng.uniform(0.5, 1.5, nn)
     C0 = rng.uniform(0.0, 1.0, nn)
-    hdr = [func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, 1, fpow]
+    int_alg = 6 if dt_alg == 3 else 1
+    hdr = [func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every, int_alg, fpow, dt_alg, dt_eps]
     out = run("ref_tg", np.concatenate([np.array(hdr, float), V2, C0]))
     nsnap = len([k for k in range(1, nsteps + 1) if k % every == 0 or k == nsteps])
     snaps = out[: nsnap * nn].reshape(nsnap, M + 2, sN + 2)
     kb = out[nsnap * nn:].reshape(8, nsteps + 1)
-    meta = dict(source="fr_d_t_g_2d.cpp TG_solver_<func,1,0>::calc_step, dt_alg=0, int_alg=1",
+    meta = dict(source=f"fr_d_t_g_2d.cpp TG_solver_<func,1,0>::calc_step, dt_alg={dt_alg}, "
+                       f"int_alg={int_alg}",
                 func=func, sN=sN, M=M, alpha=alpha, tau=tau, sigma=sigma, dv=dv, C1=C1, S=S,
                 nsteps=nsteps, every=every, fpow=fpow, int_err=1e-6, seed=seed)
+    if dt_alg:
+        meta.update(dt_alg=dt_alg, dt_eps=dt_eps)
     save(name, meta, V2=V2.reshape(M + 2, sN + 2), C0=C0.reshape(M + 2, sN + 2), snaps=snaps,
          kb=kb)
 
@@ -167,6 +172,9 @@ def main():
     tg_case("tg_a", 11, 0, 16, 12, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 40, 10)
     tg_case("tg_b", 12, 0, 9, 20, 0.4, 1e-3, 1.0, 0.2, 0.0, 0.0, 30, 15)
     tg_case("tg_psi", 13, 3, 10, 8, 0.6, 1e-2, 0.5, 0.1, 0.5, 0.0, 12, 6, fpow=0.7)
+    # binomial-series history (dt_alg 3), 25 and 10 terms
+    tg_case("tg_dt3_a", 14, 0, 12, 10, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 30, 10, dt_alg=3, dt_eps=25)
+    tg_case("tg_dt3_b", 15, 0, 9, 14, 0.5, 1e-3, 1.0, 0.2, 0.0, 0.0, 40, 20, dt_alg=3, dt_eps=10)
     mt_traj("mt_flux", "ref_mt", 21, 10, 4, 0.8, 0.01, 1.0, 1e-3, 0.0, -20.0, 1, 60, 20,
             lambda x, r: -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * r.normal(size=x.size))
     mt_traj("mt_dir_cv2", "ref_mt", 22, 10, 3, 0.6, 0.02, 1.0, 2e-3, -0.5, 0.0, 0, 50, 25,

@@ -34,6 +34,32 @@ def test_tg_matches_reference(golden, name):
     np.testing.assert_allclose(kb, z["kb"], rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("name", ["tg_dt3_a", "tg_dt3_b"])
+def test_tg_binomial_series_matches_reference(golden, name):
+    # dt_alg 3 (Htmp1/Htmp2 moments, int_alg 6) with 25 and 10 terms; the
+    # full-history trajectory differs from these by 1e-3..2e-2, and taking
+    # every coefficient list at full length (ignoring where _kb_row2 stops)
+    # by ~5e-8 in tg_dt3_a
+    m, z = golden(name)
+    dL, dLx = O.tg_ref_spacing(m["sN"], m["M"])
+    coefs = O.tg_coefs_ref(m["sN"], m["M"], z["V2"], m["dv"], dL, dLx)
+    s = O.TGSolver(m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], m["C1"], m["S"], *coefs,
+                   z["C0"], func=m["func"], fpow=m["fpow"], int_err=m["int_err"], dt3=m["dt_eps"])
+    t = 0
+    for q, k in enumerate(_snap_steps(m)):
+        s.step(k - t)
+        t = k
+        ref = z["snaps"][q]
+        assert np.max(np.abs(s.state() - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # kb() of dt_alg 3 forms only the two newest full steps' integrals; the
+    # step uses those at s = n (int_alg 6 takes _kb_row there, as int_alg 1;
+    # at s = n - 1 it may take the truncated _kb_row2, which no step reads)
+    n = m["nsteps"]
+    kb = O.tg_kb_table(m["func"], m["fpow"], m["alpha"], m["tau"], n, m["int_err"])
+    np.testing.assert_allclose(kb[:, n], z["kb"][:, n], rtol=1e-14, atol=0)
+    assert not np.any(z["kb"][:, :n - 1])
+
+
 @pytest.mark.parametrize("name,psi", [("mt_flux", 0), ("mt_dir_cv2", 0), ("psi_flux", 1)])
 def test_mt_matches_reference(golden, name, psi):
     m, z = golden(name)

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfrm.so")
 
 FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
-PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE = 0, 1, 2, 3
+PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE, PREC_BINOMIAL = 0, 1, 2, 3, 4
 KERNEL_CAPUTO, KERNEL_AB = 0, 1
 K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER, K_BLOCKFAR = 0, 1, 2, 3, 4, 5
 
@@ -43,7 +43,8 @@ class Lod2dDesc(ct.Structure):
                 ("C1", _D), ("S", _D), ("rowA", _P), ("rowB", _P), ("colA", _P), ("colB", _P),
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
                 ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
-                ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32), ("sweep_w", ct.c_int32)]
+                ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32), ("sweep_w", ct.c_int32),
+                ("series_terms", ct.c_int32), ("series_tol", _D)]
 
 
 class MT1DProblem(ct.Structure):

@@ -66,6 +66,14 @@ struct frm_lod2d {
     double* soe_q = nullptr;   // [L]
     double* soe_c = nullptr;   // [2][L]
     double soe_w1r = 0.0, soe_w1c = 0.0;  // exact lag-1 weights
+    // FRM_PREC_BINOMIAL: moments [2][terms][ld] and per-step coefficient tables
+    int bin_terms = 0;
+    double bin_Da = 0.0;
+    double* bin_H = nullptr;
+    double* bin_upd = nullptr;  // [capacity-1][terms]
+    double* bin_er = nullptr;
+    double* bin_ec = nullptr;
+    int* bin_cnt = nullptr;     // [capacity-1][4]
     // blocked MFMA form: FP32 increments in hist, BF16-exact weight parts,
     // far part of the current 64-step block
     void* atab = nullptr;      // weight-operand fragments (atab_build)
@@ -371,6 +379,56 @@ __global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const d
     part[ld + i] = s1;
 }
 
+// ------------------------------------------------------------- binomial-series memory term
+// TG_solver's dt_alg = 3 (fr_d_t_g_2d.cpp:3385-3428): per node and sweep
+// direction, moments H_k = sum over full steps s of g(t_{s+1})^(k+1) v3_k(s)
+// Delta_s (Htmp1 for rows, Htmp2 for columns), each step taking the newest
+// increment Delta_{n-1} = C_n - C_{n-1} from the two-slot ring into the
+// first cnt moments, and the memory term Da sum_k f_k H_k with the binomial
+// factors f_k of the direction's target (_kb_row2_fixed_coefs).  Products
+// and sums are rounded separately, as the reference's x86-64 build does.
+__global__ __launch_bounds__(256) void k_binomial_memterm(
+    double* __restrict__ H, const double* __restrict__ ring, int64_t ld, int n, int terms,
+    const double* __restrict__ upd, const double* __restrict__ er, const double* __restrict__ ec,
+    const int* __restrict__ cnt, double Da, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ld) return;
+    if (n == 0) {  // no full step yet: the row term is 0, the column one only its half step
+        part[i] = 0.0;
+        part[ld + i] = 0.0;
+        return;
+    }
+    const int64_t r = (int64_t)(n - 1) * terms;
+    const int* c = cnt + 4 * (int64_t)(n - 1);
+    const double x = ring[(int64_t)((n - 1) % kSoeLag0) * ld + i];
+    constexpr int B = 8;  // moments loaded together
+#pragma unroll
+    for (int dir = 0; dir < 2; ++dir) {
+        double* __restrict__ Hd = H + (int64_t)dir * terms * ld + i;
+        const double* __restrict__ f = (dir == 0 ? er : ec) + r;
+        const int cu = c[dir], ce = c[2 + dir];
+        double acc = 0.0;
+        for (int k0 = 0; k0 < terms; k0 += B) {
+            double h[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (k0 + u < terms) h[u] = Hd[(int64_t)(k0 + u) * ld];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                const int k = k0 + u;
+                if (k >= terms) break;
+                if (k < cu) {
+                    h[u] = h[u] + upd[r + k] * x;
+                    Hd[(int64_t)k * ld] = h[u];
+                }
+                if (k < ce) acc += f[k] * h[u];
+            }
+        }
+        part[(int64_t)dir * ld + i] = Da * acc;
+    }
+}
+
 // ------------------------------------------------------------- host helpers
 // Thomas alpha coefficients of one line in the reference's order
 // (al1: fr_d_t_g_2d.cpp:3267-3304, R: 3257-3265); f = Al[i+1]/B(i), or the
@@ -488,8 +546,9 @@ static ColArgs col_args(frm_lod2d* s, int nch, int n) {
     ColArgs a;
     a.C = s->C;
     a.Cprev = s->Cprev;
+    const bool ring = soe || s->precision == FRM_PREC_BINOMIAL;
     a.slot = (double*)s->hist->data +
-             (int64_t)(soe ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
+             (int64_t)(ring ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
     a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
     a.coef = s->ccoef;
     a.part = s->hist->part;
@@ -584,6 +643,11 @@ static void lod_free(frm_lod2d* s) {
     hipFree(s->soe_H);
     hipFree(s->soe_q);
     hipFree(s->soe_c);
+    hipFree(s->bin_H);
+    hipFree(s->bin_upd);
+    hipFree(s->bin_er);
+    hipFree(s->bin_ec);
+    hipFree(s->bin_cnt);
     hipFree(s->atab);
     hipFree(s->F);
     hipFree(s->G);
@@ -608,8 +672,13 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: coefficient arrays missing");
     FRM_REQUIRE(ctx,
                 d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE ||
-                    d->precision == FRM_PREC_MFMA_BF16,
+                    d->precision == FRM_PREC_MFMA_BF16 || d->precision == FRM_PREC_BINOMIAL,
                 "frm_lod2d_create: precision %d not built", d->precision);
+    FRM_REQUIRE(ctx,
+                d->precision != FRM_PREC_BINOMIAL ||
+                    (d->kernel == FRM_KERNEL_CAPUTO && d->series_terms >= 0 &&
+                     d->series_terms <= 1000 && d->series_tol >= 0),
+                "frm_lod2d_create: FRM_PREC_BINOMIAL takes the Caputo kernel and 0..1000 terms");
     FRM_REQUIRE(ctx, d->soe_terms >= 0 && d->soe_terms <= 64, "frm_lod2d_create: soe_terms %d",
                 d->soe_terms);
     FRM_REQUIRE(ctx, d->mfma_parts == 0 || d->mfma_parts == 2 || d->mfma_parts == 3,
@@ -666,10 +735,11 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     const int64_t nodes = (int64_t)s->NX * s->NY;
     const size_t nn = (size_t)s->ld * (d->M + 2);
     s->precision = d->precision;
-    // the SOE form keeps only a two-slot ring of increments, the blocked
-    // MFMA form FP32 increments
+    // the SOE and binomial forms keep only a two-slot ring of increments,
+    // the blocked MFMA form FP32 increments
     const bool mf = d->precision == FRM_PREC_MFMA_BF16;
-    int rc = hist_alloc(ctx, nodes, d->precision == FRM_PREC_SOE ? kSoeLag0 : d->capacity,
+    const bool ring = d->precision == FRM_PREC_SOE || d->precision == FRM_PREC_BINOMIAL;
+    int rc = hist_alloc(ctx, nodes, ring ? kSoeLag0 : d->capacity,
                         mf ? FRM_PREC_MFMA_BF16 : FRM_PREC_F64, &s->hist);
     if (rc) {
         lod_free(s);
@@ -741,6 +811,30 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         if (e != hipSuccess) {
             lod_free(s);
             return check_hip(ctx, e, "hipMalloc(lod2d SOE)");
+        }
+    }
+    if (d->precision == FRM_PREC_BINOMIAL) {
+        s->bin_terms = d->series_terms > 0 ? d->series_terms : 25;
+        s->bin_Da = 1.0 / gamma_cody(1.0 - d->alpha);
+        const frm::BinomialTables bt = frm::binomial_tables(
+            d->alpha, d->tau, d->series_tol > 0 ? d->series_tol : 1e-5, s->bin_terms, d->capacity);
+        const size_t hsz = sizeof(double) * 2 * (size_t)s->bin_terms * s->hist->ld;
+        const size_t tsz = std::max<size_t>(1, bt.upd.size());
+        e = hipMalloc(&s->bin_H, hsz);
+        if (e == hipSuccess) e = hipMalloc(&s->bin_upd, sizeof(double) * tsz);
+        if (e == hipSuccess) e = hipMalloc(&s->bin_er, sizeof(double) * tsz);
+        if (e == hipSuccess) e = hipMalloc(&s->bin_ec, sizeof(double) * tsz);
+        if (e == hipSuccess) e = hipMalloc(&s->bin_cnt, sizeof(int) * std::max<size_t>(1, bt.cnt.size()));
+        if (e == hipSuccess && !bt.upd.empty()) {
+            e = hipMemcpy(s->bin_upd, bt.upd.data(), sizeof(double) * bt.upd.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(s->bin_er, bt.er.data(), sizeof(double) * bt.er.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(s->bin_ec, bt.ec.data(), sizeof(double) * bt.ec.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(s->bin_cnt, bt.cnt.data(), sizeof(int) * bt.cnt.size(), hipMemcpyHostToDevice);
+        }
+        if (e == hipSuccess) e = hipMemset(s->bin_H, 0, hsz);
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMalloc(lod2d binomial)");
         }
     }
     if (mf) {
@@ -819,6 +913,11 @@ int frm_lod2d_set_state(frm_lod2d* s, const double* h_C) {
         FRM_HIP(ctx, hipMemsetAsync(s->soe_H, 0, hsz, ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    if (s->bin_H) {
+        const size_t hsz = sizeof(double) * 2 * (size_t)s->bin_terms * s->hist->ld;
+        FRM_HIP(ctx, hipMemsetAsync(s->bin_H, 0, hsz, ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return FRM_OK;
 }
 
@@ -828,6 +927,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     FRM_REQUIRE(ctx, nsteps >= 0, "frm_lod2d_step: nsteps %d", nsteps);
     const bool soe = s->precision == FRM_PREC_SOE;
     const bool mf = s->precision == FRM_PREC_MFMA_BF16;
+    const bool bin = s->precision == FRM_PREC_BINOMIAL;
     if (s->tstep + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
                          s->tstep + nsteps, s->capacity);
@@ -848,6 +948,15 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                 hipLaunchKernelGGL(k_soe_memterm<double>, grid, dim3(256), 0, ctx->stream,
                                    (double*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
                                    s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
+            FRM_HIP(ctx, hipGetLastError());
+        } else if (bin) {
+            nch = 1;
+            const int64_t ld = s->hist->ld;
+            LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * (2.0 * s->bin_terms * 16 + 8 + 2 * 8));
+            hipLaunchKernelGGL(k_binomial_memterm, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0,
+                               ctx->stream, s->bin_H, (const double*)s->hist->data, ld, n,
+                               s->bin_terms, s->bin_upd, s->bin_er, s->bin_ec, s->bin_cnt, s->bin_Da,
+                               s->hist->part);
             FRM_HIP(ctx, hipGetLastError());
         } else if (mf) {
             const int64_t ld = s->hist->ld;
@@ -884,7 +993,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_COLS, 0.0);
             FRM_HIP(ctx, launch_cols(s->csh, ctx->stream, col_args(s, nch, n)));
         }
-        if (!soe) s->hist->depth += 1;
+        if (!soe && !bin) s->hist->depth += 1;
         s->tstep = n + 1;
     }
     return FRM_OK;

@@ -320,6 +320,87 @@ void ab_series_tables(double a, double tau, int maxl, int maxkiter, double eps, 
     }
 }
 
+// Coefficient k of _kb_row2 (fr_d_t_g_2d.cpp:2970-3038) for g(t) = t on
+// [t0, t1]: the integral of x^k over [t0, t1] over t1^(k+1); k = 0 as
+// (t1 - t0)/t1, else (1 - (t0/t1)^(k+1))/(k+1) with v6 = (t0/t1)^(k+1) the
+// caller's running power.  The reference then runs its series in powers of
+// (x - t1) of the inverse's derivatives, which for g = t adds a zero term
+// (the second derivative of t is 0) and stops.
+static double binomial_v3(int k, double t0, double t1, double v6) {
+    if (k == 0) return (t1 - t0) / t1;
+    return 1.0 * ((1.0 / (k + 1.0)) * (1 - v6));
+}
+
+BinomialTables binomial_tables(double alpha, double tau, double int_err, int terms, int cap) {
+    BinomialTables bt;
+    bt.terms = terms;
+    const int rows = std::max(0, cap - 1);
+    bt.upd.assign((size_t)rows * terms, 0.0);
+    bt.er.assign((size_t)rows * terms, 0.0);
+    bt.ec.assign((size_t)rows * terms, 0.0);
+    bt.cnt.assign((size_t)rows * 4, 0);
+    std::vector<double> v3(terms);
+    for (int n = 1; n < cap; ++n) {
+        const int i = n - 1;  // the newest full step, interval [tau i, tau(i+1)]
+        const double t0 = tau * (i + 0.0), t1 = tau * (i + 0.0 + 1.0);
+        double v6 = t0 / t1;
+        for (int k = 0; k < terms; ++k) {
+            if (k != 0) v6 *= t0 / t1;
+            v3[k] = binomial_v3(k, t0, t1, v6);
+        }
+        // terms one _kb_row2 call at target gtj takes: until `terms` of them
+        // or until |v1 v2 v3| <= int_err (its `continue` goes to that test);
+        // row2_precalc keeps the largest count of the calls so far
+        auto taken = [&](double gtj) {
+            double v1 = 1, v2 = std::pow(gtj, -alpha) * t1;
+            int k = 0;
+            do {
+                if (k != 0) v1 *= -(-alpha - k + 1) / k;
+                if (k != 0) v2 *= t1 / gtj;
+                const double v = v3[k];
+                k++;
+                if (k > 10000) break;  // integr_max_niter
+                if (k < terms) {
+                    if (std::fabs(v1 * v2 * v) > int_err) continue;
+                    break;
+                }
+                break;
+            } while (true);
+            return k;
+        };
+        // position 1 (ja 1/2): called once, at step i + 1; position 5 (ja 1):
+        // at step i and again at step i + 1 (kb()'s block, 3112-3140)
+        const int c1 = taken(tau * (n + 0.5));
+        const int c5 = std::max(taken(tau * (i + 1.0)), taken(tau * (n + 1.0)));
+        double* u = bt.upd.data() + (size_t)i * terms;
+        const double g1 = n * tau;
+        double mult = g1;
+        for (int k = 0; k < std::max(c1, c5); ++k) {
+            u[k] = mult * v3[k];
+            mult *= g1;
+        }
+        int* cn = bt.cnt.data() + (size_t)i * 4;
+        cn[0] = c1;
+        cn[1] = c5;
+        // evaluation factors of _kb_row2_fixed_coefs (3041-3057)
+        for (int dir = 0; dir < 2; ++dir) {
+            const double gtj = dir == 0 ? (n + 0.5) * tau : (n + 1.0) * tau;
+            double* e = (dir == 0 ? bt.er.data() : bt.ec.data()) + (size_t)i * terms;
+            double f = std::pow(gtj, -alpha), fi = 0;
+            int used = 0;
+            do {
+                e[used] = f;
+                fi += 1.0;
+                used++;
+                f *= -(-alpha - fi + 1.0) / (fi * gtj);
+                if (!std::isfinite(f)) break;
+            } while (used != terms);
+            cn[2 + dir] = used;
+        }
+    }
+    return bt;
+}
+
 }  // namespace frm
 
 namespace frm {

@@ -47,6 +47,21 @@ constexpr int kAbSeriesRows = 5;
 void ab_series_tables(double a, double tau, int maxl, int maxkiter, double eps, int cap,
                       double* tab, int* choice);
 
+// Per-step coefficients of the binomial-series history of TG_solver with
+// dt_alg = 3 (int_alg 6), g(t) = t, `terms` moments per node and direction.
+// For step n = 1..cap-1, row (n-1): upd[k] = (n tau)^(k+1) v3_k (the moment
+// update of the newest full-step increment), er[k] / ec[k] = the evaluation
+// factors of _kb_row2_fixed_coefs at the row / column targets (n+1/2) tau and
+// (n+1) tau, and cnt[4(n-1)..] = {moments updated in the row and column
+// direction (how many coefficients row2_precalc holds), terms evaluated in
+// each (up to a non-finite factor)}.
+struct BinomialTables {
+    int terms = 0;
+    std::vector<double> upd, er, ec;  // [cap-1][terms]
+    std::vector<int> cnt;             // [cap-1][4]
+};
+BinomialTables binomial_tables(double alpha, double tau, double int_err, int terms, int cap);
+
 // E_{a,b}(z) by the method of mittag_leffler (MF_int_series.cpp:247-302):
 // the power series for z > 0; for z <= 0 the integral representation when
 // |z| > pi a (0 < a < 1, b < 1 + a), else the series summed in pairs, the
