@@ -173,6 +173,10 @@ typedef struct frm_lod2d_desc {
                              (the reference's dt_eps; 0: 25, max 1000) */
     double series_tol;    /* FRM_PREC_BINOMIAL: int_err, where _kb_row2 stops
                              taking coefficients (0: 1e-5, the reference's default) */
+    double trunc_eps;     /* FRM_PREC_F64, Caputo: the reference's restricted
+                             summation (dt_alg = 2, fr_d_t_g_2d.cpp:3356-3381): each
+                             direction's history sum, newest first, stops at the
+                             first weight below trunc_eps; 0: the full history */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);

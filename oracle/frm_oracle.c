@@ -153,6 +153,7 @@ struct orc_tg {
     /* dt_alg 3: dt3 series terms; H1/H2 = Htmp1/Htmp2 [nn][dt3]; per full
      * step i the coefficients v3 of _kb_row2 for [tau i, tau(i+1)] and how
      * many of them row2_precalc holds at positions 1 and 5 of step i's block */
+    double dt2; /* dt_alg 2: the history sum stops at the first |kb| < dt2 (0: off) */
     int dt3;
     double *H1, *H2, *r2;
     int *r2n;
@@ -308,6 +309,8 @@ static double tg_fixed_coefs(double gtj, const double* c, int n, double alpha) {
     return sum;
 }
 
+void orc_tg_set_dt2(orc_tg* t, double eps) { t->dt2 = eps; }
+
 void orc_tg_set_dt3(orc_tg* t, int terms) {
     size_t nn = (size_t)t->ld * (t->M + 2);
     t->dt3 = terms;
@@ -374,6 +377,7 @@ static void tg_line(orc_tg* t, int dir, int line, int n) {
                 for (int q = ((t->nold - 3) / 2) * 2; q >= 0; q -= 2) {
                     /* full-step increments, newest first: kb(s,1/2,0,1) or kb(s,1,0,1) */
                     double kv = dir == 0 ? TG_KB(t, 4, q / 2) : TG_KB(t, 5, q / 2);
+                    if (t->dt2 > 0 && fabs(kv) < t->dt2) break; /* dt_alg 2 (3364-3381) */
                     ts += kv * (t->old[q + 2][id] - t->old[q][id]);
                 }
             if (t->nold >= 3 && t->dt3) {

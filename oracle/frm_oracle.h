@@ -59,6 +59,8 @@ void orc_tg_set_ab(orc_tg* t, int maxl, int maxkiter, double eps);
 /* dt_alg 3: the binomial-series history with `terms` moments per node and
  * direction (fr_d_t_g_2d.cpp:3385-3428; int_alg 6), before the first step */
 void orc_tg_set_dt3(orc_tg* t, int terms);
+/* dt_alg 2: the newest-first history sum stops at the first weight below eps */
+void orc_tg_set_dt2(orc_tg* t, double eps);
 /* Grid spacings the reference constructor derives (fr_d_t_g_2d.cpp:2337-2388). */
 void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx);
 

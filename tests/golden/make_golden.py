@@ -172,6 +172,10 @@ def main():
     tg_case("tg_a", 11, 0, 16, 12, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 40, 10)
     tg_case("tg_b", 12, 0, 9, 20, 0.4, 1e-3, 1.0, 0.2, 0.0, 0.0, 30, 15)
     tg_case("tg_psi", 13, 3, 10, 8, 0.6, 1e-2, 0.5, 0.1, 0.5, 0.0, 12, 6, fpow=0.7)
+    # restricted summation (dt_alg 2): the newest-first history sum stops at
+    # the first weight below dt_eps
+    tg_case("tg_dt2_a", 16, 0, 12, 10, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 40, 10, dt_alg=2, dt_eps=1e-2)
+    tg_case("tg_dt2_b", 17, 0, 9, 14, 0.5, 1e-3, 1.0, 0.2, 0.0, 0.0, 50, 25, dt_alg=2, dt_eps=4e-3)
     # binomial-series history (dt_alg 3), 25 and 10 terms
     tg_case("tg_dt3_a", 14, 0, 12, 10, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 30, 10, dt_alg=3, dt_eps=25)
     tg_case("tg_dt3_b", 15, 0, 9, 14, 0.5, 1e-3, 1.0, 0.2, 0.0, 0.0, 40, 20, dt_alg=3, dt_eps=10)

@@ -150,3 +150,37 @@ def test_lod2d_ab_kernel_matches_oracle(ctx, alpha, steps):
     g.calc_step(steps)
     assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
     g.close()
+
+
+# Restricted summation (the reference's dt_alg 2): the reference's own
+# trajectories and the oracle at larger sizes; the kernel reads only the
+# newest slots inside the cut-off.
+@pytest.mark.parametrize("name", ["tg_dt2_a", "tg_dt2_b"])
+def test_lod2d_restricted_sum_matches_reference_golden(ctx, golden, name):
+    m, z = golden(name)
+    dL, dLx = S.tg_ref_spacing(m["sN"], m["M"])
+    coefs = S.coefs_ref(z["V2"], m["dv"], dL, dLx)
+    s = S.TGSolver(ctx, m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], coefs, z["C0"],
+                   capacity=m["nsteps"], C1=m["C1"], S=m["S"], trunc_eps=m["dt_eps"])
+    ks = [k for k in range(1, m["nsteps"] + 1) if k % m["every"] == 0 or k == m["nsteps"]]
+    t = 0
+    for q, k in enumerate(ks):
+        s.calc_step(k - t)
+        t = k
+        assert relerr(s.b_C, z["snaps"][q]) <= TOL, (name, k)
+    s.close()
+
+
+@pytest.mark.parametrize("sN,M,alpha,steps,eps", [(100, 70, 0.6, 300, 3e-3), (65, 129, 0.9, 200, 3e-3)])
+def test_lod2d_restricted_sum_matches_oracle(ctx, sN, M, alpha, steps, eps):
+    rng = np.random.default_rng(sN + M)
+    coefs, C0 = _adv_case(rng, sN, M)
+    tau, sigma, C1, Ssrc = 1e-3, 1.0, 0.0, 0.05
+    o = O.TGSolver(sN, M, alpha, tau, sigma, C1, Ssrc, *coefs, C0, dt2=eps)
+    g = S.TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=steps, C1=C1, S=Ssrc,
+                   trunc_eps=eps)
+    for chunk in (1, steps // 2 - 1, steps - steps // 2):
+        o.step(chunk)
+        g.calc_step(chunk)
+        assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    g.close()

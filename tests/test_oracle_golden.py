@@ -34,6 +34,30 @@ def test_tg_matches_reference(golden, name):
     np.testing.assert_allclose(kb, z["kb"], rtol=1e-14, atol=0)
 
 
+@pytest.mark.parametrize("name", ["tg_dt2_a", "tg_dt2_b"])
+def test_tg_restricted_sum_matches_reference(golden, name):
+    # dt_alg 2: the newest-first history sum stops at the first |kb| < dt_eps
+    # (about 20 of 40-50 lags kept; the full history differs by 8-15%)
+    m, z = golden(name)
+    dL, dLx = O.tg_ref_spacing(m["sN"], m["M"])
+    coefs = O.tg_coefs_ref(m["sN"], m["M"], z["V2"], m["dv"], dL, dLx)
+    s = O.TGSolver(m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], m["C1"], m["S"], *coefs,
+                   z["C0"], int_err=m["int_err"], dt2=m["dt_eps"])
+    t = 0
+    for q, k in enumerate(_snap_steps(m)):
+        s.step(k - t)
+        t = k
+        ref = z["snaps"][q]
+        assert np.max(np.abs(s.state() - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # the reference forms kb() newest first and stops where all eight
+    # variants fall below dt_eps: the leading values agree, the rest are 0
+    n = m["nsteps"]
+    kb = O.tg_kb_table(m["func"], m["fpow"], m["alpha"], m["tau"], n, m["int_err"])
+    kept = z["kb"][4] != 0
+    np.testing.assert_allclose(kb[:, kept], z["kb"][:, kept], rtol=1e-14, atol=0)
+    assert 0 < kept.sum() < n and np.all(kb[4, ~kept] < m["dt_eps"])
+
+
 @pytest.mark.parametrize("name", ["tg_dt3_a", "tg_dt3_b"])
 def test_tg_binomial_series_matches_reference(golden, name):
     # dt_alg 3 (Htmp1/Htmp2 moments, int_alg 6) with 25 and 10 terms; the

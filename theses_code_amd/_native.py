@@ -44,7 +44,7 @@ class Lod2dDesc(ct.Structure):
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
                 ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
                 ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32), ("sweep_w", ct.c_int32),
-                ("series_terms", ct.c_int32), ("series_tol", _D)]
+                ("series_terms", ct.c_int32), ("series_tol", _D), ("trunc_eps", _D)]
 
 
 class MT1DProblem(ct.Structure):

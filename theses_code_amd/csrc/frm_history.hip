@@ -35,12 +35,12 @@ typedef double frm_dv2 __attribute__((ext_vector_type(2)));
 
 template <int NT>
 __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f64(
-    const double* __restrict__ hist, int64_t ld, int depth, const double* __restrict__ w, int ldw,
-    int chunk, double* __restrict__ part) {
+    const double* __restrict__ hist, int64_t ld, int depth, int first, const double* __restrict__ w,
+    int ldw, int chunk, double* __restrict__ part) {
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
     if (i >= ld) return;
     const int c = blockIdx.y;
-    const int s0 = c * chunk;
+    const int s0 = first + c * chunk;
     const int s1 = min(depth, s0 + chunk);
     double a0[NT], a1[NT];
 #pragma unroll
@@ -157,23 +157,25 @@ static int pick_chunks(const frm_hist* h, int depth) {
 }
 
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
-                     int* nchunks_out) {
+                     int* nchunks_out, int first) {
     const int depth = h->depth;
-    const int nchunks = pick_chunks(h, depth);
+    first = std::max(0, std::min(first, depth));
+    const int span = depth - first;  // slots first .. depth-1, lags depth-first .. 1
+    const int nchunks = pick_chunks(h, span);
     *nchunks_out = nchunks;
     if (nchunks == 0) return FRM_OK;
-    const int chunk = (depth + nchunks - 1) / nchunks;
+    const int chunk = (span + nchunks - 1) / nchunks;
     const int threads = kMemtermThreads;
     dim3 grid((unsigned)(h->ld / 2 / threads), (unsigned)nchunks);
     LaunchTimer timer(ctx, FRM_K_MEMTERM,
-                      (double)depth * h->n_nodes * sizeof(double) +
+                      (double)span * h->n_nodes * sizeof(double) +
                           (double)ntargets * h->n_nodes * sizeof(double));
     if (ntargets == 1)
         hipLaunchKernelGGL(k_memterm_f64<1>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, d_w, ldw, chunk, h->part);
+                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, h->part);
     else
         hipLaunchKernelGGL(k_memterm_f64<2>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, d_w, ldw, chunk, h->part);
+                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, h->part);
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
 }

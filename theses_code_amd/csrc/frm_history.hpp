@@ -24,9 +24,10 @@ constexpr int32_t kMaxChunks = 64;
 
 int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
                frm_hist** out);
-// Launches the split-K partial sums for the current depth into h->part;
-// returns the number of chunks written (0 when the store is empty).
+// Launches the split-K partial sums for the current depth into h->part,
+// over slots first .. depth-1 (the newest depth - first lags); returns the
+// number of chunks written (0 when there is nothing to sum).
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
-                     int* nchunks);
+                     int* nchunks, int first = 0);
 
 }  // namespace frm

@@ -74,6 +74,9 @@ struct frm_lod2d {
     double* bin_er = nullptr;
     double* bin_ec = nullptr;
     int* bin_cnt = nullptr;     // [capacity-1][4]
+    // restricted summation: lags beyond trunc_L are not read (weights past a
+    // direction's own cut-off are zero)
+    int trunc_L = 0;
     // blocked MFMA form: FP32 increments in hist, BF16-exact weight parts,
     // far part of the current 64-step block
     void* atab = nullptr;      // weight-operand fragments (atab_build)
@@ -675,6 +678,10 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                     d->precision == FRM_PREC_MFMA_BF16 || d->precision == FRM_PREC_BINOMIAL,
                 "frm_lod2d_create: precision %d not built", d->precision);
     FRM_REQUIRE(ctx,
+                d->trunc_eps == 0 ||
+                    (d->trunc_eps > 0 && d->precision == FRM_PREC_F64 && d->kernel == FRM_KERNEL_CAPUTO),
+                "frm_lod2d_create: trunc_eps takes FRM_PREC_F64 and the Caputo kernel");
+    FRM_REQUIRE(ctx,
                 d->precision != FRM_PREC_BINOMIAL ||
                     (d->kernel == FRM_KERNEL_CAPUTO && d->series_terms >= 0 &&
                      d->series_terms <= 1000 && d->series_tol >= 0),
@@ -773,6 +780,22 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
             w[k - 1] = caputo_kb(d->alpha, d->tau, k + 0.5, 1.0);
             w[d->capacity + k - 1] = caputo_kb(d->alpha, d->tau, k + 1.0, 1.0);
         }
+    }
+    // restricted summation (dt_alg = 2): the sums run newest first and stop
+    // at the first |kb| < trunc_eps; the Caputo weights fall with the lag,
+    // so each direction keeps its leading lags and the kernel reads the
+    // newest trunc_L = max of the two
+    s->trunc_L = d->capacity;
+    if (d->trunc_eps > 0) {
+        int keep[2] = {0, 0};
+        for (int t = 0; t < 2; ++t) {
+            double* wt = w.data() + (size_t)t * d->capacity;
+            int k = 0;
+            while (k < d->capacity && !(std::fabs(wt[k]) < d->trunc_eps)) ++k;
+            keep[t] = k;
+            for (; k < d->capacity; ++k) wt[k] = 0.0;
+        }
+        s->trunc_L = std::max(keep[0], keep[1]);
     }
     // line coefficients
     std::vector<double> rc3((size_t)3 * (d->M - 1) * s->NX), cc3((size_t)3 * (d->sN - 1) * s->NY);
@@ -982,7 +1005,8 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                            N1 > N0, s->hist->part))
                 return check_hip(ctx, hipGetLastError(), "block_near");
         } else {
-            int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch);
+            int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch,
+                                      s->hist->depth - s->trunc_L);
             if (rc) return rc;
         }
         {
