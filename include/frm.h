@@ -285,6 +285,23 @@ int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
  * frm_mt1d_get_state then returns FRM_E_STATE. */
 int frm_mt1d_ab_error(frm_mt1d* s, double* out);
 
+/* Fitting error of every problem against one set of observations, as the
+ * reference's fitting drivers score a particle (solve_and_test,
+ * mt_minimal.cpp:1277-1330; the PSO loop calls it once per particle, here
+ * the particles are the problems of one batch): for each observation (T, Z,
+ * F) with tt <= T < tt + tau in the step that starts at tt (tt accumulated
+ * as tt += tau from the last set_state), U at depth Z is interpolated
+ * linearly between nodes j, j+1 with dL j < Z <= dL (j+1) before the step
+ * (j < N-1) and after it (j < N), blended by k1 = (T - tt)/tau, clamped to
+ * <= 0, and (F - v)^2 is added; a problem whose U[0] is not finite after a
+ * step scores 1e300 from then on.  Observations: host arrays of nobs
+ * values, copied; replacing them or set_state clears the errors. */
+int frm_mt1d_set_observations(frm_mt1d* s, int32_t nobs, const double* T, const double* Z,
+                              const double* F);
+/* Per-problem error (nprob values) over the steps taken since the errors
+ * were cleared. */
+int frm_mt1d_fit_error(frm_mt1d* s, double* out);
+
 #ifdef __cplusplus
 }
 #endif

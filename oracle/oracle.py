@@ -191,6 +191,46 @@ class MTSolver:
             self._h = None
 
 
+def mt_fit_error(solver, N, L, tau, nsteps, T, Z, F) -> float:
+    """The error loop of solve_and_test (mt_minimal.cpp:1288-1330) around an
+    oracle solver, step by step: observations with tt <= T < tt + tau are
+    interpolated in depth before (nodes j < N-1) and after (j < N) the step,
+    blended in time, clamped to <= 0 and scored by squares; 1e300 once U[0]
+    is not finite.  Plain Python floats: IEEE double, no fused operations.
+    (Where the reference's `vs` is never written -- a depth in the last
+    interval -- it reads uninitialised memory; here it reads 0.)"""
+    dL = L / N
+    vs = [0.0] * len(T)
+    err, tt = 0.0, 0.0
+    for _ in range(nsteps):
+        U = solver.state()
+        for i in range(len(T)):
+            if (T[i] >= tt) and (T[i] < (tt + tau)):
+                for j in range(N - 1):
+                    if ((dL * j) < Z[i]) and ((dL * (j + 1)) >= Z[i]):
+                        k2 = (Z[i] - (dL * j)) / dL
+                        vs[i] = (1 - k2) * U[j] + k2 * U[j + 1]
+                        break
+        solver.step(1)
+        U = solver.state()
+        for i in range(len(T)):
+            if (T[i] >= tt) and (T[i] < (tt + tau)):
+                k1 = (T[i] - tt) / tau
+                for j in range(N):
+                    if ((dL * j) < Z[i]) and ((dL * (j + 1)) >= Z[i]):
+                        k2 = (Z[i] - (dL * j)) / dL
+                        v = (1 - k2) * U[j] + k2 * U[j + 1]
+                        v = (1 - k1) * vs[i] + k1 * v
+                        if v > 0.0:
+                            v = 0.0
+                        err += (F[i] - v) * (F[i] - v)
+        if not np.isfinite(U[0]):
+            err = 1e300
+            break
+        tt += tau
+    return err if np.isfinite(err) else 1e300
+
+
 def mt_history(N, alpha, tau, snaps, tstep, psi=0, beta=1.0) -> np.ndarray:
     snaps = np.ascontiguousarray(snaps, dtype=np.float64)
     os_ = snaps.shape[0]

@@ -111,6 +111,8 @@ SIGNATURES = {
     "frm_mt1d_get_state": (_I, [_V, _P]),
     "frm_mt1d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_mt1d_ab_error": (_I, [_V, _P]),
+    "frm_mt1d_set_observations": (_I, [_V, _I, _P, _P, _P]),
+    "frm_mt1d_fit_error": (_I, [_V, _P]),
 }
 
 _lib = None

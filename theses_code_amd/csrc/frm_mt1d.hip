@@ -663,6 +663,79 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
     a.part[h] = ret / (1.0 - P.alpha);
 }
 
+// ---------------------------------------------------------------- fitting error
+// solve_and_test (mt_minimal.cpp:1277-1330) for every problem of the batch:
+// one thread per problem walks the observations of the current step in
+// order (so the sum is the reference's, term by term).
+struct FitArgs {
+    const MtProbDev* pr;
+    int nprob;
+    const double* U;
+    const double* T;   // observation times, depths, values
+    const double* Z;
+    const double* F;
+    const int* idx;    // the observations of this step, in order
+    int cnt;
+    double tt, tau;
+    double* vs;        // [nprob][nobs] values before the step
+    int nobs;
+    double* err;       // [nprob]
+    int* dead;         // [nprob]: U[0] was not finite after a step
+};
+
+// U at depth z between nodes j, j+1 with dL j < z <= dL (j+1), j < jmax;
+// false when no such j
+__device__ inline bool fit_interp(const double* U, double dL, int jmax, double z, double& v) {
+    const double r = z / dL;
+    if (!(r > 0.0) || r > jmax + 2.0) return false;  // no interval (also NaN)
+    int j = (int)floor(r) - 1;
+    if (j < 0) j = 0;
+    for (int q = 0; q < 3 && j < jmax; ++q, ++j)
+        if (((dL * j) < z) && ((dL * (j + 1)) >= z)) {
+            const double k2 = (z - (dL * j)) / dL;
+            v = (1 - k2) * U[j] + k2 * U[j + 1];
+            return true;
+        }
+    return false;
+}
+
+__global__ __launch_bounds__(256) void k_fit_before(FitArgs a) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nprob) return;
+    const MtProbDev& P = a.pr[p];
+    const double* U = a.U + P.uoff;
+    for (int q = 0; q < a.cnt; ++q) {
+        const int o = a.idx[q];
+        double v;
+        if (fit_interp(U, P.dL, P.N - 1, a.Z[o], v)) a.vs[(int64_t)p * a.nobs + o] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fit_after(FitArgs a) {
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.nprob || a.dead[p]) return;
+    const MtProbDev& P = a.pr[p];
+    const double* U = a.U + P.uoff;
+    double e = a.err[p];
+    for (int q = 0; q < a.cnt; ++q) {
+        const int o = a.idx[q];
+        const double k1 = (a.T[o] - a.tt) / a.tau;
+        double v;
+        if (fit_interp(U, P.dL, P.N, a.Z[o], v)) {
+            v = (1 - k1) * a.vs[(int64_t)p * a.nobs + o] + k1 * v;
+            if (v > 0.0) v = 0.0;
+            e += (a.F[o] - v) * (a.F[o] - v);
+        }
+    }
+    if (!isfinite(U[0])) {
+        e = 1e300;
+        a.dead[p] = 1;
+    }
+    a.err[p] = e;
+}
+
 }  // namespace frm
 
 using namespace frm;
@@ -700,6 +773,16 @@ struct frm_mt1d {
     double* ser_lc = nullptr;        // [2][maxL][ldh]: Lc1, Lc2
     signed char* ser_flag = nullptr; // [maxL][ldh]
     std::vector<double> ser_bytes;   // per step i >= 2: algorithmic bytes of k_ab_series
+    // fitting error (solve_and_test)
+    int nobs = 0;
+    std::vector<double> obs_T;       // host copy: which observations fall in which step
+    double tt = 0.0;                 // start of the next step, accumulated as the reference does
+    double* fit_obs = nullptr;       // [3][nobs] T, Z, F
+    int* fit_idx = nullptr;          // [nobs] scratch: the observations of one step
+    double* fit_vs = nullptr;        // [nprob][nobs]
+    double* fit_err = nullptr;       // [nprob]
+    int* fit_dead = nullptr;         // [nprob]
+    std::vector<int> fit_off;        // [capacity+1]: observations of step k at fit_idx[off[k-1] ..]
 };
 
 static void mt_free(frm_mt1d* s) {
@@ -718,6 +801,11 @@ static void mt_free(frm_mt1d* s) {
     hipFree(s->ab_ml);
     hipFree(s->ab_err);
     hipFree(s->gprob_mt);
+    hipFree(s->fit_obs);
+    hipFree(s->fit_idx);
+    hipFree(s->fit_vs);
+    hipFree(s->fit_err);
+    hipFree(s->fit_dead);
     hipFree(s->ser_tab);
     hipFree(s->ser_choice);
     hipFree(s->ser_lc);
@@ -1017,6 +1105,12 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
         FRM_HIP(ctx, hipMemsetAsync(s->ab_err, 0, sizeof(double) * s->nprob, ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
+    s->tt = 0.0;
+    if (s->fit_err) {
+        FRM_HIP(ctx, hipMemsetAsync(s->fit_err, 0, sizeof(double) * s->nprob, ctx->stream));
+        FRM_HIP(ctx, hipMemsetAsync(s->fit_dead, 0, sizeof(int) * s->nprob, ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     if (s->ser_lc) {
         const size_t nl = (size_t)s->ser_maxL * s->ldh;
         FRM_HIP(ctx, hipMemsetAsync(s->ser_lc, 0, sizeof(double) * 2 * nl, ctx->stream));
@@ -1038,6 +1132,39 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
     for (int it = 0; it < nsteps; ++it) {
         const int k = s->tstep;
         const int depth = s->depth;  // = max(0, k - 3)
+        FitArgs fa{};
+        const int fcnt = s->nobs ? s->fit_off[k] - s->fit_off[k - 1] : 0;
+        if (fcnt > 0) {
+            fa.pr = s->probs;
+            fa.nprob = s->nprob;
+            fa.U = s->U;
+            fa.T = s->fit_obs;
+            fa.Z = s->fit_obs + s->nobs;
+            fa.F = s->fit_obs + 2 * (size_t)s->nobs;
+            fa.idx = s->fit_idx + s->fit_off[k - 1];
+            fa.cnt = fcnt;
+            fa.tt = s->tt;
+            fa.tau = s->tau;
+            fa.vs = s->fit_vs;
+            fa.nobs = s->nobs;
+            fa.err = s->fit_err;
+            fa.dead = s->fit_dead;
+            LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+            hipLaunchKernelGGL(k_fit_before, dim3((unsigned)((s->nprob + 255) / 256)), dim3(256), 0,
+                               ctx->stream, fa);
+            FRM_HIP(ctx, hipGetLastError());
+        }
+        // the step's fitting error, once U holds the new state
+        auto fit_after = [&]() -> int {
+            if (fcnt > 0) {
+                LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+                hipLaunchKernelGGL(k_fit_after, dim3((unsigned)((s->nprob + 255) / 256)), dim3(256),
+                                   0, ctx->stream, fa);
+                FRM_HIP(ctx, hipGetLastError());
+            }
+            s->tt += s->tau;
+            return FRM_OK;
+        };
         if (s->kind == FRM_MT_PSI) {
             const int threads = 256;
             dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
@@ -1113,6 +1240,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             }
             s->depth = depth + 1;
             s->tstep = k + 1;
+            if (int rc = fit_after()) return rc;
             continue;
         }
         MtSweepArgs a;
@@ -1138,6 +1266,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         }
         if (k >= 3) s->depth = depth + 1;
         s->tstep = k + 1;
+        if (int rc = fit_after()) return rc;
     }
     return FRM_OK;
 }
@@ -1172,6 +1301,71 @@ int frm_mt1d_ab_error(frm_mt1d* s, double* out) {
     const int steps = s->tstep - 1;
     for (int p = 0; p < s->nprob; ++p)
         out[p] = steps > 0 ? e[p] / ((s->hprobs[p].N + 1.0) * steps) : 0.0;
+    return FRM_OK;
+}
+
+int frm_mt1d_set_observations(frm_mt1d* s, int32_t nobs, const double* T, const double* Z,
+                              const double* F) {
+    if (!s) return set_error(nullptr, FRM_E_INVALID, "frm_mt1d_set_observations: null solver");
+    frm_ctx* ctx = s->ctx;
+    FRM_REQUIRE(ctx, nobs >= 0 && (nobs == 0 || (T && Z && F)),
+                "frm_mt1d_set_observations: bad arguments");
+    hipFree(s->fit_obs);
+    hipFree(s->fit_idx);
+    hipFree(s->fit_vs);
+    s->fit_obs = s->fit_vs = nullptr;
+    s->fit_idx = nullptr;
+    s->nobs = 0;
+    s->fit_off.clear();
+    // which step each observation falls in: the reference's loop variable
+    // tt starts at 0 and grows by tau per step; step k covers [tt, tt + tau)
+    std::vector<std::vector<int>> per(s->capacity + 1);
+    double tt = 0.0;
+    for (int k = 1; k <= s->capacity; ++k) {
+        for (int o = 0; o < nobs; ++o)
+            if ((T[o] >= tt) && (T[o] < (tt + s->tau))) per[k].push_back(o);
+        tt += s->tau;
+    }
+    std::vector<int> idx;
+    s->fit_off.assign(s->capacity + 1, 0);
+    for (int k = 1; k <= s->capacity; ++k) {
+        idx.insert(idx.end(), per[k].begin(), per[k].end());
+        s->fit_off[k] = (int)idx.size();
+    }
+    hipError_t e = hipSuccess;
+    if (!s->fit_err) {
+        e = hipMalloc(&s->fit_err, sizeof(double) * s->nprob);
+        if (e == hipSuccess) e = hipMalloc(&s->fit_dead, sizeof(int) * s->nprob);
+    }
+    if (e == hipSuccess) e = hipMemset(s->fit_err, 0, sizeof(double) * s->nprob);
+    if (e == hipSuccess) e = hipMemset(s->fit_dead, 0, sizeof(int) * s->nprob);
+    if (nobs > 0) {
+        std::vector<double> obs((size_t)3 * nobs);
+        std::copy(T, T + nobs, obs.begin());
+        std::copy(Z, Z + nobs, obs.begin() + nobs);
+        std::copy(F, F + nobs, obs.begin() + 2 * (size_t)nobs);
+        if (e == hipSuccess) e = hipMalloc(&s->fit_obs, sizeof(double) * obs.size());
+        if (e == hipSuccess) e = hipMalloc(&s->fit_idx, sizeof(int) * std::max<size_t>(1, idx.size()));
+        if (e == hipSuccess) e = hipMalloc(&s->fit_vs, sizeof(double) * (size_t)nobs * s->nprob);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->fit_obs, obs.data(), sizeof(double) * obs.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !idx.empty())
+            e = hipMemcpy(s->fit_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(s->fit_vs, 0, sizeof(double) * (size_t)nobs * s->nprob);
+        s->nobs = nobs;
+    }
+    return e == hipSuccess ? FRM_OK : check_hip(ctx, e, "frm_mt1d_set_observations");
+}
+
+int frm_mt1d_fit_error(frm_mt1d* s, double* out) {
+    if (!s || !out) return set_error(s ? s->ctx : nullptr, FRM_E_INVALID, "frm_mt1d_fit_error: null");
+    frm_ctx* ctx = s->ctx;
+    FRM_REQUIRE(ctx, s->fit_err != nullptr, "frm_mt1d_fit_error: no observations were set");
+    std::vector<double> e(s->nprob);
+    FRM_HIP(ctx, hipMemcpyAsync(e.data(), s->fit_err, sizeof(double) * s->nprob, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < s->nprob; ++p) out[p] = std::isfinite(e[p]) ? e[p] : 1e300;
     return FRM_OK;
 }
 

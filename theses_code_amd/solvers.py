@@ -180,6 +180,23 @@ class MT1DBatch:
         N._check(N.lib().frm_mt1d_ab_error(self.h, N._dp(out)), self.ctx.h)
         return out
 
+    def set_observations(self, T, Z, F):
+        """Observations (time, depth, value) scored after every step as the
+        reference's fitting drivers score a particle (solve_and_test); each
+        problem of the batch is one particle.  Clears the errors."""
+        self._obs = [np.ascontiguousarray(a, dtype=np.float64) for a in (T, Z, F)]
+        n = self._obs[0].size
+        assert all(a.size == n for a in self._obs)
+        N._check(N.lib().frm_mt1d_set_observations(self.h, n, *[N._dp(a) for a in self._obs]),
+                 self.ctx.h)
+
+    def fit_error(self) -> np.ndarray:
+        """Per problem: sum of squared misfits so far (1e300 once U[0] was
+        not finite), as solve_and_test returns it."""
+        out = np.empty(len(self.sizes))
+        N._check(N.lib().frm_mt1d_fit_error(self.h, N._dp(out)), self.ctx.h)
+        return out
+
     def close(self):
         if getattr(self, "h", None):
             N.lib().frm_mt1d_destroy(self.h)
