@@ -101,6 +101,22 @@ class H_batch {
         return e;
     }
 
+    // The PSO / steepest-descent drivers (fit_and_solve, mt_minimal.cpp:1401)
+    // score every particle with solve_and_test (1277-1330); with one problem
+    // per particle, one batch scores a whole iteration:
+    //   H_batch b(ctx, desc_of_particles, U0s); b.set_observations(T, Z, F);
+    //   for (...) b.calc_step();  auto err = b.fit_error();
+    void set_observations(const std::vector<double>& T, const std::vector<double>& Z,
+                          const std::vector<double>& F) {
+        check(frm_mt1d_set_observations(s_, (int32_t)T.size(), T.data(), Z.data(), F.data()),
+              ctx_.get());
+    }
+    std::vector<double> fit_error() {
+        std::vector<double> e(b_U.size());
+        check(frm_mt1d_fit_error(s_, e.data()), ctx_.get());
+        return e;
+    }
+
     int tstep = 1;  // the reference's H_solver starts at tstep = 1
     std::vector<std::vector<double>> b_U;
 
