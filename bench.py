@@ -60,8 +60,9 @@ def parse():
     ap.add_argument("--breakdown-plain", action="store_true",
                     help="c2/c4: also time plain FP64 per kernel (a second full run)")
     ap.add_argument("--forms", default="soe,mfma",
-                    help="comma-separated chosen forms timed against plain FP64 on the same "
-                         "workload (soe, mfma; empty for none)")
+                    help="comma-separated forms timed against plain FP64 on the same workload "
+                         "(soe, mfma; binomial = the reference's own series history, c1/c2; "
+                         "empty for none)")
     return ap.parse_args()
 
 
@@ -379,7 +380,10 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
 
 FORMS = {"soe": "sum-of-exponentials history (FP64 mode states)",
          "mfma": "blocked Toeplitz on MFMA: FP32 increments, split hi+lo BF16, FP32 acc, "
-                 "64-step blocks, FP64 near part"}
+                 "64-step blocks, FP64 near part",
+         "binomial": "the reference's single-series history (TG_solver dt_alg 3, 25 terms, "
+                     "int_err 1e-5): its method, not a chosen form"}
+BINOMIAL_TERMS = 25
 
 
 def run_lod_forms(args, cfg_name):
@@ -405,7 +409,13 @@ def run_lod_forms(args, cfg_name):
     forms = args.forms.split(",")
     fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=args.soe_f32,
                        sweep_w=args.sweep_w),
-           "mfma": dict(precision=N.PREC_MFMA_BF16, sweep_w=args.sweep_w)}
+           "mfma": dict(precision=N.PREC_MFMA_BF16, sweep_w=args.sweep_w),
+           "binomial": dict(precision=N.PREC_BINOMIAL, series_terms=BINOMIAL_TERMS,
+                            sweep_w=args.sweep_w)}
+    if cfg_name == "c4" and "binomial" in forms:
+        print("# the binomial-series history is the Caputo solver's (dt_alg 3); skipped on the AB "
+              "sets", file=sys.stderr)
+        forms.remove("binomial")
     tot_nodes, t_plain, kt_plain = 0, 0.0, {}
     res = {f: dict(seconds=0.0, errs=[], rms=[], fits=[], kernels_ms={}, work={}) for f in forms}
     for r in runs:
@@ -575,9 +585,10 @@ def main():
     if args.forms:
         s.close()
         for f in args.forms.split(","):
-            pr = {"mfma": N.PREC_MFMA_BF16, "soe": N.PREC_SOE}[f]
+            fk = {"mfma": dict(precision=N.PREC_MFMA_BF16), "soe": dict(precision=N.PREC_SOE),
+                  "binomial": dict(precision=N.PREC_BINOMIAL, series_terms=BINOMIAL_TERMS)}[f]
             sf = TGSolver(ctx, sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], coefs, C0,
-                          capacity=cap, C1=0.0, S=0.0, precision=pr)
+                          capacity=cap, C1=0.0, S=0.0, **fk)
             if W:
                 sf.calc_step(W)
                 ctx.sync()
