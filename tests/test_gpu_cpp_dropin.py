@@ -1,5 +1,6 @@
-"""The C++ drop-in class (include/frm_solvers.hpp) runs the LOD solver on the
-GPU and matches the oracle (plain FP64, 1e-12)."""
+"""The C++ drop-in classes (include/frm_solvers.hpp) on the GPU: TG_solver
+matches the oracle (plain FP64, 1e-12); H_batch scores a PSO iteration's
+particles as solve_and_test does (oracle's loop, 1e-11)."""
 import os
 import subprocess
 import tempfile
@@ -34,3 +35,31 @@ def test_cpp_tg_solver_dropin():
     o.step(steps)
     ref = o.state().reshape(-1)
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_cpp_pso_batch_scores_particles():
+    binp = os.path.join(REPO, "tests", "cpp", "pso_batch")
+    assert os.path.exists(binp), "build() compiles tests/cpp/pso_batch"
+    rng = np.random.default_rng(17)
+    nprob, N, steps, tau, nobs = 12, 50, 40, 0.01, 30
+    par = np.stack([rng.uniform(0.55, 0.95, nprob), rng.uniform(5e-4, 5e-3, nprob)], 1)
+    x = np.arange(N + 2) / N
+    U0 = -5.0 + np.sin(np.pi * x)
+    U0[0] = U0[N] = -5.0
+    T = rng.uniform(0.0, steps * tau, nobs)
+    Z = rng.uniform(0.05, 0.95, nobs)
+    F = -5.0 + rng.normal(0.0, 0.3, nobs)
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        np.concatenate([np.array([nprob, N, steps, tau, nobs], float), par.reshape(-1), U0,
+                        T, Z, F]).tofile(fi)
+        r = subprocess.run([binp, fi, fo], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert f"tstep {steps + 1} particles {nprob}" in r.stdout
+        got = np.fromfile(fo)
+    for p in range(nprob):
+        o = O.MTSolver(N, 1.0, par[p, 0], tau, U0, form=O.FORM_CONST, D=par[p, 1],
+                       top=O.TOP_DIRICHLET, top_val=U0[0], bottom_cond=1, Hbottom=U0[N])
+        want = O.mt_fit_error(o, N, 1.0, tau, steps, T, Z, F)
+        assert got[p] == pytest.approx(want, rel=1e-11), p

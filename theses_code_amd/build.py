@@ -55,16 +55,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-def build_cpp_example() -> str:
-    """Host-only C++ program that uses include/frm_solvers.hpp (tests/cpp)."""
-    src = os.path.join(REPO, "tests", "cpp", "tg_dropin.cpp")
-    out = os.path.join(REPO, "tests", "cpp", "tg_dropin")
-    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), src, "-o", out,
-           "-L" + HERE, "-lfrm", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../theses_code_amd"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"C++ example build failed:\n{r.stderr}")
-    return out
+def build_cpp_example() -> list:
+    """Host-only C++ programs that use include/frm_solvers.hpp (tests/cpp)."""
+    outs = []
+    for name in ("tg_dropin", "pso_batch"):
+        src = os.path.join(REPO, "tests", "cpp", name + ".cpp")
+        out = os.path.join(REPO, "tests", "cpp", name)
+        cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), src, "-o", out,
+               "-L" + HERE, "-lfrm", "-Wl,-rpath," + HERE,
+               "-Wl,-rpath,$ORIGIN/../../theses_code_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ example {name} build failed:\n{r.stderr}")
+        outs.append(out)
+    return outs
 
 
 if __name__ == "__main__":

@@ -49,6 +49,23 @@ struct MtProbDev {
     int64_t ser_off, ser_coff;
 };
 
+// sum_{c < nch} part[c ldh + q] for nch >= 1: the partials of 8 chunks are
+// loaded together (clamped chunk index, so every load is in bounds), then
+// added in increasing c, as a plain loop adds them.
+__device__ __forceinline__ double chunk_sum(const double* __restrict__ part, int64_t ldh, int nch,
+                                            int64_t q) {
+    double h = 0.0;
+    for (int c0 = 0; c0 < nch; c0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)min(c0 + u, nch - 1) * ldh + q];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (c0 + u < nch) h += x[u];
+    }
+    return h;
+}
+
 // ---------------------------------------------------------------- physics
 // KK(i) and inv_dw_dh(i) of H_solver at state value x (mt_minimal.cpp:481-562
 // for cv_form 2 and k_form 0; CONST and KEXP are this build's forms).
@@ -181,12 +198,14 @@ __device__ inline double psi_inv_der(double beta, double t, int n, double d) {
     return d * ((1.0 / beta) - n + 1) / t;
 }
 
-__device__ double psi_kb_row(double beta, double t0, double t1, double gtj, double alpha) {
+// v_first = f'(gtj) gtj^(1-alpha), the series' first factor (the same for
+// every interval of a step)
+__device__ double psi_kb_row(double beta, double t0, double t1, double gtj, double alpha,
+                             double v_first) {
     const double gt1 = pow(t1, beta), gt0 = pow(t0, beta);
     double sum = 0.0, i = 0.0;
     int niter = 0;
-    double v = psi_inv_der(beta, gtj, 1, psi_inv_der(beta, gtj, 0, 1));
-    v *= pow(gtj, 1 - alpha);
+    double v = v_first;
     double v3 = pow(fabs(1.0 - gt1 / gtj), 1 - alpha);
     double v4 = pow(fabs(1.0 - gt0 / gtj), 1 - alpha);
     double v2 = -(v3 - v4) / (1 - alpha);
@@ -205,12 +224,17 @@ __device__ double psi_kb_row(double beta, double t0, double t1, double gtj, doub
     return sum;
 }
 
-__device__ double psi_kb_row2(double beta, double t0, double t1, double gtj, double alpha) {
+// gtj_na = gtj^-alpha (the same for every interval of a step)
+__device__ double psi_kb_row2(double beta, double t0, double t1, double gtj, double alpha,
+                              double gtj_na) {
     const double gt0 = pow(t0, beta), gt1 = pow(t1, beta);
     double sum = 0.0, v1 = 1.0, v2, v3, v4, v5, v6, v7, i = 0.0, m;
     int niter = 0;
-    v2 = pow(gtj, -alpha) * gt1;
+    v2 = gtj_na * gt1;
     v6 = gt0 / gt1;
+    // the first derivative of the inverse at g(t1) starts every term's inner
+    // series; it depends on the interval only (three pow calls, made once)
+    const double v4_first = psi_inv_der(beta, gt1, 1, psi_inv_der(beta, gt1, 0, 0));
     do {
         if (i != 0.0) {
             v1 *= -(-alpha - i + 1.0) / i;
@@ -222,7 +246,7 @@ __device__ double psi_kb_row2(double beta, double t0, double t1, double gtj, dou
         } else {
             v3 = 0.0;
             m = 0.0;
-            v4 = psi_inv_der(beta, gt1, 1, psi_inv_der(beta, gt1, 0, 0));
+            v4 = v4_first;
             v5 = (1.0 / (i + 1.0)) * (1.0 - v6);
             v3 += v4 * v5;
             v7 = 1;
@@ -249,11 +273,27 @@ __device__ double psi_kb_row2(double beta, double t0, double t1, double gtj, dou
     return sum;
 }
 
-__device__ double psi_calc_int(double alpha, double beta, int a, int A, double tau) {
-    const double gtj = pow(A * tau, beta);
+// The factors of calc_int that depend only on the target g(A tau) of the
+// step, formed once per problem and step: gtj, f'(gtj) gtj^(1-alpha) and
+// gtj^-alpha (the same pow calls the per-interval series would make).
+struct PsiStep {
+    double gtj, v_first, gtj_na;
+};
+
+__device__ inline PsiStep psi_step(double alpha, double beta, int A, double tau) {
+    PsiStep q;
+    q.gtj = pow(A * tau, beta);
+    double v = psi_inv_der(beta, q.gtj, 1, psi_inv_der(beta, q.gtj, 0, 1));
+    v *= pow(q.gtj, 1 - alpha);
+    q.v_first = v;
+    q.gtj_na = pow(q.gtj, -alpha);
+    return q;
+}
+
+__device__ double psi_calc_int(double alpha, double beta, int a, int A, double tau, const PsiStep& q) {
     if ((a + 1 <= 0.5 * A) && !(beta == 1))
-        return psi_kb_row2(beta, a * tau, (a + 1) * tau, gtj, alpha);
-    return psi_kb_row(beta, a * tau, (a + 1) * tau, gtj, alpha);
+        return psi_kb_row2(beta, a * tau, (a + 1) * tau, q.gtj, alpha, q.gtj_na);
+    return psi_kb_row(beta, a * tau, (a + 1) * tau, q.gtj, alpha, q.v_first);
 }
 
 // Step k (= tstep): w[p*ldw + l] = (1-alpha) kb_time[depth-l] for l < depth
@@ -263,11 +303,15 @@ __global__ void k_psi_weights(const MtProbDev* __restrict__ pr, int nprob, int k
                               double tau, double* __restrict__ w, int ldw, float* __restrict__ w32,
                               double* __restrict__ d0, int* __restrict__ bad) {
     const int p = blockIdx.y;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nprob || x > depth) return;
+    if (p >= nprob) return;  // the whole workgroup: one problem per grid row
     const MtProbDev P = pr[p];
+    __shared__ PsiStep q;
+    if (threadIdx.x == 0) q = psi_step(P.alpha, P.beta, k + 1, tau);
+    __syncthreads();
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x > depth) return;
     const int a = (x < depth) ? x + 1 : k;
-    const double v = psi_calc_int(P.alpha, P.beta, a, k + 1, tau);
+    const double v = psi_calc_int(P.alpha, P.beta, a, k + 1, tau, q);
     if (!isfinite(v)) atomicOr(bad, 1);
     if (x < depth) {
         const double wl = v * (1 - P.alpha);
@@ -314,9 +358,7 @@ __global__ __launch_bounds__(256) void k_mt_sweep(MtSweepArgs a) {
         hs[k] = 0.0;
         if (e < n) {
             u0[k] = xs[e + 1];
-            double h = 0.0;
-            for (int c = 0; c < a.nch; ++c) h += a.part[(int64_t)c * a.ldh + P.hoff + e];
-            hs[k] = h;
+            hs[k] = a.nch > 0 ? chunk_sum(a.part, a.ldh, a.nch, P.hoff + e) : 0.0;
         }
     }
     const double d0 = a.d0[p];
@@ -523,8 +565,7 @@ __global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
                 if (P.ser) {  // MF_intF_choice's value, already over (1 - a)
                     rp -= a.part[P.hoff + e];
                 } else {
-                    double hs = 0.0;
-                    for (int c = 0; c < a.nch; ++c) hs += a.part[(int64_t)c * a.ldh + P.hoff + e];
+                    const double hs = a.nch > 0 ? chunk_sum(a.part, a.ldh, a.nch, P.hoff + e) : 0.0;
                     rp -= hs / (1.0 - al);
                 }
             }
