@@ -44,12 +44,13 @@ def parse():
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
     ap.add_argument("--config", default="c1",
-                    choices=["c1", "c0", "c2", "c3", "c4", "ab1d", "ab1d_series"],
+                    choices=["c1", "c0", "c2", "c3", "c4", "ab1d", "ab1d_series", "timesum3d"],
                     help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
                          "c2: 2D LOD 1024x1024 plain FP64 vs the chosen forms; c3: 4096 "
                          "psi-Caputo problems, FP64 vs compensated FP32; c4: 16 AB sets, plain "
                          "FP64 vs the chosen forms; ab1d: 1024 AB test problems, full history; "
-                         "ab1d_series: 1024 AB test problems, the reference's series history")
+                         "ab1d_series: 1024 AB test problems, the reference's series history; "
+                         "timesum3d: time sums of the 3D irrigation solver, 128x128x64 nodes")
     ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
     ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -499,6 +500,126 @@ def run_lod_forms(args, cfg_name):
     ctx.close()
 
 
+def run_timesum(args):
+    """Time sums of the 3D irrigation solver (the Sum family): plain FP64
+    full-history sums (sum_alg 0), the compensated FP32 store as the chosen
+    form, and the reference's series form (sum_alg 2).  A step is one
+    push of the newest level plus the sums of every node, with the weights
+    (kb_3_cache, tau) handed over from the host as the reference's caller
+    does; the levels are device-resident (a pool of distinct levels pushed in
+    turn)."""
+    from oracle import oracle as O
+    from theses_code_amd import _native as N
+    from theses_code_amd import workloads as Wl
+    from theses_code_amd.solvers import TimeSum
+
+    cfg = Wl.TIMESUM3D_WORKLOAD
+    n = (cfg["sN"] + 2) * (cfg["M"] + 2) * (cfg["K"] + 2)
+    K = args.steps if args.steps != 5000 else cfg["steps"]
+    tau, alpha = cfg["tau"], cfg["alpha"]
+    T = np.arange(K + 2) * tau
+    taus = np.full(K + 1, tau)
+    kbs = [Wl.caputo_interval_weights(T[:k + 1], T[k + 1], alpha) for k in range(K)]
+    ctx = N.Context(0)
+    pool_h = Wl.timesum3d_levels(n, cfg["pool"])
+    pool = [N.DeviceArray.from_host(ctx, a) for a in pool_h]
+
+    def run(precision, sum_alg=0, timing=False):
+        ts = TimeSum(ctx, n, sum_alg=sum_alg, capacity=K + 1, precision=precision,
+                     sum_param=cfg["series_terms"] if sum_alg == 2 else 0.0)
+        # warm up the kernels once, then restart from an empty store
+        ts.push(pool[0])
+        ts.push(pool[1])
+        if sum_alg == 2:
+            ts.eval_device(coefs=np.ones(cfg["series_terms"]), gtj=2 * tau, gt=tau,
+                           alpha=alpha, tau_n=tau)
+        else:
+            ts.eval_device(kb_3_cache=kbs[1], tau=taus)
+        ts.clear()
+        j = np.arange(cfg["series_terms"])
+        if timing:
+            ctx.timing_reset()
+            ctx.timing_enable(1 << N.K_MEMTERM)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(K):
+            ts.push(pool[k % len(pool)])
+            if sum_alg == 2:
+                T0, T1 = T[k - 1] if k else 0.0, T[k]
+                coefs = (1.0 - (T0 / T1) ** (j + 1)) / (j + 1) if k else j * 0.0
+                ts.eval_device(coefs=coefs, gtj=T[k + 1], gt=T1, alpha=alpha, tau_n=tau)
+            else:
+                ts.eval_device(kb_3_cache=kbs[k], tau=taus)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        kt = ctx.timing(N.K_MEMTERM) if timing else None
+        if timing:
+            ctx.timing_enable(0)
+        S = ts.eval_device(kb_3_cache=kbs[K - 1], tau=taus) if sum_alg != 2 else ts._S
+        out = S.to_host()
+        ts.close()
+        return dt, kt, out
+
+    dt, kt, S64 = run(N.PREC_F64, timing=True)
+    launches, ms, nbytes = kt
+    value = n * K / dt
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms else 0.0
+    # parity of the last step against the oracle on a node sample, all levels
+    rng = np.random.default_rng(cfg["seed"])
+    sub = np.sort(rng.choice(n, 4096, replace=False))
+    old_sub = np.stack([pool_h[k % len(pool_h)][sub] for k in range(K)])
+    ref = O.timesum01(0, 0.0, old_sub, kbs[K - 1], taus)
+    par = float(np.max(np.abs(S64[sub] - ref)) / np.max(np.abs(ref)))
+    dt32, kt32, S32 = run(N.PREC_F32C, timing=True)
+    d = S32 - S64
+    chosen = {"form": "FP32 level differences, Kahan-compensated FP32 sums (FRM_PREC_F32C)",
+              "value": n * K / dt32, "seconds": dt32, "speedup_vs_plain_fp64": dt / dt32,
+              "max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(S64))),
+              "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(S64 ** 2))),
+              "kernel_ms": kt32[1],
+              "kernel_GBs": kt32[2] / (kt32[1] * 1e-3) / 1e9 if kt32[1] else 0.0}
+    dts, kts, _ = run(N.PREC_F64, sum_alg=2, timing=True)
+    series = {"form": f"the reference's series sums (sum_alg 2), {cfg['series_terms']} moments "
+                      "per node; its accuracy is the method's and the caller's coefficients'",
+              "value": n * K / dts, "seconds": dts, "kernel_ms": kts[1],
+              "kernel_GBs": kts[2] / (kts[1] * 1e-3) / 1e9 if kts[1] else 0.0}
+    for a in pool:
+        a.free()
+    del old_sub
+    # CPU: the oracle's time_summing on a node sample at the run's mean depth
+    cpu = None
+    if args.cpu_steps > 0:
+        ns, depth = 65536, max(K // 2, 1)
+        old_cpu = np.stack([pool_h[k % len(pool_h)][:ns] for k in range(depth + 1)])
+        kb = Wl.caputo_interval_weights(T[:depth + 1], T[depth + 1], alpha)
+        O.timesum01(0, 0.0, old_cpu[:8], kb, taus)
+        reps, t0 = 0, time.perf_counter()
+        while reps < 3 or time.perf_counter() - t0 < 5.0:
+            O.timesum01(0, 0.0, old_cpu, kb, taus)
+            reps += 1
+        dcpu = (time.perf_counter() - t0) / reps
+        lib = O.lib()
+        cpu = {"value": ns / dcpu, "unit": "node-steps/s",
+               "cores": lib.orc_get_threads() if hasattr(lib, "orc_get_threads") else 1,
+               "kind": "port",
+               "sample": f"oracle time_summing (sum_alg 0) on {ns} nodes at depth {depth} "
+                         f"(the run's mean), {reps} repetitions; OpenMP over nodes"}
+    line = {"metric": METRIC, "value": value, "unit": "node-steps/s", "n_gpus": 1, "steps": K,
+            "warmup": 1, "ms_per_step": dt * 1e3 / K, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": cfg["name"], "nodes": n, "tau": tau, "alpha": alpha,
+                       "sum_alg": 0, "parallelism": "1 GPU"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_memterm_f64<1>", "launches": launches,
+                         "avg_launch_ms": ms / launches if launches else None},
+            "parity": {"vs": "oracle time_summing restatement, last step, 4096-node sample",
+                       "max_rel_err": par},
+            "chosen_form": chosen, "series_form": series, "cpu_baseline": cpu}
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -513,6 +634,9 @@ def main():
         return
     if args.config in ("ab1d", "ab1d_series"):
         run_ab1d(args, series=args.config == "ab1d_series")
+        return
+    if args.config == "timesum3d":
+        run_timesum(args)
         return
 
     from theses_code_amd import _native as N

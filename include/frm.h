@@ -302,6 +302,53 @@ int frm_mt1d_set_observations(frm_mt1d* s, int32_t nobs, const double* T, const 
  * were cleared. */
 int frm_mt1d_fit_error(frm_mt1d* s, double* out);
 
+/* ------------------------------------- time sums of the 3D solver (Sum) ----
+ * Drop-in for fr_ocl_call_sum and its Sum kernel (irrigation_fractional.cpp:
+ * driver 1828-1905, OpenCL kernel 1254-1320, CUDA kernel 1416-1489) and for
+ * the CPU loop they replace, H_solver::time_summing (4099-4170), which reads
+ * the weights precalc_kb_3 leaves in kb_3_cache (4081-4098).  The time
+ * levels (oldH, n_nodes = (sN+2)(_M+2)(K+2) values each) stay on the device;
+ * per node id, with old_size levels pushed:
+ *   sum_alg 0: S[id] = sum_{t=old_size-2}^{0} kb_3_cache[t] (old[t+1][id] - old[t][id]) / tau[t]
+ *   sum_alg 1: the same, newest first, stopping before the first t with
+ *              |kb_3_cache[t]| < sum_param (as the CPU loop; the reference's
+ *              GPU kernel also adds the first such term)
+ *   sum_alg 2: Htmp[id][j] += gt^(j+1) coefs[j] (old[last][id] - old[last-1][id])
+ *              for j < min(ceil(sum_param), ncoefs), then
+ *              S[id] = _kb_row2_fixed_coefs(gtj, gtj^-alpha, Htmp[id], ceil(sum_param), alpha) / tau_n
+ * S is 0 while fewer than two levels exist.  sum_alg 2 folds each increment
+ * into the moments once: the first eval after a push folds it, later evals
+ * before the next push only re-evaluate.  The reference's double / float /
+ * half kernel variants map to FRM_PREC_F64 and FRM_PREC_F32C (FP32 levels
+ * differences, compensated sums; sum_alg 0 and 1). */
+typedef struct frm_timesum frm_timesum;
+typedef struct frm_timesum_desc {
+    int64_t n_nodes;
+    int32_t capacity;  /* time levels held (sum_alg 0 and 1); push beyond it fails */
+    int32_t sum_alg;   /* 0, 1 or 2 */
+    double sum_param;  /* sum_alg 1: weight threshold; 2: number of series terms */
+    int32_t precision; /* FRM_PREC_F64 or FRM_PREC_F32C (sum_alg 0 and 1) */
+} frm_timesum_desc;
+typedef struct frm_timesum_args {
+    const double* kb_3_cache; /* host, old_size-1 weights by level t (sum_alg 0, 1) */
+    const double* tau;        /* host, tau[t] for t < old_size-1 (sum_alg 0, 1) */
+    const double* coefs;      /* host, row2_precalc[time_i1][0] (sum_alg 2) */
+    int32_t ncoefs;           /* its length (sum_alg 2) */
+    double gtj;               /* g(Time[tstep+1]) (sum_alg 2) */
+    double gt;                /* g(Time[tstep]) (sum_alg 2) */
+    double alpha;             /* time derivative order (sum_alg 2) */
+    double tau_n;             /* tau[tstep] (sum_alg 2) */
+} frm_timesum_args;
+int frm_timesum_create(frm_ctx* ctx, const frm_timesum_desc* desc, frm_timesum** out);
+int frm_timesum_destroy(frm_timesum* s);
+/* Clears the levels and the moments (a new run on the same grid). */
+int frm_timesum_clear(frm_timesum* s);
+/* oldH.push_back(level): d_level is a device array of n_nodes values. */
+int frm_timesum_push(frm_timesum* s, const double* d_level);
+int frm_timesum_levels(const frm_timesum* s, int32_t* old_size);
+/* S (device, n_nodes values) for the levels pushed so far. */
+int frm_timesum_eval(frm_timesum* s, const frm_timesum_args* args, double* d_S);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1542,3 +1542,50 @@ int orc_get_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int n) { (void)n; }
 int orc_get_threads(void) { return 1; }
 #endif
+
+/* ---------------------------------------------------------------- Sum family
+ * H_solver::time_summing (irrigation_fractional.cpp:4099-4170), the CPU loop
+ * that the Sum kernel (1254-1320, 1416-1489) accelerates.  sum_alg 0/1: newest
+ * level pair first; sum_alg 1 stops before the first |kb_3_cache[t]| <
+ * sum_param (4129-4142); each term adds kv*diff/tau[t] (4144). */
+void orc_timesum01(int sum_alg, double sum_param, int64_t n, int nlev, const double* old,
+                   const double* kb, const double* tau, double* S) {
+#pragma omp parallel for schedule(static)
+    for (int64_t id = 0; id < n; ++id) {
+        double time_sum = 0;
+        for (int t = nlev - 2; t >= 0; t--) {
+            double kv = kb[t];
+            double diff = old[(int64_t)(t + 1) * n + id] - old[(int64_t)t * n + id];
+            if (sum_alg == 1 && fabs(kv) < sum_param) break;
+            time_sum += kv * diff / tau[t];
+        }
+        S[id] = time_sum;
+    }
+}
+
+/* sum_alg 2 (4150-4164): mult starts at g(Time[tstep]) and is multiplied by
+ * it after each coefficient that exists; _kb_row2_fixed_coefs with right = 0
+ * starts v1 at gtj^-alpha (1168-1188). */
+void orc_timesum2(int64_t n, int nterms, double* H, const double* diff, const double* coefs,
+                  int ncoefs, double gtj, double gt, double alpha, double tau_n, double* S) {
+    double gtjma = pow(gtj, -alpha);
+#pragma omp parallel for schedule(static)
+    for (int64_t id = 0; id < n; ++id) {
+        double mult = gt;
+        for (int j = 0; j < nterms; j++)
+            if (ncoefs > j) {
+                H[(int64_t)j * n + id] += mult * coefs[j] * diff[id];
+                mult *= gt;
+            }
+        double sum = 0.0, v1 = gtjma, i = 0;
+        int ii = 0;
+        do {
+            sum += v1 * H[(int64_t)ii * n + id];
+            i += 1.0;
+            ii++;
+            v1 *= -(-alpha - i + 1.0) / (i * gtj);
+        } while (ii != nterms);
+        S[id] = sum / tau_n;
+    }
+}
+

@@ -115,6 +115,19 @@ double orc_ab_solve_series(double a, double T, int n, int m, int maxl, int maxki
 /* MF_int_choice(tj, ts1, ts, a, a, maxl, maxkiter, eps) */
 double orc_ab_int(double tj, double ts1, double ts, double a, int maxl, int maxkiter, double eps);
 
+/* ---- Sum family of irrigation_fractional.cpp ------------------------- */
+/* H_solver::time_summing for sum_alg 0 and 1 (irrigation_fractional.cpp:
+ * 4121-4148) at every node: old holds nlev levels of n values (level-major),
+ * kb the nlev-1 weights kb_3_cache[t], tau the steps tau[t]. */
+void orc_timesum01(int sum_alg, double sum_param, int64_t n, int nlev, const double* old,
+                   const double* kb, const double* tau, double* S);
+/* time_summing for sum_alg 2 (4150-4164): folds the newest increment diff
+ * into the moments H (nterms x n, term-major) with the first
+ * min(nterms, ncoefs) coefficients, then evaluates _kb_row2_fixed_coefs
+ * (1168-1188) over all nterms moments and divides by tau_n. */
+void orc_timesum2(int64_t n, int nterms, double* H, const double* diff, const double* coefs,
+                  int ncoefs, double gtj, double gt, double alpha, double tau_n, double* S);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,8 @@ def lib():
             L.orc_ab_series_history.argtypes = [_D, _D, _I, _I, _D, _I, _P, _P, _P]
             L.orc_ab_solve_series.restype = _D
             L.orc_ab_solve_series.argtypes = [_D, _D, _I, _I, _I, _I, _D, _I, _P]
+        L.orc_timesum01.argtypes = [_I, _D, ct.c_int64, _I, _P, _P, _P, _P]
+        L.orc_timesum2.argtypes = [ct.c_int64, _I, _P, _P, _P, _I, _D, _D, _D, _D, _P]
         if hasattr(L, "orc_set_threads"):
             L.orc_set_threads.argtypes = [_I]
             L.orc_get_threads.restype = _I
@@ -294,3 +296,40 @@ def ab_solve_series(a, T, n, m, maxl, maxkiter, eps, F_func=0):
 
 def ab_int(tj, ts1, ts, a, maxl=50, maxkiter=1000, eps=1e-10) -> float:
     return lib().orc_ab_int(tj, ts1, ts, a, maxl, maxkiter, eps)
+
+
+def timesum01(sum_alg, sum_param, old, kb, tau):
+    """H_solver::time_summing, sum_alg 0 or 1, at every node of the levels
+    old[nlev, n] (irrigation_fractional.cpp:4121-4148)."""
+    old = np.ascontiguousarray(old, dtype=np.float64)
+    nlev, n = old.shape
+    kb = np.ascontiguousarray(kb, dtype=np.float64)
+    tau = np.ascontiguousarray(tau, dtype=np.float64)
+    assert kb.size >= nlev - 1 and tau.size >= nlev - 1
+    S = np.empty(n)
+    lib().orc_timesum01(sum_alg, sum_param, n, nlev, _p(old), _p(kb), _p(tau), _p(S))
+    return S
+
+
+class TimeSum2:
+    """time_summing with sum_alg 2 (irrigation_fractional.cpp:4150-4164): the
+    per-node moments Htmp, folded with the newest increment at each call."""
+
+    def __init__(self, n, sum_param):
+        self.n = n
+        self.nterms = int(np.ceil(sum_param))
+        self.H = np.zeros((self.nterms, n))
+
+    def __call__(self, diff, coefs, gtj, gt, alpha, tau_n):
+        diff = np.ascontiguousarray(diff, dtype=np.float64)
+        coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+        if coefs.size == 0:
+            coefs = np.zeros(1)
+            nc = 0
+        else:
+            nc = coefs.size
+        S = np.empty(self.n)
+        lib().orc_timesum2(self.n, self.nterms, _p(self.H), _p(diff), _p(coefs), nc, gtj, gt,
+                           alpha, tau_n, _p(S))
+        return S
+

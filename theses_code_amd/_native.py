@@ -61,6 +61,16 @@ class MT1DDesc(ct.Structure):
                 ("probs", ct.POINTER(MT1DProblem))]
 
 
+class TimesumDesc(ct.Structure):
+    _fields_ = [("n_nodes", ct.c_int64), ("capacity", ct.c_int32), ("sum_alg", ct.c_int32),
+                ("sum_param", _D), ("precision", ct.c_int32)]
+
+
+class TimesumArgs(ct.Structure):
+    _fields_ = [("kb_3_cache", _P), ("tau", _P), ("coefs", _P), ("ncoefs", ct.c_int32),
+                ("gtj", _D), ("gt", _D), ("alpha", _D), ("tau_n", _D)]
+
+
 MT_L1, MT_PSI, MT_AB = 0, 1, 2
 FORM_CONST, FORM_CV2, FORM_KEXP = 0, 1, 2
 TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
@@ -113,6 +123,12 @@ SIGNATURES = {
     "frm_mt1d_ab_error": (_I, [_V, _P]),
     "frm_mt1d_set_observations": (_I, [_V, _I, _P, _P, _P]),
     "frm_mt1d_fit_error": (_I, [_V, _P]),
+    "frm_timesum_create": (_I, [_V, ct.POINTER(TimesumDesc), ct.POINTER(_V)]),
+    "frm_timesum_destroy": (_I, [_V]),
+    "frm_timesum_clear": (_I, [_V]),
+    "frm_timesum_push": (_I, [_V, _V]),
+    "frm_timesum_levels": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_timesum_eval": (_I, [_V, ct.POINTER(TimesumArgs), _V]),
 }
 
 _lib = None

@@ -85,6 +85,54 @@ __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f64(
     }
 }
 
+// FP32 increments and weights, one target: each lane's terms go through a
+// Kahan-compensated FP32 sum, returned in FP64 per chunk.  Eight slots in
+// flight per lane move the same 64 bytes as the FP64 kernel's four.
+typedef float frm_fv2 __attribute__((ext_vector_type(2)));
+
+__device__ inline void kahan_add(float& s, float& c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__global__ __launch_bounds__(kMemtermThreads) void k_memterm_f32c(
+    const float* __restrict__ hist, int64_t ld, int depth, int first, const float* __restrict__ w,
+    int chunk, double* __restrict__ part) {
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (i >= ld) return;
+    const int c = blockIdx.y;
+    const int s0 = first + c * chunk;
+    const int s1 = min(depth, s0 + chunk);
+    float sx = 0.f, cx = 0.f, sy = 0.f, cy = 0.f;
+    const int64_t ldv = ld / 2;
+    const frm_fv2* p = reinterpret_cast<const frm_fv2*>(hist + (int64_t)s0 * ld + i);
+    const float* wp = w + (depth - 1 - s0);
+    int s = s0;
+    for (; s + 8 <= s1; s += 8) {
+        frm_fv2 x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(p + (int64_t)u * ldv);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            kahan_add(sx, cx, wp[-u] * x[u].x);
+            kahan_add(sy, cy, wp[-u] * x[u].y);
+        }
+        p += 8 * ldv;
+        wp -= 8;
+    }
+    for (; s < s1; ++s) {
+        const frm_fv2 x = __builtin_nontemporal_load(p);
+        kahan_add(sx, cx, wp[0] * x.x);
+        kahan_add(sy, cy, wp[0] * x.y);
+        p += ldv;
+        wp -= 1;
+    }
+    part[(int64_t)c * ld + i] = (double)sx - (double)cx;
+    part[(int64_t)c * ld + i + 1] = (double)sy - (double)cy;
+}
+
 // out[t*n + i] = scale * sum_c part[(c*NT+t)*ld + i] (+ out)
 __global__ void k_memterm_reduce(const double* __restrict__ part, int64_t ld, int nchunks,
                                  int nt, int64_t n, double scale, int accumulate,
@@ -109,9 +157,12 @@ __global__ void k_sub(const double* __restrict__ a, const double* __restrict__ b
 int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precision,
                frm_hist** out) {
     FRM_REQUIRE(ctx, ctx && out && n_nodes > 0 && capacity > 0, "frm_hist_create: bad arguments");
-    FRM_REQUIRE(ctx, precision == FRM_PREC_F64 || precision == FRM_PREC_MFMA_BF16,
+    FRM_REQUIRE(ctx,
+                precision == FRM_PREC_F64 || precision == FRM_PREC_MFMA_BF16 ||
+                    precision == FRM_PREC_F32C,
                 "frm_hist_create: precision %d not supported", precision);
-    // FP64 increments, or FP32 increments for the blocked MFMA form
+    // FP64 increments, or FP32 increments for the blocked MFMA and the
+    // compensated FP32 forms
     const size_t esz = precision == FRM_PREC_F64 ? sizeof(double) : sizeof(float);
     frm_hist* h = new frm_hist();
     h->ctx = ctx;
@@ -157,7 +208,7 @@ static int pick_chunks(const frm_hist* h, int depth) {
 }
 
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
-                     int* nchunks_out, int first) {
+                     int* nchunks_out, int first, const float* d_w32) {
     const int depth = h->depth;
     first = std::max(0, std::min(first, depth));
     const int span = depth - first;  // slots first .. depth-1, lags depth-first .. 1
@@ -167,6 +218,18 @@ int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w,
     const int chunk = (span + nchunks - 1) / nchunks;
     const int threads = kMemtermThreads;
     dim3 grid((unsigned)(h->ld / 2 / threads), (unsigned)nchunks);
+    if (h->precision == FRM_PREC_F32C) {
+        FRM_REQUIRE(ctx, ntargets == 1 && d_w32, "memterm_partials: FP32 store takes one target");
+        LaunchTimer timer(ctx, FRM_K_MEMTERM,
+                          (double)span * h->n_nodes * sizeof(float) +
+                              (double)h->n_nodes * sizeof(double));
+        hipLaunchKernelGGL(k_memterm_f32c, grid, dim3(threads), 0, ctx->stream,
+                           (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+        FRM_HIP(ctx, hipGetLastError());
+        return FRM_OK;
+    }
+    FRM_REQUIRE(ctx, h->precision == FRM_PREC_F64, "memterm_partials: precision %d",
+                h->precision);
     LaunchTimer timer(ctx, FRM_K_MEMTERM,
                       (double)span * h->n_nodes * sizeof(double) +
                           (double)ntargets * h->n_nodes * sizeof(double));
@@ -176,6 +239,15 @@ int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w,
     else
         hipLaunchKernelGGL(k_memterm_f64<2>, grid, dim3(threads), 0, ctx->stream,
                            (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, h->part);
+    FRM_HIP(ctx, hipGetLastError());
+    return FRM_OK;
+}
+
+int memterm_reduce1(frm_ctx* ctx, const frm_hist* h, int nchunks, double* d_out) {
+    const int threads = 256;
+    hipLaunchKernelGGL(k_memterm_reduce, dim3((unsigned)((h->n_nodes + threads - 1) / threads)),
+                       dim3(threads), 0, ctx->stream, (const double*)h->part, h->ld, nchunks, 1,
+                       h->n_nodes, 1.0, 0, d_out);
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
 }

@@ -204,3 +204,77 @@ class MT1DBatch:
 
     def __del__(self):
         self.close()
+
+
+class TimeSum:
+    """The Sum family of irrigation_fractional.cpp on the MI355X: the time
+    sums H_solver::time_summing computes (4099-4170) and fr_ocl_call_sum
+    offloads (1828-1905), over time levels kept on the device.
+
+    push(level) is oldH.push_back(level) (a host array of n_nodes values, or a
+    DeviceArray); sums(...) returns S[id] for every node, given what the
+    reference's caller holds at that step: kb_3_cache and tau (sum_alg 0, 1),
+    or row2_precalc's coefficients with g(Time[tstep+1]), g(Time[tstep]),
+    alpha and tau[tstep] (sum_alg 2)."""
+
+    def __init__(self, ctx: N.Context, n_nodes: int, sum_alg: int = 0, sum_param: float = 0.0,
+                 capacity: int = 1, precision: int = N.PREC_F64):
+        self.ctx, self.n = ctx, int(n_nodes)
+        d = N.TimesumDesc(self.n, capacity, sum_alg, sum_param, precision)
+        h = N._V()
+        N._check(N.lib().frm_timesum_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
+        self.h = h
+        ctx.adopt(self)
+        self._lev = N.DeviceArray(ctx, self.n)
+        self._S = N.DeviceArray(ctx, self.n)
+
+    def push(self, level):
+        if isinstance(level, N.DeviceArray):
+            assert level.n == self.n
+            ptr = level.ptr
+        else:
+            level = np.ascontiguousarray(level, dtype=np.float64).reshape(-1)
+            assert level.size == self.n
+            self.ctx.upload(self._lev.ptr, level)
+            ptr = self._lev.ptr
+        N._check(N.lib().frm_timesum_push(self.h, ptr), self.ctx.h)
+
+    @property
+    def old_size(self) -> int:
+        v = ct.c_int32()
+        N._check(N.lib().frm_timesum_levels(self.h, ct.byref(v)), self.ctx.h)
+        return v.value
+
+    def eval_device(self, kb_3_cache=None, tau=None, coefs=None, gtj=0.0, gt=0.0, alpha=0.0,
+                    tau_n=1.0) -> N.DeviceArray:
+        """The sums into a device buffer (no copy back)."""
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return N._dp(a)
+
+        nc = 0 if coefs is None else int(np.asarray(coefs).size)
+        a = N.TimesumArgs(ptr(kb_3_cache), ptr(tau), ptr(coefs), nc, gtj, gt, alpha, tau_n)
+        N._check(N.lib().frm_timesum_eval(self.h, ct.byref(a), self._S.ptr), self.ctx.h)
+        return self._S
+
+    def sums(self, **kw) -> np.ndarray:
+        return self.eval_device(**kw).to_host()
+
+    def clear(self):
+        N._check(N.lib().frm_timesum_clear(self.h), self.ctx.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().frm_timesum_destroy(self.h)
+            self.h = None
+            self._lev.free()
+            self._S.free()
+
+    def __del__(self):
+        self.close()
+

@@ -169,3 +169,28 @@ def ab1d_case(nprob: int = AB1D_WORKLOAD["nprob"], seed: int = AB1D_WORKLOAD["se
                           ab_maxl=AB1D_WORKLOAD["maxl"], ab_maxkiter=AB1D_WORKLOAD["maxkiter"],
                           ab_eps=AB1D_WORKLOAD["eps"]))
     return probs
+
+
+# Time sums of the 3D irrigation solver (irrigation_fractional.cpp, the Sum
+# family): a (sN+2)(M+2)(K+2) = 128 x 128 x 64 node grid, its default step of
+# 50 s (tau_m = 1000/20, 9569), g(t) = t, the full-history sum (sum_alg 0)
+# over 2000 steps with the exact Caputo interval weights as kb_3_cache, and
+# the series form (sum_alg 2) with 16 moments per node.
+TIMESUM3D_WORKLOAD = dict(name="timesum3d_128x128x64", sN=126, M=126, K=62, tau=50.0,
+                          alpha=0.8, steps=2000, series_terms=16, pool=16,
+                          seed=SEED_BASE + 7)
+
+
+def timesum3d_levels(n: int, count: int, seed: int = TIMESUM3D_WORKLOAD["seed"]):
+    """`count` distinct synthetic time levels of n nodes (smooth in time)."""
+    rng = np.random.default_rng(seed)
+    base = rng.uniform(-1, 1, n)
+    rate = rng.uniform(-1, 1, n)
+    return [base + rate * np.sqrt(k + 1.0) for k in range(count)]
+
+
+def caputo_interval_weights(T: np.ndarray, T_end: float, alpha: float) -> np.ndarray:
+    """kb(alpha, t, t+1, n+1) for g(t) = t without the 1/Gamma(1-alpha)
+    factor: the integral of (T_end - x)^-alpha over [T[t], T[t+1]]."""
+    return ((T_end - T[:-1]) ** (1 - alpha) - (T_end - T[1:]) ** (1 - alpha)) / (1 - alpha)
+
