@@ -87,9 +87,12 @@ def test_sum01_linear_history_is_caputo_of_linear_at_full_size(ctx):
     g.close()
 
 
-@pytest.mark.parametrize("sum_alg", [0, 1])
-def test_sum01_f32c_within_1e6(ctx, sum_alg):
-    n, nlev = 50000, 200
+# 50000 nodes take the 2-nodes-per-lane FP32 kernel, 102^3 + 5 (a store
+# stride of 1037 x 1024) the 4-nodes-per-lane one with a half-idle last
+# workgroup
+@pytest.mark.parametrize("sum_alg,n,nlev", [(0, 50000, 200), (1, 50000, 200),
+                                            (0, 102 ** 3 + 5, 40)])
+def test_sum01_f32c_within_1e6(ctx, sum_alg, n, nlev):
     rng = np.random.default_rng(9 + sum_alg)
     old = levels(rng, nlev, n)
     tau = rng.uniform(0.5, 1.5, nlev)

@@ -86,9 +86,12 @@ __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f64(
 }
 
 // FP32 increments and weights, one target: each lane's terms go through a
-// Kahan-compensated FP32 sum, returned in FP64 per chunk.  Eight slots in
-// flight per lane move the same 64 bytes as the FP64 kernel's four.
-typedef float frm_fv2 __attribute__((ext_vector_type(2)));
+// Kahan-compensated FP32 sum, returned in FP64 per chunk, eight slots in
+// flight.  A lane owns NPL = 2 or 4 adjacent nodes (8- or 16-byte loads).
+// Measured on MI355X at depth 1000: 2 nodes per lane is best up to ~0.8M
+// nodes (7.2 TB/s at 262144, where 4 per lane leaves too few wavefronts),
+// 4 per lane from ~1M nodes (6.0 vs 5.6 TB/s at 1048576).
+constexpr int64_t kF32WideLd = 1 << 20;
 
 __device__ inline void kahan_add(float& s, float& c, float x) {
     const float y = x - c;
@@ -97,40 +100,47 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
     s = t;
 }
 
+template <int NPL>
 __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f32c(
     const float* __restrict__ hist, int64_t ld, int depth, int first, const float* __restrict__ w,
     int chunk, double* __restrict__ part) {
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    typedef float vt __attribute__((ext_vector_type(NPL)));
+    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * NPL;
     if (i >= ld) return;
     const int c = blockIdx.y;
     const int s0 = first + c * chunk;
     const int s1 = min(depth, s0 + chunk);
-    float sx = 0.f, cx = 0.f, sy = 0.f, cy = 0.f;
-    const int64_t ldv = ld / 2;
-    const frm_fv2* p = reinterpret_cast<const frm_fv2*>(hist + (int64_t)s0 * ld + i);
+    float sm[NPL], cm[NPL];
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) sm[q] = cm[q] = 0.f;
+    const int64_t ldv = ld / NPL;
+    const vt* p = reinterpret_cast<const vt*>(hist + (int64_t)s0 * ld + i);
     const float* wp = w + (depth - 1 - s0);
     int s = s0;
     for (; s + 8 <= s1; s += 8) {
-        frm_fv2 x[8];
+        vt x[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(p + (int64_t)u * ldv);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            kahan_add(sx, cx, wp[-u] * x[u].x);
-            kahan_add(sy, cy, wp[-u] * x[u].y);
+            const float wt = wp[-u];
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) kahan_add(sm[q], cm[q], wt * x[u][q]);
         }
         p += 8 * ldv;
         wp -= 8;
     }
     for (; s < s1; ++s) {
-        const frm_fv2 x = __builtin_nontemporal_load(p);
-        kahan_add(sx, cx, wp[0] * x.x);
-        kahan_add(sy, cy, wp[0] * x.y);
+        const vt x = __builtin_nontemporal_load(p);
+        const float wt = wp[0];
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) kahan_add(sm[q], cm[q], wt * x[q]);
         p += ldv;
         wp -= 1;
     }
-    part[(int64_t)c * ld + i] = (double)sx - (double)cx;
-    part[(int64_t)c * ld + i + 1] = (double)sy - (double)cy;
+    double* o = part + (int64_t)c * ld + i;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) o[q] = (double)sm[q] - (double)cm[q];
 }
 
 // out[t*n + i] = scale * sum_c part[(c*NT+t)*ld + i] (+ out)
@@ -223,8 +233,15 @@ int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w,
         LaunchTimer timer(ctx, FRM_K_MEMTERM,
                           (double)span * h->n_nodes * sizeof(float) +
                               (double)h->n_nodes * sizeof(double));
-        hipLaunchKernelGGL(k_memterm_f32c, grid, dim3(threads), 0, ctx->stream,
-                           (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+        if (h->ld >= kF32WideLd) {
+            // ld is a multiple of 4 * 256: the last workgroup may be half idle
+            dim3 g4((unsigned)((h->ld / 4 + threads - 1) / threads), (unsigned)nchunks);
+            hipLaunchKernelGGL(k_memterm_f32c<4>, g4, dim3(threads), 0, ctx->stream,
+                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+        } else {
+            hipLaunchKernelGGL(k_memterm_f32c<2>, grid, dim3(threads), 0, ctx->stream,
+                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+        }
         FRM_HIP(ctx, hipGetLastError());
         return FRM_OK;
     }
