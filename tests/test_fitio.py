@@ -1,0 +1,151 @@
+"""frm::read_fit_input and frm::write_res_rows (include/frm_fitio.hpp): the
+fitting input format of mt_minimal.cpp:1955-2043 and the result rows of
+1631-1635.  The reference ships no sample input, so the files here are
+written from the format; the expected values come from splitting the text
+into whitespace-separated tokens in the order the reference reads them.
+Host code only (g++), no GPU."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("fitio") / "fitio_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "cpp", "fitio_check.cpp"), "-o", out], check=True)
+    return out
+
+
+def run(checker, path, hmult, et="-", pv="-"):
+    r = subprocess.run([checker, str(path), repr(hmult), str(et), str(pv)], capture_output=True,
+                       text=True, check=True)
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("ok ")
+    if lines[0] == "ok 0":
+        return None, lines
+    fields = {}
+    evf = []
+    for ln in lines[1:]:
+        if ln.count(" ") >= 3 and ln[0].isdigit():
+            break  # result rows
+        name, *vals = ln.split(" ")
+        if name == "EV_F":
+            evf.append([float(v) for v in vals])
+        else:
+            fields[name] = [float(v) for v in vals]
+    fields["EV_F"] = evf
+    return fields, lines
+
+
+def sample(rng, nvalues=4, ninit=3, ncheck=2, nperc=2, nirr=1, nEV_T=3, nEV_F=2, nparam=5,
+           ev_inline=True, params_inline=True):
+    vals = {k: rng.uniform(-5, 5, n * c).round(6).reshape(n, c) for k, n, c in
+            [("values", nvalues, 3), ("init", ninit, 2), ("perc", nperc, 2), ("irr", nirr, 2),
+             ("check", ncheck, 3), ("ev", nEV_T, 2 + nEV_F)]}
+    params = rng.uniform(0, 1, nparam).round(8)
+    head = f"1 {nvalues} {ninit} {ncheck} {nperc} {nirr} {nEV_T} {nEV_F} 0.25\n"
+    head += "30 0.7 1.4 1.6 1e-6 200\n"
+    body = ""
+    for k in ("values", "init", "perc", "irr", "check"):
+        for r in vals[k]:
+            body += " ".join(repr(float(x)) for x in r) + "\n"
+    ev = "".join(" ".join(repr(float(x)) for x in r) + " \n" for r in vals["ev"])
+    pv = " ".join(repr(float(x)) for x in params) + "\n"
+    text = head + body + (ev if ev_inline else "") + (pv if params_inline else "")
+    return text, vals, params, ev, pv
+
+
+def check_fields(f, vals, params, hmult):
+    assert f["values_T"] == list(vals["values"][:, 0])
+    assert f["values_Z"] == list(vals["values"][:, 1])
+    assert f["values_F"] == list(vals["values"][:, 2] * hmult)
+    assert f["init_Z"] == list(vals["init"][:, 0])
+    assert f["init_F"] == list(vals["init"][:, 1] * hmult)
+    assert f["perc_T"] == list(vals["perc"][:, 0]) and f["perc_A"] == list(vals["perc"][:, 1])
+    assert f["irr_T"] == list(vals["irr"][:, 0]) and f["irr_A"] == list(vals["irr"][:, 1])
+    assert f["check_T"] == list(vals["check"][:, 0])
+    assert f["check_F"] == list(vals["check"][:, 2] * hmult)
+    assert f["EV_T"] == list(vals["ev"][:, 0]) and f["LAI"] == list(vals["ev"][:, 1])
+    assert f["EV_F"] == [list(vals["ev"][:, 2 + j]) for j in range(vals["ev"].shape[1] - 2)]
+    assert f["params"] == list(params[:20])
+
+
+def test_reads_the_format(checker, tmp_path):
+    rng = np.random.default_rng(1)
+    text, vals, params, _, _ = sample(rng)
+    p = tmp_path / "fit_input.txt"
+    p.write_text(text)
+    f, lines = run(checker, p, 10.0)
+    assert lines[1] == "counts 1 4 3 2 2 1 3 2"
+    assert f["ev_mu"] == [0.25] and f["pso"] == [30, 0.7, 1.4, 1.6, 1e-6, 200]
+    check_fields(f, vals, params, 10.0)
+
+
+def test_layout_is_whitespace_separated(checker, tmp_path):
+    # the reference reads with fscanf: line breaks and blank lines do not matter
+    rng = np.random.default_rng(2)
+    text, vals, params, _, _ = sample(rng)
+    toks = text.split()
+    odd = ""
+    for i, t in enumerate(toks):
+        odd += t + ("\n\n" if i % 5 == 4 else "  " if i % 3 else "\t")
+    p = tmp_path / "fit_input.txt"
+    p.write_text(odd)
+    f, _ = run(checker, p, 1.0)
+    check_fields(f, vals, params, 1.0)
+
+
+def test_separate_et_and_parameter_files(checker, tmp_path):
+    rng = np.random.default_rng(3)
+    text, vals, _, ev, _ = sample(rng, nparam=25, ev_inline=False, params_inline=False)
+    params = rng.uniform(0, 1, 25).round(8)
+    (tmp_path / "fit_input.txt").write_text(text)
+    (tmp_path / "et.txt").write_text(ev)
+    (tmp_path / "pv.txt").write_text(" ".join(repr(float(x)) for x in params))
+    f, _ = run(checker, tmp_path / "fit_input.txt", 2.0, tmp_path / "et.txt", tmp_path / "pv.txt")
+    check_fields(f, vals, params, 2.0)  # the first 20 parameters
+    assert len(f["params"]) == 20
+
+
+@pytest.mark.parametrize("nparam", [0, 1, 19, 20])
+def test_parameter_count(checker, tmp_path, nparam):
+    rng = np.random.default_rng(10 + nparam)
+    text, vals, params, _, _ = sample(rng, nparam=nparam)
+    p = tmp_path / "fit_input.txt"
+    p.write_text(text)
+    f, _ = run(checker, p, 1.0)
+    assert f["params"] == list(params)
+
+
+def test_malformed_files_are_rejected(checker, tmp_path):
+    rng = np.random.default_rng(4)
+    text, *_ = sample(rng, nparam=0)
+    toks = text.split()
+    # header, values, init, perc, irr, check, ev tokens: cut inside the check rows
+    cut = 15 + 4 * 3 + 3 * 2 + 2 * 2 + 1 * 2 + 3
+    p = tmp_path / "short.txt"
+    p.write_text(" ".join(toks[:cut]))
+    assert run(checker, p, 1.0)[0] is None
+    bad = list(toks)
+    bad[17] = "abc"  # a value to fit
+    p.write_text(" ".join(bad))
+    assert run(checker, p, 1.0)[0] is None
+    p.write_text("1 2 3\n")
+    assert run(checker, p, 1.0)[0] is None
+    assert run(checker, tmp_path / "missing.txt", 1.0)[0] is None
+
+
+def test_result_rows_format(checker, tmp_path):
+    rng = np.random.default_rng(5)
+    text, *_ = sample(rng)
+    p = tmp_path / "fit_input.txt"
+    p.write_text(text)
+    _, lines = run(checker, p, 1.0)
+    rows = lines[-3:]
+    U, w = [-1.5, -0.25, 0.0], [0.3, 0.35, 0.4]
+    assert rows == ["%g %g %g %g " % (1000.0, i * 0.05, U[i], w[i]) for i in range(3)]
