@@ -318,7 +318,9 @@ int frm_mt1d_fit_error(frm_mt1d* s, double* out);
  *              S[id] = _kb_row2_fixed_coefs(gtj, gtj^-alpha, Htmp[id], ceil(sum_param), alpha) / tau_n
  * S is 0 while fewer than two levels exist.  sum_alg 2 folds each increment
  * into the moments once: the first eval after a push folds it, later evals
- * before the next push only re-evaluate.  The reference's double / float /
+ * before the next push only re-evaluate; as in the reference, which folds
+ * only the newest difference per call, an increment pushed over without an
+ * eval never enters the moments.  The reference's double / float /
  * half kernel variants map to FRM_PREC_F64 and FRM_PREC_F32C (FP32 levels
  * differences, compensated sums; sum_alg 0 and 1). */
 typedef struct frm_timesum frm_timesum;
