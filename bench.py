@@ -542,11 +542,23 @@ def run_timesum(args):
     T = np.arange(K + 2) * tau
     taus = np.full(K + 1, tau)
     kbs = [Wl.caputo_interval_weights(T[:k + 1], T[k + 1], alpha) for k in range(K)]
+    # series form: the moments of x^j over the newest interval [T_{k-1}, T_k],
+    # over T_k^(j+1), as its coefficients (the caller's, prepared like kbs)
+    j = np.arange(cfg["series_terms"])
+    ser_coefs = [np.zeros(j.size)] + [(1.0 - (T[k - 1] / T[k]) ** (j + 1)) / (j + 1)
+                                      for k in range(1, K)]
     ctx = N.Context(0)
     pool_h = Wl.timesum3d_levels(n, cfg["pool"])
     pool = [N.DeviceArray.from_host(ctx, a) for a in pool_h]
 
     def run(precision, sum_alg=0, timing=False):
+        """Wall clock without per-launch events; with `timing`, a second,
+        event-timed run gives the kernel times."""
+        dt, _, out = run1(precision, sum_alg, False)
+        kt = run1(precision, sum_alg, True)[1] if timing else None
+        return dt, kt, out
+
+    def run1(precision, sum_alg, timing):
         ts = TimeSum(ctx, n, sum_alg=sum_alg, capacity=K + 1, precision=precision,
                      sum_param=cfg["series_terms"] if sum_alg == 2 else 0.0)
         # warm up the kernels once, then restart from an empty store
@@ -558,7 +570,6 @@ def run_timesum(args):
         else:
             ts.eval_device(kb_3_cache=kbs[1], tau=taus)
         ts.clear()
-        j = np.arange(cfg["series_terms"])
         if timing:
             ctx.timing_reset()
             ctx.timing_enable(1 << N.K_MEMTERM)
@@ -567,9 +578,8 @@ def run_timesum(args):
         for k in range(K):
             ts.push(pool[k % len(pool)])
             if sum_alg == 2:
-                T0, T1 = T[k - 1] if k else 0.0, T[k]
-                coefs = (1.0 - (T0 / T1) ** (j + 1)) / (j + 1) if k else j * 0.0
-                ts.eval_device(coefs=coefs, gtj=T[k + 1], gt=T1, alpha=alpha, tau_n=tau)
+                ts.eval_device(coefs=ser_coefs[k], gtj=T[k + 1], gt=T[k], alpha=alpha,
+                               tau_n=tau)
             else:
                 ts.eval_device(kb_3_cache=kbs[k], tau=taus)
         ctx.sync()
