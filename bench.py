@@ -510,6 +510,8 @@ def run_lod_forms(args, cfg_name):
             "plain_fp64": {"value": tot_nodes * K / t_plain, "seconds": t_plain,
                            "kernels_ms": kt_plain or None},
             "speedup_vs_plain_fp64": t_plain / out_forms[best]["seconds"]}
+    if not args.no_copy_probe:
+        line["measured_copy_GBs"] = ctx.copy_bandwidth()
     if args.cpu_steps > 0:
         # the oracle on a window at depth T/2 of the first set (the AB sets
         # with the reference's MF_int_choice weights)

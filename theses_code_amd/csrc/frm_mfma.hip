@@ -20,7 +20,6 @@
 namespace frm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kKc = 32;   // slots per MFMA k-chunk
@@ -38,10 +37,15 @@ static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block i
 // x1 y1 and takes each FP32 increment exactly (three 8-bit parts), leaving
 // ~2^-24 -- the FP32 accumulation's own level.
 //
-// The history chunk (32 slots x 128 nodes, FP32) is loaded as float4 (4
-// nodes of one slot per load, 4 slots per thread) with non-temporal loads
-// (the stream is read once; the weight table stays in L2), split into BF16
-// parts and transposed to node-major LDS rows [node][slot].  Two chunks are
+// The history chunk (32 slots x 128 nodes, FP32) is loaded with
+// non-temporal loads (the stream is read once; the weight table stays in
+// L2): thread (g = tid % 64, q = tid / 64) reads slots 8q..8q+7 of nodes g
+// and g + 64 (four bytes per lane, 256 contiguous bytes per wavefront load),
+// splits them into BF16 parts and stores each node's 8 slots as one 16-byte
+// LDS write into node-major rows [node][slot].  With 80-byte rows, the 64
+// nodes of one store fall on all 8 groups of 4 banks: the earlier layout
+// (float4 loads, 4 nodes x 4 slots per thread, 8-byte stores 4 rows apart)
+// left 75% of the LDS cycles to bank conflicts and ran 10% slower.  Two chunks are
 // in flight: the loop is unrolled by two over register buffers x (even
 // chunks) and y (odd chunks), each refilled right after its chunk's LDS
 // store, with the weight loads issued first so that waiting for them never
@@ -61,7 +65,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1
     float* __restrict__ F) {
     constexpr int NS = NP == 6 ? 3 : 2;    // BF16 parts per operand
     constexpr int CT = kNt / 16;           // column tiles per workgroup
-    constexpr int SPT = kKc * kNt / 4 / 256;  // slots per loader thread (4 nodes each)
+    static_assert(kNt == 128 && kKc == 32, "the loader map takes 2 x 64 nodes, 4 x 8 slots");
     __shared__ __attribute__((aligned(16))) __bf16 sB[NS][kNt][kBpad];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kg = lane >> 4, lc = lane & 15;
@@ -72,20 +76,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < CT; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // loader map: nodes 4*ng .. 4*ng+3 of slots SPT*sq .. SPT*sq + SPT-1 of
-    // the chunk.  N0 is a multiple of two chunk lengths (block_far checks
-    // it), so every chunk is whole and the loads need no guard: a guarded
-    // vector load compiles to a select between global and private addresses
-    // and a flat load, which serialises the prefetch.
-    constexpr int NG = kNt / 4;
-    const int ng = tid % NG, sq = tid / NG;
-    float4 x[SPT], y[SPT];
-    auto load = [&](float4 (&d)[SPT], int s0) {
-        const float* q = hist + (int64_t)(s0 + SPT * sq) * ld + i0 + 4 * ng;
+    // loader map: slots 8*sq .. 8*sq+7 of the chunk for nodes ng and ng + 64.
+    // N0 is a multiple of two chunk lengths (block_far checks it), so every
+    // chunk is whole and the loads need no guard: a guarded load compiles to
+    // a select between global and private addresses and a flat load, which
+    // serialises the prefetch.
+    const int ng = tid & 63, sq = tid >> 6;
+    float x[8][2], y[8][2];
+    auto load = [&](float (&d)[8][2], int s0) {
+        const float* q = hist + (int64_t)(s0 + 8 * sq) * ld + i0 + ng;
 #pragma unroll
-        for (int u = 0; u < SPT; ++u) {
-            const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(q + (int64_t)u * ld));
-            d[u] = make_float4(v.x, v.y, v.z, v.w);
+        for (int u = 0; u < 8; ++u) {
+            d[u][0] = __builtin_nontemporal_load(q + (int64_t)u * ld);
+            d[u][1] = __builtin_nontemporal_load(q + (int64_t)u * ld + 64);
         }
     };
     // an: fragment of row tile r0 for the chunk at s0 (d = N0 - s0); acar:
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1
     };
     // one chunk: BF16 parts of d into LDS, then (after the barrier) the next
     // loads, then the products
-    auto chunk = [&](float4 (&d)[SPT], int next_a, int next_h) {
+    auto chunk = [&](float (&d)[8][2], int next_a, int next_h) {
         bf16x8 ap[2][NS];
 #pragma unroll
         for (int p = 0; p < NS; ++p) {
@@ -107,20 +110,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1
             acar[p] = an[p];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float r[SPT];
+        for (int h = 0; h < 2; ++h) {
+            float r[8];
 #pragma unroll
-            for (int u = 0; u < SPT; ++u)
-                r[u] = q == 0 ? d[u].x : q == 1 ? d[u].y : q == 2 ? d[u].z : d[u].w;
+            for (int u = 0; u < 8; ++u) r[u] = d[u][h];
 #pragma unroll
             for (int p = 0; p < NS; ++p) {
-                bf16x4 v;
+                bf16x8 v;
 #pragma unroll
-                for (int u = 0; u < SPT; ++u) {
+                for (int u = 0; u < 8; ++u) {
                     v[u] = (__bf16)r[u];
                     r[u] -= (float)v[u];
                 }
-                *reinterpret_cast<bf16x4*>(&sB[p][4 * ng + q][SPT * sq]) = v;
+                *reinterpret_cast<bf16x8*>(&sB[p][ng + 64 * h][8 * sq]) = v;
             }
         }
         __syncthreads();
