@@ -30,8 +30,8 @@ extern "C" int frm_diag_copy(frm_ctx* ctx, void* d_dst, const void* d_src, size_
     FRM_HIP(ctx, hipEventSynchronize(b));
     float ms = 0.f;
     FRM_HIP(ctx, hipEventElapsedTime(&ms, a, b));
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     if (avg_ms) *avg_ms = ms / reps;
     return FRM_OK;
 }

@@ -189,7 +189,7 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
     }
     e = hipMalloc(&h->part, (size_t)h->max_chunks * 2 * h->ld * sizeof(double));
     if (e != hipSuccess) {
-        hipFree(h->data);
+        (void)hipFree(h->data);
         delete h;
         return check_hip(ctx, e, "hipMalloc(partials)");
     }
@@ -197,8 +197,8 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
     e = hipMemsetAsync(h->data, 0, bytes, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        hipFree(h->data);
-        hipFree(h->part);
+        (void)hipFree(h->data);
+        (void)hipFree(h->part);
         delete h;
         return check_hip(ctx, e, "hipMemset(history store)");
     }
@@ -284,9 +284,9 @@ int frm_hist_create(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t pre
 
 int frm_hist_destroy(frm_hist* h) {
     if (!h) return FRM_OK;
-    hipStreamSynchronize(h->ctx->stream);
-    hipFree(h->data);
-    hipFree(h->part);
+    (void)hipStreamSynchronize(h->ctx->stream);
+    (void)hipFree(h->data);
+    (void)hipFree(h->part);
     delete h;
     return FRM_OK;
 }

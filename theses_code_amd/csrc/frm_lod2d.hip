@@ -630,30 +630,30 @@ static int tune_sweeps(frm_lod2d* s, int force) {
             if (e != hipSuccess) rc = check_hip(ctx, e, "sweep coefficients");
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
 static void lod_free(frm_lod2d* s) {
     if (!s) return;
     if (s->hist) frm_hist_destroy(s->hist);
-    hipFree(s->C);
-    hipFree(s->Cprev);
-    hipFree(s->w);
-    hipFree(s->rcoef);
-    hipFree(s->ccoef);
-    hipFree(s->soe_H);
-    hipFree(s->soe_q);
-    hipFree(s->soe_c);
-    hipFree(s->bin_H);
-    hipFree(s->bin_upd);
-    hipFree(s->bin_er);
-    hipFree(s->bin_ec);
-    hipFree(s->bin_cnt);
-    hipFree(s->atab);
-    hipFree(s->F);
-    hipFree(s->G);
+    (void)hipFree(s->C);
+    (void)hipFree(s->Cprev);
+    (void)hipFree(s->w);
+    (void)hipFree(s->rcoef);
+    (void)hipFree(s->ccoef);
+    (void)hipFree(s->soe_H);
+    (void)hipFree(s->soe_q);
+    (void)hipFree(s->soe_c);
+    (void)hipFree(s->bin_H);
+    (void)hipFree(s->bin_upd);
+    (void)hipFree(s->bin_er);
+    (void)hipFree(s->bin_ec);
+    (void)hipFree(s->bin_cnt);
+    (void)hipFree(s->atab);
+    (void)hipFree(s->F);
+    (void)hipFree(s->G);
     delete s;
 }
 
@@ -918,7 +918,7 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
 
 int frm_lod2d_destroy(frm_lod2d* s) {
     if (!s) return FRM_OK;
-    hipStreamSynchronize(s->ctx->stream);
+    (void)hipStreamSynchronize(s->ctx->stream);
     lod_free(s);
     return FRM_OK;
 }

@@ -833,29 +833,29 @@ struct frm_mt1d {
 
 static void mt_free(frm_mt1d* s) {
     if (!s) return;
-    hipFree(s->probs);
-    hipFree(s->gprob);
-    hipFree(s->U);
-    hipFree(s->hist);
-    hipFree(s->part);
-    hipFree(s->w);
-    hipFree(s->w32);
-    hipFree(s->d0);
-    hipFree(s->bad);
-    hipFree(s->ab_cf);
-    hipFree(s->ab_cal);
-    hipFree(s->ab_ml);
-    hipFree(s->ab_err);
-    hipFree(s->gprob_mt);
-    hipFree(s->fit_obs);
-    hipFree(s->fit_idx);
-    hipFree(s->fit_vs);
-    hipFree(s->fit_err);
-    hipFree(s->fit_dead);
-    hipFree(s->ser_tab);
-    hipFree(s->ser_choice);
-    hipFree(s->ser_lc);
-    hipFree(s->ser_flag);
+    (void)hipFree(s->probs);
+    (void)hipFree(s->gprob);
+    (void)hipFree(s->U);
+    (void)hipFree(s->hist);
+    (void)hipFree(s->part);
+    (void)hipFree(s->w);
+    (void)hipFree(s->w32);
+    (void)hipFree(s->d0);
+    (void)hipFree(s->bad);
+    (void)hipFree(s->ab_cf);
+    (void)hipFree(s->ab_cal);
+    (void)hipFree(s->ab_ml);
+    (void)hipFree(s->ab_err);
+    (void)hipFree(s->gprob_mt);
+    (void)hipFree(s->fit_obs);
+    (void)hipFree(s->fit_idx);
+    (void)hipFree(s->fit_vs);
+    (void)hipFree(s->fit_err);
+    (void)hipFree(s->fit_dead);
+    (void)hipFree(s->ser_tab);
+    (void)hipFree(s->ser_choice);
+    (void)hipFree(s->ser_lc);
+    (void)hipFree(s->ser_flag);
     delete s;
 }
 
@@ -1135,7 +1135,7 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
 
 int frm_mt1d_destroy(frm_mt1d* s) {
     if (!s) return FRM_OK;
-    hipStreamSynchronize(s->ctx->stream);
+    (void)hipStreamSynchronize(s->ctx->stream);
     mt_free(s);
     return FRM_OK;
 }
@@ -1356,9 +1356,9 @@ int frm_mt1d_set_observations(frm_mt1d* s, int32_t nobs, const double* T, const 
     frm_ctx* ctx = s->ctx;
     FRM_REQUIRE(ctx, nobs >= 0 && (nobs == 0 || (T && Z && F)),
                 "frm_mt1d_set_observations: bad arguments");
-    hipFree(s->fit_obs);
-    hipFree(s->fit_idx);
-    hipFree(s->fit_vs);
+    (void)hipFree(s->fit_obs);
+    (void)hipFree(s->fit_idx);
+    (void)hipFree(s->fit_vs);
     s->fit_obs = s->fit_vs = nullptr;
     s->fit_idx = nullptr;
     s->nobs = 0;

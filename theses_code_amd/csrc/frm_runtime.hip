@@ -115,11 +115,11 @@ int frm_ctx_create(int device, frm_ctx** out) {
 
 int frm_ctx_destroy(frm_ctx* ctx) {
     if (!ctx) return FRM_OK;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     frm::harvest(ctx);
-    for (auto ev : ctx->event_pool) hipEventDestroy(ev);
-    hipStreamDestroy(ctx->stream);
+    for (auto ev : ctx->event_pool) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return FRM_OK;
 }
