@@ -125,4 +125,73 @@ class H_batch {
     frm_mt1d* s_ = nullptr;
 };
 
+// The time sums of the 3D irrigation solver (irrigation_fractional.cpp):
+// what H_solver::time_summing returns per node (4099-4170) and
+// fr_ocl_call_sum computes on the GPU (1828-1905).  The levels stay on the
+// device: push() takes each new level as oldH.push_back does (a host array
+// of n_nodes values here; frm_timesum_push takes device pointers), sums()
+// fills Sums[] for every node from the weights the caller's precalc_kb_3
+// produced:
+//   TimeSum ts(ctx, nodes, sum_alg, sum_param, max_steps + 1);
+//   ts.push(level0);  ...  ts.push(level);  ts.sums(kb_3_cache, tau, Sums);
+class TimeSum {
+   public:
+    TimeSum(Context& ctx, int64_t n_nodes, int sum_alg, double sum_param, int capacity,
+            int precision = FRM_PREC_F64)
+        : ctx_(ctx), n_(n_nodes) {
+        frm_timesum_desc d{n_nodes, capacity, sum_alg, sum_param, precision};
+        check(frm_timesum_create(ctx.get(), &d, &s_), ctx.get());
+        check(frm_malloc(ctx.get(), (size_t)n_nodes * sizeof(double), &lev_), ctx.get());
+        check(frm_malloc(ctx.get(), (size_t)n_nodes * sizeof(double), &out_), ctx.get());
+    }
+    ~TimeSum() {
+        frm_timesum_destroy(s_);
+        frm_free(ctx_.get(), lev_);
+        frm_free(ctx_.get(), out_);
+    }
+    TimeSum(const TimeSum&) = delete;
+    TimeSum& operator=(const TimeSum&) = delete;
+
+    void push(const double* level) {
+        check(frm_upload(ctx_.get(), lev_, level, (size_t)n_ * sizeof(double)), ctx_.get());
+        check(frm_timesum_push(s_, (const double*)lev_), ctx_.get());
+    }
+    int old_size() const {
+        int32_t v = 0;
+        check(frm_timesum_levels(s_, &v), ctx_.get());
+        return v;
+    }
+    // sum_alg 0 and 1: kb_3_cache[t] and tau[t] for t < old_size - 1
+    void sums(const double* kb_3_cache, const double* tau, double* Sums) {
+        frm_timesum_args a{};
+        a.kb_3_cache = kb_3_cache;
+        a.tau = tau;
+        eval(a, Sums);
+    }
+    // sum_alg 2: row2_precalc[time_i1][0], g(Time[tstep+1]), g(Time[tstep]),
+    // alpha and tau[tstep]
+    void sums_series(const std::vector<double>& coefs, double gtj, double gt, double alpha,
+                     double tau_n, double* Sums) {
+        frm_timesum_args a{};
+        a.coefs = coefs.data();
+        a.ncoefs = (int32_t)coefs.size();
+        a.gtj = gtj;
+        a.gt = gt;
+        a.alpha = alpha;
+        a.tau_n = tau_n;
+        eval(a, Sums);
+    }
+
+   private:
+    void eval(const frm_timesum_args& a, double* Sums) {
+        check(frm_timesum_eval(s_, &a, (double*)out_), ctx_.get());
+        check(frm_download(ctx_.get(), Sums, out_, (size_t)n_ * sizeof(double)), ctx_.get());
+    }
+    Context& ctx_;
+    int64_t n_;
+    frm_timesum* s_ = nullptr;
+    void* lev_ = nullptr;
+    void* out_ = nullptr;
+};
+
 }  // namespace frm

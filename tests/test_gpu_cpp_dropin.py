@@ -63,3 +63,34 @@ def test_cpp_pso_batch_scores_particles():
                        top=O.TOP_DIRICHLET, top_val=U0[0], bottom_cond=1, Hbottom=U0[N])
         want = O.mt_fit_error(o, N, 1.0, tau, steps, T, Z, F)
         assert got[p] == pytest.approx(want, rel=1e-11), p
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sum_alg,eps", [(0, 0.0), (1, 0.3), (2, 9.0)])
+def test_cpp_timesum_matches_oracle(sum_alg, eps):
+    # the C++ frm::TimeSum as the 3D irrigation solver would call it
+    binp = os.path.join(REPO, "tests", "cpp", "timesum_dropin")
+    assert os.path.exists(binp), "build() compiles tests/cpp/timesum_dropin"
+    rng = np.random.default_rng(30 + sum_alg)
+    n, nlev = 5000, 40
+    lev = np.cumsum(rng.standard_normal((nlev, n)) * 1e-2, axis=0) + rng.uniform(-1, 1, n)
+    kb = np.linspace(0.05, 1.0, nlev - 1)
+    tau = rng.uniform(0.5, 1.5, nlev - 1)
+    coefs = rng.uniform(0.1, 1.0, 7) / np.arange(1, 8)
+    gtj, gt, alpha, tau_n = 2.5, 2.0, 0.6, 0.7
+    head = np.array([n, nlev, sum_alg, eps, coefs.size, gtj, gt, alpha, tau_n], float)
+    with tempfile.TemporaryDirectory() as d:
+        fi, fo = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        np.concatenate([head, lev.reshape(-1), kb, tau, coefs]).tofile(fi)
+        r = subprocess.run([binp, fi, fo], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert f"old_size {nlev} nodes {n}" in r.stdout
+        got = np.fromfile(fo)
+    if sum_alg == 2:
+        o = O.TimeSum2(n, eps)
+        for k in range(1, nlev):
+            want = o(lev[k] - lev[k - 1], coefs, gtj, gt, alpha, tau_n)
+        assert np.array_equal(got, want)
+    else:
+        want = O.timesum01(sum_alg, eps, lev, kb, tau)
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))

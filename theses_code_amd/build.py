@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> str:
 def build_cpp_example() -> list:
     """Host-only C++ programs that use include/frm_solvers.hpp (tests/cpp)."""
     outs = []
-    for name in ("tg_dropin", "pso_batch"):
+    for name in ("tg_dropin", "pso_batch", "timesum_dropin"):
         src = os.path.join(REPO, "tests", "cpp", name + ".cpp")
         out = os.path.join(REPO, "tests", "cpp", name)
         cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), src, "-o", out,
