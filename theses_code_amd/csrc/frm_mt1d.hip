@@ -86,21 +86,26 @@ __device__ inline double mt_inv(const MtProbDev& P, double x) {
 // ---------------------------------------------------------------- memory term
 // part[c*ldh + j] = sum over slots of chunk c of w[prob(j)*ldw + depth-1-s] * Delta_s[j]
 // One wavefront = one 128-node group = one problem (gprob), so the weight
-// address is wavefront-uniform.
+// address is wavefront-uniform.  gvalid holds each group's count of real
+// nodes: the lanes past it (the padding of a problem's last group) issue no
+// loads, so the stream carries only the problems' own nodes (with 128-node
+// padding it carried 12% more on the 4096 uneven problems of the psi batch).
 typedef double mt_dv2 __attribute__((ext_vector_type(2)));
 
 __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f64(
     const double* __restrict__ hist, int64_t ldh, int depth, const double* __restrict__ w,
-    int ldw, const int* __restrict__ gprob, int chunk, double* __restrict__ part) {
+    int ldw, const int* __restrict__ gprob, const int* __restrict__ gvalid, int chunk,
+    double* __restrict__ part) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t j = t * 2;
     if (j >= ldh) return;
     const int p = gprob[t >> 6];
+    const bool live = p >= 0 && (int)(j & (kGroup - 1)) < gvalid[t >> 6];
     const int c = blockIdx.y;
     const int s0 = c * chunk;
     const int s1 = min(depth, s0 + chunk);
     double a0 = 0.0, a1 = 0.0;
-    if (p >= 0) {
+    if (live) {
         const int64_t ldv = ldh / 2;
         const mt_dv2* q = reinterpret_cast<const mt_dv2*>(hist + (int64_t)s0 * ldh + j);
         const double* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
@@ -142,16 +147,18 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
 
 __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f32c(
     const float* __restrict__ hist, int64_t ldh, int depth, const float* __restrict__ w, int ldw,
-    const int* __restrict__ gprob, int chunk, double* __restrict__ part) {
+    const int* __restrict__ gprob, const int* __restrict__ gvalid, int chunk,
+    double* __restrict__ part) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t j = t * 2;
     if (j >= ldh) return;
     const int p = gprob[t >> 6];
+    const bool live = p >= 0 && (int)(j & (kGroup - 1)) < gvalid[t >> 6];
     const int c = blockIdx.y;
     const int s0 = c * chunk;
     const int s1 = min(depth, s0 + chunk);
     float s0x = 0.f, c0x = 0.f, s1x = 0.f, c1x = 0.f;
-    if (p >= 0) {
+    if (live) {
         const int64_t ldv = ldh / 2;
         const mt_fv2* q = reinterpret_cast<const mt_fv2*>(hist + (int64_t)s0 * ldh + j);
         const float* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
@@ -798,6 +805,8 @@ struct frm_mt1d {
     std::vector<MtProbDev> hprobs;
     MtProbDev* probs = nullptr;
     int* gprob = nullptr;
+    int* gvalid = nullptr;    // real nodes per group (0 for padding groups)
+    int64_t hnodes = 0, hnodes_mt = 0;  // real history nodes (all; full-history problems)
     double* U = nullptr;
     void* hist = nullptr;     // slots * ldh (FP64 or FP32)
     double* part = nullptr;   // kMaxChunks * ldh
@@ -835,6 +844,7 @@ static void mt_free(frm_mt1d* s) {
     if (!s) return;
     (void)hipFree(s->probs);
     (void)hipFree(s->gprob);
+    (void)hipFree(s->gvalid);
     (void)hipFree(s->U);
     (void)hipFree(s->hist);
     (void)hipFree(s->part);
@@ -1028,7 +1038,7 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     const bool is_ab = d->kind == FRM_MT_AB;
     s->slots = is_ab ? d->capacity : std::max(1, d->capacity - 2);
     s->ldw = s->slots;
-    std::vector<int> gp;
+    std::vector<int> gp, gv;
     int64_t uoff = 0, hoff = 0;
     for (int p = 0; p < d->nprob; ++p) {
         const frm_mt1d_problem& q = d->probs[p];
@@ -1070,7 +1080,12 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         }
         s->hprobs.push_back(P);
         const int ng = (q.N - 1 + kGroup - 1) / kGroup;
-        for (int g = 0; g < ng; ++g) gp.push_back(p);
+        for (int g = 0; g < ng; ++g) {
+            gp.push_back(p);
+            gv.push_back(std::min(kGroup, q.N - 1 - g * kGroup));
+        }
+        s->hnodes += q.N - 1;
+        if (!ser) s->hnodes_mt += q.N - 1;
         hoff += (int64_t)ng * kGroup;
         uoff += q.N + 2;
         s->maxN = std::max(s->maxN, q.N);
@@ -1080,10 +1095,14 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     const int64_t wg_nodes = 2 * kMemtermThreads;
     s->ldh = (hoff + wg_nodes - 1) / wg_nodes * wg_nodes;
     s->ngroups = s->ldh / kGroup;
-    while ((int64_t)gp.size() < s->ngroups) gp.push_back(-1);
+    while ((int64_t)gp.size() < s->ngroups) {
+        gp.push_back(-1);
+        gv.push_back(0);
+    }
     const size_t esz = d->precision == FRM_PREC_F32C ? sizeof(float) : sizeof(double);
     hipError_t e = hipMalloc(&s->probs, sizeof(MtProbDev) * d->nprob);
     if (e == hipSuccess) e = hipMalloc(&s->gprob, sizeof(int) * gp.size());
+    if (e == hipSuccess) e = hipMalloc(&s->gvalid, sizeof(int) * gv.size());
     if (e == hipSuccess) e = hipMalloc(&s->U, sizeof(double) * s->nU);
     if (e == hipSuccess) e = hipMalloc(&s->hist, esz * (size_t)s->slots * s->ldh);
     if (e == hipSuccess) e = hipMalloc(&s->part, sizeof(double) * (size_t)kMaxChunks * s->ldh);
@@ -1112,6 +1131,7 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     }
     e = hipMemcpy(s->probs, s->hprobs.data(), sizeof(MtProbDev) * d->nprob, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->gprob, gp.data(), sizeof(int) * gp.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->gvalid, gv.data(), sizeof(int) * gv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->w, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !w32.empty())
         e = hipMemcpy(s->w32, w32.data(), sizeof(float) * w32.size(), hipMemcpyHostToDevice);
@@ -1227,15 +1247,17 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             nch = std::max(1, std::min<int>({nch, (depth + kMinChunk - 1) / kMinChunk, kMaxChunks}));
             const int chunk = (depth + nch - 1) / nch;
             dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
-            LaunchTimer t(ctx, FRM_K_MEMTERM, (double)depth * s->ldh * esz);
+            // algorithmic bytes: the problems' own nodes (padding lanes load nothing)
+            LaunchTimer t(ctx, FRM_K_MEMTERM,
+                          (double)depth * (s->gprob_mt ? s->hnodes_mt : s->hnodes) * esz);
             if (f32)
                 hipLaunchKernelGGL(k_mt_memterm_f32c, grid, dim3(kMemtermThreads), 0, ctx->stream,
                                    (const float*)s->hist, s->ldh, depth, s->w32, s->ldw, s->gprob,
-                                   chunk, s->part);
+                                   s->gvalid, chunk, s->part);
             else
                 hipLaunchKernelGGL(k_mt_memterm_f64, grid, dim3(kMemtermThreads), 0, ctx->stream,
                                    (const double*)s->hist, s->ldh, depth, s->w, s->ldw,
-                                   s->gprob_mt ? s->gprob_mt : s->gprob, chunk, s->part);
+                                   s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid, chunk, s->part);
             FRM_HIP(ctx, hipGetLastError());
         }
         if (s->kind == FRM_MT_AB && s->ser_count > 0 && k >= 2) {
