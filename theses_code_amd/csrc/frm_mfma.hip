@@ -56,11 +56,11 @@ static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block i
 // Row tile rt + 2 of target t at lag base d equals row tile rt at d + 32
 // (rows j and j + 32, same k), i.e. the fragment rt held one chunk earlier;
 // so wavefront w owns row tiles r0 and r0 + 2 (r0 = 0, 1, 4, 5) and loads
-// one fragment per chunk, keeping the previous one.  The two-part form is
-// held to 168 registers (three workgroups per CU); the three-part form
-// would spill under that limit.
+// one fragment per chunk, keeping the previous one.  Both forms are held to
+// 168 registers, three workgroups per CU (150 and 161 registers without
+// spills since the loader stores 16-byte pieces).
 template <int NP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 6 ? 1 : 3, 8))) void k_block_far(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_block_far(
     const float* __restrict__ hist, int64_t ld, int N0, const bf16x8* __restrict__ atab,
     float* __restrict__ F) {
     constexpr int NS = NP == 6 ? 3 : 2;    // BF16 parts per operand
