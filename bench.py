@@ -596,6 +596,14 @@ def run_timesum(args):
 
     dt, kt, S64 = run(N.PREC_F64, timing=True)
     launches, ms, nbytes = kt
+    # HBM bytes per launch: the PMC-measured ratio of counted to algorithmic
+    # bytes of this kernel (profiles/memterm_timesum3d_traffic.json) times
+    # this run's algorithmic bytes per launch
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "memterm_timesum3d_traffic.json")
+    if os.path.exists(tf) and launches:
+        with open(tf) as f:
+            traffic = json.load(f)["traffic_over_algorithmic"] * nbytes / launches
     value = n * K / dt
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms else 0.0
     # parity of the last step against the oracle on a node sample, all levels
@@ -644,7 +652,7 @@ def run_timesum(args):
             "config": {"workload": cfg["name"], "nodes": n, "tau": tau, "alpha": alpha,
                        "sum_alg": 0, "parallelism": "1 GPU"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                          "kernel": "k_memterm_f64<1>", "launches": launches,
                          "avg_launch_ms": ms / launches if launches else None},
             "parity": {"vs": "oracle time_summing restatement, last step, 4096-node sample",
