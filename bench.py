@@ -191,10 +191,16 @@ def run_mt1d(args, cfg_name):
                        "tau": cfg["tau"], "picard": picard, "parallelism": "1 GPU"},
             "kernels_ms": {k: v[1] for k, v in kt.items()}}
     ms, nb = kt["memterm"][1], kt["memterm"][2]
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "mt1d_psi_traffic.json")
+    if cfg_name == "c3" and os.path.exists(tf) and kt["memterm"][0]:
+        # PMC-measured ratio of HBM to algorithmic bytes of this kernel
+        with open(tf) as f:
+            traffic = json.load(f)["traffic_over_algorithmic"] * nb / kt["memterm"][0]
     line["roofline"] = {"bound": "hbm", "achieved": nb / (ms * 1e-3) / 1e9 if ms else 0.0,
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
-                        "traffic": None, "kernel": "k_mt_memterm_f64"}
+                        "traffic": traffic, "kernel": "k_mt_memterm_f64"}
     if cfg_name == "c3" and args.cpu_steps > 0:
         # the oracle (H_solver restatement with the psi weights rebuilt every
         # step, as the reference does) on whole runs of the first problems,
