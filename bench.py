@@ -55,7 +55,9 @@ def parse():
     ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
     ap.add_argument("--plain-sample", type=int, default=64,
                     help="c3: problems timed through the one-solver-per-problem plain path")
-    ap.add_argument("--soe-f32", action="store_true", help="c2/c4: FP32 mode states")
+    ap.add_argument("--soe-f32", action="store_true", help="c2: FP32 mode states")
+    ap.add_argument("--soe-f64", action="store_true",
+                    help="c4: FP64 mode states (the configuration's own are FP32)")
     ap.add_argument("--sweep-w", type=int, default=0,
                     help="wavefronts per sweep line for the chosen forms (0: autotuned)")
     ap.add_argument("--breakdown-plain", action="store_true",
@@ -407,7 +409,7 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
     return dt, out, kt, fit, work
 
 
-FORMS = {"soe": "sum-of-exponentials history (FP64 mode states)",
+FORMS = {"soe": "sum-of-exponentials history (FP64 mode states unless stated)",
          "mfma": "blocked Toeplitz on MFMA: FP32 increments, split hi+lo BF16, FP32 acc, "
                  "64-step blocks, FP64 near part",
          "binomial": "the reference's single-series history (TG_solver dt_alg 3, 25 terms, "
@@ -436,7 +438,10 @@ def run_lod_forms(args, cfg_name):
         kw = dict(kernel=N.KERNEL_AB, ab_exact=True)
     K = args.steps if args.steps != 5000 else cfg["steps"]
     forms = args.forms.split(",")
-    fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=args.soe_f32,
+    # the AB sets run their exponential-series recurrences in FP32, as the
+    # configuration states; the 1024^2 Caputo run keeps FP64 states unless asked
+    soe_f32 = args.soe_f32 or (cfg_name == "c4" and not args.soe_f64)
+    fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=soe_f32,
                        sweep_w=args.sweep_w),
            "mfma": dict(precision=N.PREC_MFMA_BF16, sweep_w=args.sweep_w),
            "binomial": dict(precision=N.PREC_BINOMIAL, series_terms=BINOMIAL_TERMS,
@@ -488,8 +493,10 @@ def run_lod_forms(args, cfg_name):
              "max_rel_err_vs_fp64": max(q["errs"]), "rms_rel_err_vs_fp64": max(q["rms"]),
              "sweep_shapes_rows_cols": q["sweep_shapes"]}
         if f == "soe":
+            if soe_f32:
+                o["form"] = "sum-of-exponentials history (FP32 mode states)"
             o["terms"] = cfg["soe_terms"]
-            o["states"] = "fp32" if args.soe_f32 else "fp64"
+            o["states"] = "fp32" if soe_f32 else "fp64"
             o["max_fit_rel_err"] = max(q["fits"])
         if f == "mfma" and q["kernels_ms"].get("blockfar"):
             # useful flops of the far-part products; the split products run
