@@ -62,6 +62,8 @@ def parse():
                     help="wavefronts per sweep line for the chosen forms (0: autotuned)")
     ap.add_argument("--breakdown-plain", action="store_true",
                     help="c2/c4: also time plain FP64 per kernel (a second full run)")
+    ap.add_argument("--graph", action="store_true",
+                    help="c0/c3: also time the FP64 run with its steps captured in one HIP graph")
     ap.add_argument("--forms", default="soe,mfma",
                     help="comma-separated forms timed against plain FP64 on the same workload "
                          "(soe, mfma; binomial = the reference's own series history, c1/c2; "
@@ -128,8 +130,9 @@ def run_mt1d(args, cfg_name):
     ctx = N.Context(0)
     results = {}
 
-    def one_run(prec, timing):
-        b = MT1DBatch(ctx, probs, cfg["tau"], K, U0, kind=kind, precision=prec, picard=picard)
+    def one_run(prec, timing, graph=False):
+        b = MT1DBatch(ctx, probs, cfg["tau"], K, U0, kind=kind, precision=prec, picard=picard,
+                      graph=graph)
         if timing:
             ctx.timing_reset()
             ctx.timing_enable((1 << N.K_MEMTERM) | (1 << N.K_OTHER) | (1 << N.K_SWEEP1D))
@@ -159,6 +162,13 @@ def run_mt1d(args, cfg_name):
             kt = one_run(prec, True)[2]
         results[prec] = (dt, U, kt)
     dt, U64, kt = results[N.PREC_F64]
+    graph = None
+    if args.graph:
+        # the same FP64 run with all K steps recorded into one HIP graph and
+        # launched once (capture and instantiation inside the timed region)
+        dtg, Ug, _ = one_run(N.PREC_F64, False, graph=True)
+        graph = {"value": nodes * K / dtg, "ms_per_step": dtg * 1e3 / K,
+                 "identical_to_eager": bool(np.array_equal(Ug, U64))}
     per_problem = None
     if cfg_name == "c3" and args.plain_sample > 0:
         # The plain FP64 drop-in as the reference's callers use it: one solver
@@ -203,6 +213,8 @@ def run_mt1d(args, cfg_name):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
                         "traffic": traffic, "kernel": "k_mt_memterm_f64"}
+    if graph:
+        line["graph_capture"] = graph
     if cfg_name == "c3" and args.cpu_steps > 0:
         # the oracle (H_solver restatement with the psi weights rebuilt every
         # step, as the reference does) on whole runs of the first problems,

@@ -248,6 +248,9 @@ typedef struct frm_mt1d_desc {
     int32_t picard;    /* Picard iterations per step, >= 1 */
     double tau;
     const frm_mt1d_problem* probs; /* host array of nprob */
+    int32_t graph;     /* 1: a frm_mt1d_step call of several steps records them
+                          into one HIP graph and launches it once (not while
+                          per-launch timing events are on) */
 } frm_mt1d_desc;
 
 int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* desc, frm_mt1d** out);

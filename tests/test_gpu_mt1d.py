@@ -130,3 +130,42 @@ def test_mt1d_capacity_error(ctx):
     with pytest.raises(N.FrmError):
         b.calc_step(1)
     b.close()
+
+
+def _ab_batch():
+    probs = [dict(N=n, alpha=a, src=s, ab_maxl=50, ab_maxkiter=1000, ab_eps=1e-10, ab_series=ser)
+             for n, a, s, ser in ((40, 0.6, 0, 0), (97, 0.3, 1, 1), (64, 0.85, 2, 0),
+                                  (130, 0.5, 1, 1))]
+    return probs, [np.zeros(p["N"] + 2) for p in probs]
+
+
+@pytest.mark.parametrize("kind,picard", [(N.MT_L1, 1), (N.MT_PSI, 2), (N.MT_AB, 1)])
+def test_mt1d_graph_matches_eager(ctx, kind, picard):
+    # calls of several steps recorded into one HIP graph (graph=True) give the
+    # eager launches' results exactly; state reads (synchronisations) and
+    # single-step eager calls sit between the graphs
+    if kind == N.MT_AB:
+        probs, U0 = _ab_batch()
+        tau = 1.0 / 300
+    else:
+        probs, U0 = _uneven_batch(kind, 300 + kind)
+        tau = 2e-3
+    e = S.MT1DBatch(ctx, probs, tau, 60, U0, kind=kind, picard=picard)
+    g = S.MT1DBatch(ctx, probs, tau, 60, U0, kind=kind, picard=picard, graph=True)
+    if kind == N.MT_L1:
+        T = np.array([0.005, 0.03, 0.07, 0.11])
+        Z = np.array([0.3, 0.5, 0.2, 0.8])
+        F = np.array([0.2, 0.25, 0.3, 0.1])
+        e.set_observations(T, Z, F)
+        g.set_observations(T, Z, F)
+    for chunk in (7, 1, 20, 12, 20):
+        e.calc_step(chunk)
+        g.calc_step(chunk)
+        for a, b in zip(e.state(), g.state()):
+            assert np.array_equal(a, b), chunk
+    if kind == N.MT_L1:
+        assert np.array_equal(e.fit_error(), g.fit_error())
+    if kind == N.MT_AB:
+        assert np.array_equal(e.ab_error(), g.ab_error())
+    e.close()
+    g.close()

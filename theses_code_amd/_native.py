@@ -58,7 +58,7 @@ class MT1DProblem(ct.Structure):
 class MT1DDesc(ct.Structure):
     _fields_ = [("nprob", ct.c_int32), ("kind", ct.c_int32), ("capacity", ct.c_int32),
                 ("precision", ct.c_int32), ("picard", ct.c_int32), ("tau", _D),
-                ("probs", ct.POINTER(MT1DProblem))]
+                ("probs", ct.POINTER(MT1DProblem)), ("graph", ct.c_int32)]
 
 
 class TimesumDesc(ct.Structure):

@@ -811,6 +811,8 @@ struct frm_mt1d {
     MtProbDev* probs = nullptr;
     int* gprob = nullptr;
     int* gvalid = nullptr;    // real nodes per group (0 for padding groups)
+    int graph = 0;                  // capture multi-step calls in a HIP graph
+    hipGraphExec_t gexec = nullptr; // the last captured sequence (launched)
     int64_t hnodes = 0, hnodes_mt = 0;  // real history nodes (all; full-history problems)
     double* U = nullptr;
     void* hist = nullptr;     // slots * ldh (FP64 or FP32)
@@ -850,6 +852,7 @@ static void mt_free(frm_mt1d* s) {
     (void)hipFree(s->probs);
     (void)hipFree(s->gprob);
     (void)hipFree(s->gvalid);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     (void)hipFree(s->U);
     (void)hipFree(s->hist);
     (void)hipFree(s->part);
@@ -1039,6 +1042,7 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     s->precision = d->precision;
     s->picard = d->picard;
     s->tau = d->tau;
+    s->graph = d->graph != 0;
     // L1/psi: increments enter from step 3 on; AB: every increment
     const bool is_ab = d->kind == FRM_MT_AB;
     s->slots = is_ab ? d->capacity : std::max(1, d->capacity - 2);
@@ -1200,7 +1204,8 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
                          s->tstep - 1 + nsteps, s->capacity);
     const bool f32 = s->precision == FRM_PREC_F32C;
     const size_t esz = f32 ? sizeof(float) : sizeof(double);
-    for (int it = 0; it < nsteps; ++it) {
+    // one step: the launches of step s->tstep, queued on the context's stream
+    auto step_once = [&]() -> int {
         const int k = s->tstep;
         const int depth = s->depth;  // = max(0, k - 3)
         FitArgs fa{};
@@ -1314,7 +1319,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             s->depth = depth + 1;
             s->tstep = k + 1;
             if (int rc = fit_after()) return rc;
-            continue;
+            return FRM_OK;
         }
         MtSweepArgs a;
         a.pr = s->probs;
@@ -1340,6 +1345,50 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         if (k >= 3) s->depth = depth + 1;
         s->tstep = k + 1;
         if (int rc = fit_after()) return rc;
+        return FRM_OK;
+    };
+    // With graph capture the whole sequence of steps is recorded into one
+    // HIP graph (every step's arguments as it is queued) and launched once;
+    // not while per-launch timing events are on.
+    const bool cap = s->graph && nsteps > 1 && ctx->timing_mask == 0;
+    const int tstep0 = s->tstep, depth0 = s->depth;
+    const double tt0 = s->tt;
+    // a capture that fails queues nothing: the solver stays at its step
+    auto uncapture = [&]() {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        s->tstep = tstep0;
+        s->depth = depth0;
+        s->tt = tt0;
+    };
+    if (cap) {
+        if (s->gexec) {  // the previous call's graph has been launched
+            FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            (void)hipGraphExecDestroy(s->gexec);
+            s->gexec = nullptr;
+        }
+        FRM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    }
+    for (int it = 0; it < nsteps; ++it) {
+        if (int rc = step_once()) {
+            if (cap) uncapture();
+            return rc;
+        }
+    }
+    if (cap) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            s->gexec = nullptr;
+            s->tstep = tstep0;
+            s->depth = depth0;
+            s->tt = tt0;
+        }
+        FRM_HIP(ctx, e);
+        FRM_HIP(ctx, hipGraphLaunch(s->gexec, ctx->stream));
     }
     return FRM_OK;
 }
