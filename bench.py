@@ -44,13 +44,15 @@ def parse():
     ap.add_argument("--no-copy-probe", action="store_true",
                     help="skip the STREAM-copy bandwidth probe")
     ap.add_argument("--config", default="c1",
-                    choices=["c1", "c0", "c2", "c3", "c4", "ab1d", "ab1d_series", "timesum3d"],
+                    choices=["c1", "c0", "c2", "c3", "c4", "ab1d", "ab1d_series", "timesum3d",
+                             "soe_step"],
                     help="c1 (default): 2D LOD 512x512 plain FP64; c0: 1D mt_minimal N=200; "
                          "c2: 2D LOD 1024x1024 plain FP64 vs the chosen forms; c3: 4096 "
                          "psi-Caputo problems, FP64 vs compensated FP32; c4: 16 AB sets, plain "
                          "FP64 vs the chosen forms; ab1d: 1024 AB test problems, full history; "
                          "ab1d_series: 1024 AB test problems, the reference's series history; "
-                         "timesum3d: time sums of the 3D irrigation solver, 128x128x64 nodes")
+                         "timesum3d: time sums of the 3D irrigation solver, 128x128x64 nodes; "
+                         "soe_step: the standalone SOE history, 39.2M nodes, 64 modes")
     ap.add_argument("--nprob", type=int, default=None, help="c3: number of problems")
     ap.add_argument("--sets", type=int, default=None, help="c4: number of parameter sets")
     ap.add_argument("--plain-sample", type=int, default=64,
@@ -687,6 +689,85 @@ def run_timesum(args):
     ctx.close()
 
 
+def run_soe_step(args):
+    """The standalone sum-of-exponentials history (frm_soe_step) at the node
+    count of the 16 AB parameter sets (16 x 2.45e6 nodes): 64 modes fitted to
+    the L1 Caputo weights, one target; a step folds one
+    device-resident increment and evaluates the sum at every node.  The line
+    is the FP64-state form; FP32 states on the same inputs are reported with
+    their error (random signed increments cancel against the slow modes'
+    large states, so that error is well above the FP32 rounding unit); a node
+    sample is checked against the direct history sum with the fitted
+    weights."""
+    from theses_code_amd import _native as N
+    from theses_code_amd.solvers import SoeHistory
+
+    n, L, alpha = 16 * 2_450_000, 64, 0.6
+    K = args.steps if args.steps != 5000 else 200
+    lags = np.arange(1, 4001, dtype=np.float64)
+    q, c, fit_rel = N.host_soe_fit(lags ** (1 - alpha) - (lags - 1) ** (1 - alpha), 1, L, 1e-3)
+    ctx = N.Context(0)
+    rng = np.random.default_rng(0x5EED0004)
+    pool_h = [rng.uniform(-1.0, 1.0, n) for _ in range(4)]
+    pool = [N.DeviceArray.from_host(ctx, a) for a in pool_h]
+
+    def run(f32, timing):
+        s = SoeHistory(ctx, n, L, 1, f32=f32)
+        s.step_device(pool[0], q, c)  # warm-up, then a fresh run
+        s.clear()
+        if timing:
+            ctx.timing_reset()
+            ctx.timing_enable(1 << N.K_MEMTERM)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for k in range(K):
+            out = s.step_device(pool[k % len(pool)], q, c)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        kt = None
+        if timing:
+            kt = ctx.timing(N.K_MEMTERM)
+            ctx.timing_enable(0)
+        res = out.to_host()
+        s.close()
+        return dt, kt, res
+
+    dt64, _, o64 = run(False, False)
+    _, kt, _ = run(False, True)
+    dt32, _, o32 = run(True, False)
+    launches, ms, nbytes = kt
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms else 0.0
+    d = o32 - o64
+    sub = np.sort(rng.choice(n, 4096, replace=False))
+    w = c[0] @ (q[:, None] ** np.arange(K)[None, :])
+    Dsub = np.stack([pool_h[k % len(pool_h)][sub] for k in range(K)])
+    ref = w[::-1] @ Dsub
+    par = float(np.max(np.abs(o64[sub] - ref)) / np.max(np.abs(ref)))
+    for a in pool:
+        a.free()
+    line = {"metric": METRIC, "value": n * K / dt64, "unit": "node-steps/s", "n_gpus": 1,
+            "steps": K, "warmup": 1, "ms_per_step": dt64 * 1e3 / K, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "soe_history_39.2M_nodes_64_modes", "nodes": n, "modes": L,
+                       "alpha": alpha, "fit_max_rel": fit_rel, "parallelism": "1 GPU"},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_soe_step<double, 1>", "launches": launches,
+                         "avg_launch_ms": ms / launches if launches else None,
+                         "bytes_per_launch": nbytes / launches if launches else None,
+                         "measured_copy_GBs": None if args.no_copy_probe else ctx.copy_bandwidth()},
+            "f32_states": {"value": n * K / dt32, "seconds": dt32,
+                           "max_rel_err_vs_fp64": float(np.max(np.abs(d)) / np.max(np.abs(o64))),
+                           "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) /
+                                                        np.sqrt(np.mean(o64 ** 2)))},
+            "parity": {"vs": "direct history sum with the fitted weights, 4096-node sample, "
+                             "FP64 states", "max_rel_err": par},
+            "cpu_baseline": None}
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -704,6 +785,9 @@ def main():
         return
     if args.config == "timesum3d":
         run_timesum(args)
+        return
+    if args.config == "soe_step":
+        run_soe_step(args)
         return
 
     from theses_code_amd import _native as N

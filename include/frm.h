@@ -354,6 +354,31 @@ int frm_timesum_levels(const frm_timesum* s, int32_t* old_size);
 /* S (device, n_nodes values) for the levels pushed so far. */
 int frm_timesum_eval(frm_timesum* s, const frm_timesum_args* args, double* d_S);
 
+/* ------------------------------------------- sum-of-exponentials history ----
+ * Per-node exponential accumulators (the series forms H4/H5: the Lc2 sums of
+ * MF_intF_Fv, MF_int_series.cpp:720-836, advanced with the newest increment
+ * only; the dt_alg 3 moments, fr_d_t_g_2d.cpp:3385-3428).  Per node i and
+ * mode l < nterms:
+ *   H_l[i] = q[l] * H_l[i] + d_incr[i]                 (skipped if d_incr is NULL)
+ *   d_out[t*n_nodes + i] (+)= scale * sum_l h_coef[t*nterms + l] * H_l[i]
+ * for targets t < ntargets.  h_q (nterms) and h_coef (ntargets*nterms) are
+ * host arrays read at the call; d_incr and d_out are device arrays.  The
+ * states start at zero.  frm_host_soe_fit gives q and c for lag-indexed
+ * weights. */
+typedef struct frm_soe frm_soe;
+typedef struct frm_soe_desc {
+    int64_t n_nodes;
+    int32_t nterms;   /* 1..64 */
+    int32_t ntargets; /* 1 or 2 */
+    int32_t f32;      /* 1: FP32 mode states (recurrence and sums in FP64) */
+} frm_soe_desc;
+int frm_soe_create(frm_ctx* ctx, const frm_soe_desc* desc, frm_soe** out);
+int frm_soe_destroy(frm_soe* s);
+/* Zeroes the states (a new run). */
+int frm_soe_clear(frm_soe* s);
+int frm_soe_step(frm_ctx* ctx, frm_soe* s, const double* d_incr, const double* h_q,
+                 const double* h_coef, double scale, int32_t accumulate, double* d_out);
+
 #ifdef __cplusplus
 }
 #endif

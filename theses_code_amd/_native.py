@@ -71,6 +71,11 @@ class TimesumArgs(ct.Structure):
                 ("gtj", _D), ("gt", _D), ("alpha", _D), ("tau_n", _D)]
 
 
+class SoeDesc(ct.Structure):
+    _fields_ = [("n_nodes", ct.c_int64), ("nterms", ct.c_int32), ("ntargets", ct.c_int32),
+                ("f32", ct.c_int32)]
+
+
 MT_L1, MT_PSI, MT_AB = 0, 1, 2
 FORM_CONST, FORM_CV2, FORM_KEXP = 0, 1, 2
 TOP_AUTO, TOP_DIRICHLET, TOP_FLUX = 0, 1, 2
@@ -129,6 +134,10 @@ SIGNATURES = {
     "frm_timesum_push": (_I, [_V, _V]),
     "frm_timesum_levels": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_timesum_eval": (_I, [_V, ct.POINTER(TimesumArgs), _V]),
+    "frm_soe_create": (_I, [_V, ct.POINTER(SoeDesc), ct.POINTER(_V)]),
+    "frm_soe_destroy": (_I, [_V]),
+    "frm_soe_clear": (_I, [_V]),
+    "frm_soe_step": (_I, [_V, _V, _V, _P, _P, _D, ct.c_int32, _V]),
 }
 
 _lib = None

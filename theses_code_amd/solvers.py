@@ -279,3 +279,62 @@ class TimeSum:
     def __del__(self):
         self.close()
 
+
+
+class SoeHistory:
+    """Per-node sum-of-exponentials accumulators (frm_soe_*): the form the
+    reference's series histories take (the Lc2 sums of MF_intF_Fv,
+    MF_int_series.cpp:720-836; the dt_alg 3 moments, fr_d_t_g_2d.cpp:3385-3428).
+
+    step(incr, q, c) folds the increment into every mode, H_l = q_l H_l + incr,
+    and returns scale * sum_l c[t, l] H_l for each target t; step(None, c=c)
+    evaluates without folding."""
+
+    def __init__(self, ctx: N.Context, n_nodes: int, nterms: int, ntargets: int = 1,
+                 f32: bool = False):
+        self.ctx, self.n, self.nterms, self.ntargets = ctx, int(n_nodes), int(nterms), int(ntargets)
+        d = N.SoeDesc(self.n, self.nterms, self.ntargets, int(f32))
+        h = N._V()
+        N._check(N.lib().frm_soe_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
+        self.h = h
+        ctx.adopt(self)
+        self._inc = N.DeviceArray(ctx, self.n)
+        self._out = N.DeviceArray(ctx, self.n * self.ntargets)
+
+    def step_device(self, incr, q=None, c=None, scale: float = 1.0,
+                    accumulate: bool = False) -> N.DeviceArray:
+        """As step(), leaving the targets in a device buffer (n_nodes values
+        per target).  `incr` is a host array, a DeviceArray or None."""
+        if incr is None:
+            iptr = None
+        elif isinstance(incr, N.DeviceArray):
+            assert incr.n == self.n
+            iptr = incr.ptr
+        else:
+            a = np.ascontiguousarray(incr, dtype=np.float64).reshape(-1)
+            assert a.size == self.n
+            self.ctx.upload(self._inc.ptr, a)
+            iptr = self._inc.ptr
+        qa = None if q is None else np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        ca = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
+        assert ca.size == self.nterms * self.ntargets and (qa is None or qa.size == self.nterms)
+        N._check(N.lib().frm_soe_step(self.ctx.h, self.h, iptr, None if qa is None else N._dp(qa),
+                                      N._dp(ca), scale, int(accumulate), self._out.ptr),
+                 self.ctx.h)
+        return self._out
+
+    def step(self, incr, q=None, c=None, scale: float = 1.0) -> np.ndarray:
+        return self.step_device(incr, q, c, scale).to_host().reshape(self.ntargets, self.n)
+
+    def clear(self):
+        N._check(N.lib().frm_soe_clear(self.h), self.ctx.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().frm_soe_destroy(self.h)
+            self.h = None
+            self._inc.free()
+            self._out.free()
+
+    def __del__(self):
+        self.close()
