@@ -364,10 +364,11 @@ __global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const d
     const double x = fold ? ring[(int64_t)((n - kSoeLag0) % kSoeLag0) * ld + i] : 0.0;
     double s0 = 0.0, s1 = 0.0;
     for (int l = 0; l < L; ++l) {
-        double h = (double)H[(int64_t)l * ld + i];
+        // the mode planes are streamed once per step: non-temporal access
+        double h = (double)__builtin_nontemporal_load(H + (int64_t)l * ld + i);
         if (fold) {
             h = q[l] * h + x;
-            H[(int64_t)l * ld + i] = (HT)h;
+            __builtin_nontemporal_store((HT)h, H + (int64_t)l * ld + i);
             h = (double)(HT)h;
         }
         s0 += c[l] * h;

@@ -54,11 +54,13 @@ __global__ __launch_bounds__(kSoeStepThreads) void k_soe_step(
     const double x = fold ? __builtin_nontemporal_load(incr + i) : 0.0;
     double s0 = 0.0, s1 = 0.0;
     for (int l = 0; l < nterms; ++l) {
-        double h = (double)H[(int64_t)l * ld + i];
+        // the planes are streamed once per step (far larger than the
+        // caches): non-temporal loads and stores
+        double h = (double)__builtin_nontemporal_load(H + (int64_t)l * ld + i);
         if (fold) {
             h = cf.q[l] * h + x;
             const HT hs = (HT)h;
-            H[(int64_t)l * ld + i] = hs;
+            __builtin_nontemporal_store(hs, H + (int64_t)l * ld + i);
             h = (double)hs;
         }
         s0 += cf.c[l] * h;
