@@ -753,7 +753,7 @@ def run_soe_step(args):
                        "alpha": alpha, "fit_max_rel": fit_rel, "parallelism": "1 GPU"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "k_soe_step<double, 1>", "launches": launches,
+                         "kernel": "k_soe_step<double, 1, true>", "launches": launches,
                          "avg_launch_ms": ms / launches if launches else None,
                          "bytes_per_launch": nbytes / launches if launches else None,
                          "measured_copy_GBs": None if args.no_copy_probe else ctx.copy_bandwidth()},

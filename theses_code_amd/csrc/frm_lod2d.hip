@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "frm_history.hpp"
+#include "frm_stream.hpp"
 #include "frm_mfma.hpp"
 #include "frm_wavescan.hpp"
 #include "frm_soe.hpp"
@@ -352,7 +353,7 @@ __global__ __launch_bounds__((col_width<CM, W>() * kWave * W)) void k_lod_cols(C
 // the states are stored in FP32.
 constexpr int kSoeLag0 = 2;
 
-template <typename HT>
+template <typename HT, bool NT>
 __global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const double* __restrict__ ring,
                                                      int64_t ld, int n, int L,
                                                      const double* __restrict__ q,
@@ -364,11 +365,12 @@ __global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const d
     const double x = fold ? ring[(int64_t)((n - kSoeLag0) % kSoeLag0) * ld + i] : 0.0;
     double s0 = 0.0, s1 = 0.0;
     for (int l = 0; l < L; ++l) {
-        // the mode planes are streamed once per step: non-temporal access
-        double h = (double)__builtin_nontemporal_load(H + (int64_t)l * ld + i);
+        // the mode planes are streamed once per step (non-temporal when
+        // they exceed the cache)
+        double h = (double)stream_load<NT>(H + (int64_t)l * ld + i);
         if (fold) {
             h = q[l] * h + x;
-            __builtin_nontemporal_store((HT)h, H + (int64_t)l * ld + i);
+            stream_store<NT>((HT)h, H + (int64_t)l * ld + i);
             h = (double)(HT)h;
         }
         s0 += c[l] * h;
@@ -964,14 +966,19 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_MEMTERM,
                           (double)ld * (2.0 * s->soe_L * (s->soe_f32 ? 4 : 8) + 2 * 8 + 2 * 8));
             const dim3 grid((unsigned)((ld + 255) / 256));
-            if (s->soe_f32)
-                hipLaunchKernelGGL(k_soe_memterm<float>, grid, dim3(256), 0, ctx->stream,
-                                   (float*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
-                                   s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
-            else
-                hipLaunchKernelGGL(k_soe_memterm<double>, grid, dim3(256), 0, ctx->stream,
-                                   (double*)s->soe_H, (const double*)s->hist->data, ld, n, s->soe_L,
-                                   s->soe_q, s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
+            const bool nt = (double)ld * s->soe_L * (s->soe_f32 ? 4 : 8) >= kStreamBypassBytes;
+            auto launch = [&](auto kern, auto* H) {
+                hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, H,
+                                   (const double*)s->hist->data, ld, n, s->soe_L, s->soe_q,
+                                   s->soe_c, s->soe_w1r, s->soe_w1c, s->hist->part);
+            };
+            if (s->soe_f32) {
+                if (nt) launch(k_soe_memterm<float, true>, (float*)s->soe_H);
+                else launch(k_soe_memterm<float, false>, (float*)s->soe_H);
+            } else {
+                if (nt) launch(k_soe_memterm<double, true>, (double*)s->soe_H);
+                else launch(k_soe_memterm<double, false>, (double*)s->soe_H);
+            }
             FRM_HIP(ctx, hipGetLastError());
         } else if (bin) {
             nch = 1;

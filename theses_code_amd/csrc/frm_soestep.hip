@@ -18,6 +18,7 @@
 // (2 * nterms * 8 B, or 4 B per value with FP32 mode states) and reads the
 // increment and writes the targets.
 #include "frm_internal.hpp"
+#include "frm_stream.hpp"
 
 namespace frm {
 
@@ -44,7 +45,7 @@ struct frm_soe {
 // and c[l] are scalar loads from the kernel arguments.  With FP32 states the
 // recurrence runs in FP64 and the stored (rounded) value is the one summed,
 // so the output is the sum of the states the next step starts from.
-template <typename HT, int NT>
+template <typename HT, int NT, bool BYPASS>
 __global__ __launch_bounds__(kSoeStepThreads) void k_soe_step(
     HT* __restrict__ H, int64_t ld, int64_t n, int nterms, const double* __restrict__ incr,
     SoeCoefs cf, double scale, int accumulate, double* __restrict__ out) {
@@ -54,13 +55,13 @@ __global__ __launch_bounds__(kSoeStepThreads) void k_soe_step(
     const double x = fold ? __builtin_nontemporal_load(incr + i) : 0.0;
     double s0 = 0.0, s1 = 0.0;
     for (int l = 0; l < nterms; ++l) {
-        // the planes are streamed once per step (far larger than the
-        // caches): non-temporal loads and stores
-        double h = (double)__builtin_nontemporal_load(H + (int64_t)l * ld + i);
+        // the planes are streamed once per step (non-temporal when they
+        // exceed the cache)
+        double h = (double)stream_load<BYPASS>(H + (int64_t)l * ld + i);
         if (fold) {
             h = cf.q[l] * h + x;
             const HT hs = (HT)h;
-            __builtin_nontemporal_store(hs, H + (int64_t)l * ld + i);
+            stream_store<BYPASS>(hs, H + (int64_t)l * ld + i);
             h = (double)hs;
         }
         s0 += cf.c[l] * h;
@@ -155,21 +156,20 @@ int frm_soe_step(frm_ctx* ctx, frm_soe* s, const double* d_incr, const double* h
                                          s->ntargets * 8.0 * (accumulate ? 2.0 : 1.0));
     const dim3 grid((unsigned)((s->n + kSoeStepThreads - 1) / kSoeStepThreads));
     const int acc = accumulate != 0;
+    const bool nt = (double)s->nterms * s->ld * esz >= kStreamBypassBytes;
+    auto launch = [&](auto kern, auto* H) {
+        hipLaunchKernelGGL(kern, grid, dim3(kSoeStepThreads), 0, ctx->stream, H, s->ld, s->n,
+                           s->nterms, d_incr, cf, scale, acc, d_out);
+    };
     LaunchTimer t(ctx, FRM_K_MEMTERM, bytes);
     if (s->f32) {
-        if (s->ntargets == 2)
-            hipLaunchKernelGGL((k_soe_step<float, 2>), grid, dim3(kSoeStepThreads), 0, ctx->stream,
-                               (float*)s->H, s->ld, s->n, s->nterms, d_incr, cf, scale, acc, d_out);
-        else
-            hipLaunchKernelGGL((k_soe_step<float, 1>), grid, dim3(kSoeStepThreads), 0, ctx->stream,
-                               (float*)s->H, s->ld, s->n, s->nterms, d_incr, cf, scale, acc, d_out);
+        float* H = (float*)s->H;
+        if (s->ntargets == 2) nt ? launch(k_soe_step<float, 2, true>, H) : launch(k_soe_step<float, 2, false>, H);
+        else nt ? launch(k_soe_step<float, 1, true>, H) : launch(k_soe_step<float, 1, false>, H);
     } else {
-        if (s->ntargets == 2)
-            hipLaunchKernelGGL((k_soe_step<double, 2>), grid, dim3(kSoeStepThreads), 0, ctx->stream,
-                               (double*)s->H, s->ld, s->n, s->nterms, d_incr, cf, scale, acc, d_out);
-        else
-            hipLaunchKernelGGL((k_soe_step<double, 1>), grid, dim3(kSoeStepThreads), 0, ctx->stream,
-                               (double*)s->H, s->ld, s->n, s->nterms, d_incr, cf, scale, acc, d_out);
+        double* H = (double*)s->H;
+        if (s->ntargets == 2) nt ? launch(k_soe_step<double, 2, true>, H) : launch(k_soe_step<double, 2, false>, H);
+        else nt ? launch(k_soe_step<double, 1, true>, H) : launch(k_soe_step<double, 1, false>, H);
     }
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
