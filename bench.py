@@ -391,10 +391,12 @@ def run_ab1d(args, series=False):
     ctx.close()
 
 
-def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
+def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, window=None, **kw):
     """One full run from t = 0.  Wall time is taken with per-launch timing off
     unless `timing` (then every launch is bracketed by HIP events and the
-    per-kernel totals are returned; the wall time then includes the events)."""
+    per-kernel totals are returned; the wall time then includes the events).
+    `window` = (start, steps): the steps start+1 .. start+steps are also timed
+    on their own, between two synchronisations inside the same run."""
     from theses_code_amd.solvers import TGSolver
 
     sN, M, coefs, C0 = case
@@ -406,11 +408,22 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, K, precision, timing=False, **kw):
     ctx.timing_enable(sum(1 << k for _, k in classes) if timing else 0)
     ctx.sync()
     t0 = time.perf_counter()
-    s.calc_step(K)
+    if window:
+        s.calc_step(window[0])
+        ctx.sync()
+        tw = time.perf_counter()
+        s.calc_step(window[1])
+        ctx.sync()
+        tw = time.perf_counter() - tw
+        s.calc_step(K - window[0] - window[1])
+    else:
+        s.calc_step(K)
     ctx.sync()
     dt = time.perf_counter() - t0
     ctx.timing_enable(0)
     kt, work = {}, {}
+    if window:
+        work["window_s"] = tw
     for name, k in classes if timing else ():
         n, ms, by = ctx.timing(k)
         if n:
@@ -465,13 +478,18 @@ def run_lod_forms(args, cfg_name):
               "sets", file=sys.stderr)
         forms.remove("binomial")
     tot_nodes, t_plain, kt_plain = 0, 0.0, {}
+    # a window of 64 steps at depth T/2 (the CPU baseline's depth), aligned to
+    # the MFMA form's 64-step blocks, timed inside the full runs
+    win = (K // 2 // 64 * 64, 64) if K >= 192 else None
+    tw_plain, tw_forms = 0.0, {f: 0.0 for f in forms}
     res = {f: dict(seconds=0.0, errs=[], rms=[], fits=[], kernels_ms={}, work={}) for f in forms}
     for r in runs:
         sN, M = r["case"][0], r["case"][1]
         nodes = (sN - 1) * (M - 1)
         tot_nodes += nodes
-        dp, up, kp, _, _ = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
-                                    **kw, **{"precision": N.PREC_F64})
+        dp, up, kp, _, wp = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
+                                     window=win, **kw, **{"precision": N.PREC_F64})
+        tw_plain += wp.get("window_s", 0.0)
         if args.breakdown_plain:
             kp = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"], K,
                           timing=True, **kw, **{"precision": N.PREC_F64})[2]
@@ -481,7 +499,8 @@ def run_lod_forms(args, cfg_name):
         msg = f"# set alpha={r['alpha']:.3f} grid={sN - 1}^2 plain {dp:.3f}s"
         for f in forms:
             ds, us, _, fit, w0 = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"],
-                                          cfg["sigma"], K, **kw, **fkw[f])
+                                          cfg["sigma"], K, window=win, **kw, **fkw[f])
+            tw_forms[f] += w0.get("window_s", 0.0)
             res[f].setdefault("sweep_shapes", []).append(w0["sweep_shape"])
             # per-kernel breakdown from a second, event-timed run
             _, _, ks, _, wk = _lod_run(ctx, N, r["case"], r["alpha"], cfg["tau"], cfg["sigma"],
@@ -495,7 +514,7 @@ def run_lod_forms(args, cfg_name):
             for k, v in ks.items():
                 q["kernels_ms"][k] = q["kernels_ms"].get(k, 0.0) + v
             for k, v in wk.items():
-                if k != "sweep_shape":
+                if k not in ("sweep_shape", "window_s"):
                     q["work"][k] = q["work"].get(k, 0.0) + v
             msg += f" {f} {ds:.3f}s err {q['errs'][-1]:.2e}"
         print(msg, file=sys.stderr, flush=True)
@@ -537,6 +556,15 @@ def run_lod_forms(args, cfg_name):
             "plain_fp64": {"value": tot_nodes * K / t_plain, "seconds": t_plain,
                            "kernels_ms": kt_plain or None},
             "speedup_vs_plain_fp64": t_plain / out_forms[best]["seconds"]}
+    if win:
+        line["window_T2"] = {
+            "start_step": win[0], "steps": win[1],
+            "note": "the same runs, steps start+1..start+steps timed between two "
+                    "synchronisations (per-step cost grows with depth, so depth T/2 is the "
+                    "run average)",
+            "plain_fp64": tot_nodes * win[1] / tw_plain if tw_plain else None,
+            "forms": {f: tot_nodes * win[1] / tw_forms[f] if tw_forms[f] else None
+                      for f in forms}}
     if not args.no_copy_probe:
         line["measured_copy_GBs"] = ctx.copy_bandwidth()
     if args.cpu_steps > 0:
