@@ -765,6 +765,12 @@ def run_soe_step(args):
     dt32, _, o32 = run(True, False)
     launches, ms, nbytes = kt
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms else 0.0
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "soe_step_traffic.json")
+    if os.path.exists(tf) and launches:
+        # PMC-measured ratio of HBM to algorithmic bytes of this kernel
+        with open(tf) as f:
+            traffic = json.load(f)["traffic_over_algorithmic"] * nbytes / launches
     d = o32 - o64
     sub = np.sort(rng.choice(n, 4096, replace=False))
     w = c[0] @ (q[:, None] ** np.arange(K)[None, :])
@@ -780,7 +786,7 @@ def run_soe_step(args):
             "config": {"workload": "soe_history_39.2M_nodes_64_modes", "nodes": n, "modes": L,
                        "alpha": alpha, "fit_max_rel": fit_rel, "parallelism": "1 GPU"},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                          "kernel": "k_soe_step<double, 1, true>", "launches": launches,
                          "avg_launch_ms": ms / launches if launches else None,
                          "bytes_per_launch": nbytes / launches if launches else None,
