@@ -162,7 +162,12 @@ typedef struct frm_lod2d_desc {
     int32_t ab_maxkiter; /* AB inner series limit (0: 1000) */
     double ab_eps;       /* AB series tolerance (0: 1e-10) */
     int32_t ab_exact;    /* 1: AB weights to FP64 accuracy instead of the MF_int_choice methods */
-    int32_t soe_terms;   /* FRM_PREC_SOE: exponentials (0: 32, max 64) */
+    int32_t soe_terms;   /* FRM_PREC_SOE: exponentials (0: 32, max 64); a
+                            history of fewer lags than terms is fitted by
+                            capacity - 1 of them.  Creation fails with
+                            FRM_E_INVALID when the fit misses the weights by
+                            more than 1e-6 relative (e.g. the truncated AB
+                            weights without ab_exact) */
     int32_t soe_f32;     /* FRM_PREC_SOE: 1 stores the mode states in FP32 */
     int32_t mfma_parts;  /* FRM_PREC_MFMA_BF16: BF16 parts per operand, 2 (three
                             products, ~2^-16) or 3 (six products, exact FP32

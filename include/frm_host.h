@@ -41,8 +41,10 @@ int frm_host_caputo_lod_weights(double alpha, double tau, int32_t D, double* out
 /* Sum-of-exponentials fit of lag-indexed weights w[t*K + k-1] (targets
  * t < ntargets, lags k = k0..K) by L decays q_l = exp(-r_l tau) with
  * log-spaced rates and FP64 least squares in relative weighting:
- * w_t(k) ~ sum_l c[t*L + l] q_l^(k - k0).  Returns the largest relative
- * error over the fitted lags. */
+ * w_t(k) ~ sum_l c[t*L + l] q_l^(k - k0).  Stores the largest relative
+ * error over the fitted lags in *max_rel.  With fewer lags than modes
+ * (K - k0 + 1 < L) only K - k0 + 1 modes are fitted and the others get
+ * q = c = 0.  FRM_E_INVALID for bad arguments or when no fit is finite. */
 int frm_host_soe_fit(const double* w, int32_t ntargets, int32_t K, int32_t k0, int32_t L,
                      double tau, double* q, double* c, double* max_rel);
 

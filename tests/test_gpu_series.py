@@ -111,3 +111,33 @@ def test_caputo_weights_closed_form(ctx):
     assert np.sum(wc) == pytest.approx(c * ((n + 1) ** p - 1), rel=1e-12)
     assert kr == pytest.approx(c * 0.5 ** p, rel=1e-14)
     assert kh == pytest.approx(c * (1 - 0.5 ** p), rel=1e-14)
+
+
+# A history shorter than the number of exponentials (capacity 20, 32 terms:
+# 19 fitted lags): the fit takes 19 modes, the run stays finite and within
+# the chosen forms' bar of the full FP64 history.
+@pytest.mark.parametrize("cap,L", [(20, 32), (5, 64)])
+def test_soe_short_capacity(ctx, cap, L):
+    sN, M = 40, 33
+    coefs, C0 = _case(9, sN, M, 1e-3)
+    full = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, C0, capacity=cap, C1=0.0)
+    soe = S.TGSolver(ctx, sN, M, 0.7, 1e-3, 1.0, coefs, C0, capacity=cap, C1=0.0,
+                     precision=N.PREC_SOE, soe_terms=L)
+    assert np.isfinite(soe.fit_error) and soe.fit_error <= 1e-8
+    full.calc_step(cap)
+    soe.calc_step(cap)
+    out = soe.b_C
+    assert np.all(np.isfinite(out))
+    assert relerr(out, full.b_C) <= 1e-6
+    full.close()
+    soe.close()
+
+
+# The truncated MF_int_choice weights (not FP64-accurate) at alpha 0.7 over
+# 400 lags switch to the quadrature branch, which 64 exponentials fit only to
+# ~2e-4: the solver refuses the form instead of running it.
+def test_soe_refuses_a_fit_above_the_bar(ctx):
+    coefs, C0 = _case(10, 20, 20, 5e-2)
+    with pytest.raises(N.FrmError, match="exponential sum"):
+        S.TGSolver(ctx, 20, 20, 0.7, 1e-3, 1.0, coefs, C0, capacity=400, C1=0.1,
+                   kernel=N.KERNEL_AB, ab_exact=False, precision=N.PREC_SOE, soe_terms=64)

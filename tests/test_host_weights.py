@@ -96,3 +96,28 @@ def test_soe_fit_reports_its_true_error(L, bound):
     err = np.max(np.abs(approx - w[:, k0 - 1:]) / np.abs(w[:, k0 - 1:]))
     assert err == pytest.approx(max_rel, rel=1e-3, abs=1e-15)
     assert max_rel < bound
+
+
+@pytest.mark.parametrize("K,L", [(20, 32), (3, 64), (33, 32)])
+def test_soe_fit_short_history(K, L):
+    # fewer fitted lags (K - k0 + 1) than modes: the fit uses that many modes,
+    # the others carry q = c = 0, and the sum reproduces every fitted lag
+    alpha, tau, k0 = 0.6, 1e-3, 2
+    _, _, _, wr, wc = N.host_caputo_lod_weights(alpha, tau, K)
+    w = np.stack([wr, wc])
+    q, c, max_rel = N.host_soe_fit(w, k0, L, tau)
+    m = K - k0 + 1
+    assert np.all(np.isfinite(q)) and np.all(np.isfinite(c)) and np.isfinite(max_rel)
+    used = min(L, m)
+    assert np.all((q[:used] > 0) & (q[:used] < 1))
+    assert np.all(q[used:] == 0.0) and np.all(c[:, used:] == 0.0)
+    k = np.arange(k0, K + 1)
+    approx = c @ (q[:, None] ** (k - k0)[None, :])
+    err = np.max(np.abs(approx - w[:, k0 - 1:]) / np.abs(w[:, k0 - 1:]))
+    assert err <= 1e-8 and err == pytest.approx(max_rel, rel=1e-2, abs=1e-14)
+
+
+def test_soe_fit_rejects_bad_arguments():
+    w = np.ones((1, 10))
+    with pytest.raises(RuntimeError):
+        N.host_soe_fit(w, 0, 8, 1e-3)  # lag 0 does not exist

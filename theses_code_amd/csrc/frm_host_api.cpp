@@ -64,10 +64,12 @@ int frm_host_soe_fit(const double* w, int32_t ntargets, int32_t K, int32_t k0, i
     std::vector<const double*> tw;
     for (int t = 0; t < ntargets; ++t) tw.push_back(w + (size_t)t * K);
     const SoeFit f = soe_fit(tw, K, k0, L, tau);
+    if (f.q.size() != (size_t)L || f.c.size() != (size_t)ntargets * L) return FRM_E_INVALID;
     for (int l = 0; l < L; ++l) q[l] = f.q[l];
     for (size_t i = 0; i < f.c.size(); ++i) c[i] = f.c[i];
     *max_rel = f.max_rel;
-    return FRM_OK;
+    // no finite fit over the rate ranges tried
+    return std::isfinite(f.max_rel) ? FRM_OK : FRM_E_INVALID;
 }
 
 }  // extern "C"

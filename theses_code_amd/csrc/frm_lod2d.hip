@@ -352,6 +352,9 @@ __global__ __launch_bounds__((col_width<CM, W>() * kWave * W)) void k_lod_cols(C
 // multiply is done in FP64 so that slow modes keep their exact factor when
 // the states are stored in FP32.
 constexpr int kSoeLag0 = 2;
+// largest relative error of a fitted exponential sum over the weights it
+// replaces that frm_lod2d_create accepts
+constexpr double kSoeFitTol = 1e-6;
 
 template <typename HT, bool NT>
 __global__ __launch_bounds__(256) void k_soe_memterm(HT* __restrict__ H, const double* __restrict__ ring,
@@ -827,6 +830,16 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         frm::SoeFit fit = frm::soe_fit({w.data(), w.data() + d->capacity}, d->capacity, kSoeLag0,
                                        s->soe_L, d->tau);
         s->soe_err = fit.max_rel;
+        // a fit that misses the weights by more than the chosen forms' error
+        // bar (or is not finite) is refused rather than run
+        if (!(fit.max_rel <= kSoeFitTol)) {
+            lod_free(s);
+            return set_error(ctx, FRM_E_INVALID,
+                             "frm_lod2d_create: the %d-term exponential sum fits the history "
+                             "weights to %.3g relative (limit %.0e)%s",
+                             d->soe_terms > 0 ? d->soe_terms : 32, fit.max_rel, kSoeFitTol,
+                             is_ab && !d->ab_exact ? "; truncated AB weights: set ab_exact" : "");
+        }
         const size_t hsz = (s->soe_f32 ? sizeof(float) : sizeof(double)) * (size_t)s->soe_L * s->hist->ld;
         e = hipMalloc(&s->soe_H, hsz);
         if (e == hipSuccess) e = hipMalloc(&s->soe_q, sizeof(double) * s->soe_L);

@@ -15,10 +15,11 @@
 namespace frm {
 
 namespace {
-// Least squares min |A x - b| for A m x n (column-major), several right-hand
-// sides; A and B are overwritten.  Returns x (n x nrhs).
+// Least squares min |A x - b| for A m x n (column-major, m >= n), several
+// right-hand sides; A and B are overwritten.  Returns x (n x nrhs).
 std::vector<double> householder_lsq(int m, int n, std::vector<double>& A, int nrhs,
                                     std::vector<double>& B) {
+    if (m < n) return std::vector<double>((size_t)n * nrhs, 0.0);  // callers fit at most m modes
     for (int j = 0; j < n; ++j) {
         double* col = &A[(size_t)j * m];
         double norm = 0.0;
@@ -56,22 +57,26 @@ std::vector<double> householder_lsq(int m, int n, std::vector<double>& A, int nr
 
 SoeFit soe_fit(const std::vector<const double*>& w, int K, int k0, int L, double tau) {
     SoeFit best;
-    best.max_rel = 1e300;
+    best.max_rel = HUGE_VAL;
     const int m = K - k0 + 1;
     const int nt = (int)w.size();
+    best.q.assign(L > 0 ? L : 0, 0.0);
+    best.c.assign((size_t)nt * (L > 0 ? L : 0), 0.0);
     if (m <= 0 || L <= 0) {
         best.max_rel = 0.0;
-        best.q.assign(L, 0.0);
-        best.c.assign((size_t)nt * L, 0.0);
         return best;
     }
+    // A history with fewer fitted lags than modes is fitted by m modes (the
+    // system is then square); the remaining modes keep q = 0 and c = 0, so
+    // their recurrences add nothing.
+    const int Lf = std::min(L, m);
     const double lo_f[] = {1e-3, 1e-2, 1e-1}, hi_f[] = {1.0, 3.0, 10.0};
     for (double flo : lo_f) {
         for (double fhi : hi_f) {
             const double rlo = flo / (K * tau), rhi = fhi / (k0 * tau);
-            std::vector<double> q(L);
-            for (int l = 0; l < L; ++l) {
-                const double x = L == 1 ? std::log(rlo) : std::log(rlo) + (std::log(rhi) - std::log(rlo)) * l / (L - 1);
+            std::vector<double> q(L, 0.0);
+            for (int l = 0; l < Lf; ++l) {
+                const double x = Lf == 1 ? std::log(rlo) : std::log(rlo) + (std::log(rhi) - std::log(rlo)) * l / (Lf - 1);
                 q[l] = std::exp(-std::exp(x) * tau);
             }
             // columns q_l^(k-k0), each scaled to unit max; rows scaled by 1/|w|
@@ -82,9 +87,9 @@ SoeFit soe_fit(const std::vector<const double*>& w, int K, int k0, int L, double
             cur.c.assign((size_t)nt * L, 0.0);
             cur.max_rel = 0.0;
             for (int t = 0; t < nt; ++t) {
-                std::vector<double> A((size_t)m * L), B(m);
-                std::vector<double> colmax(L, 0.0);
-                for (int l = 0; l < L; ++l) {
+                std::vector<double> A((size_t)m * Lf), B(m);
+                std::vector<double> colmax(Lf, 0.0);
+                for (int l = 0; l < Lf; ++l) {
                     double p = 1.0;
                     for (int i = 0; i < m; ++i) {
                         A[(size_t)l * m + i] = p;
@@ -95,25 +100,26 @@ SoeFit soe_fit(const std::vector<const double*>& w, int K, int k0, int L, double
                     const double y = w[t][k0 - 1 + i];
                     const double sc = y != 0.0 ? 1.0 / std::fabs(y) : 1.0;
                     B[i] = y * sc;
-                    for (int l = 0; l < L; ++l) A[(size_t)l * m + i] *= sc;
+                    for (int l = 0; l < Lf; ++l) A[(size_t)l * m + i] *= sc;
                 }
-                for (int l = 0; l < L; ++l) {
+                for (int l = 0; l < Lf; ++l) {
                     for (int i = 0; i < m; ++i) colmax[l] = std::max(colmax[l], std::fabs(A[(size_t)l * m + i]));
                     if (colmax[l] > 0)
                         for (int i = 0; i < m; ++i) A[(size_t)l * m + i] /= colmax[l];
                 }
-                std::vector<double> x = householder_lsq(m, L, A, 1, B);
-                for (int l = 0; l < L; ++l) cur.c[(size_t)t * L + l] = colmax[l] > 0 ? x[l] / colmax[l] : 0.0;
-                // residual
-                std::vector<double> p(L, 1.0);
+                std::vector<double> x = householder_lsq(m, Lf, A, 1, B);
+                for (int l = 0; l < Lf; ++l) cur.c[(size_t)t * L + l] = colmax[l] > 0 ? x[l] / colmax[l] : 0.0;
+                // residual; a non-finite one counts as an infinite error
+                std::vector<double> p(Lf, 1.0);
                 for (int i = 0; i < m; ++i) {
                     double s = 0.0;
-                    for (int l = 0; l < L; ++l) {
+                    for (int l = 0; l < Lf; ++l) {
                         s += cur.c[(size_t)t * L + l] * p[l];
                         p[l] *= q[l];
                     }
                     const double y = w[t][k0 - 1 + i];
-                    const double e = std::fabs(s - y) / std::max(std::fabs(y), 1e-300);
+                    double e = std::fabs(s - y) / std::max(std::fabs(y), 1e-300);
+                    if (!std::isfinite(e)) e = HUGE_VAL;
                     cur.max_rel = std::max(cur.max_rel, e);
                 }
             }
