@@ -4,7 +4,13 @@
  * and store per lane) `reps` times on the context's stream and reports the
  * average kernel time from HIP events: the measured achievable HBM
  * bandwidth the memory-term roofline is compared against, and the known-size
- * kernel that scales rocprof FETCH_SIZE / WRITE_SIZE counters. */
+ * kernel that scales rocprof FETCH_SIZE / WRITE_SIZE counters.
+ *
+ * frm_diag_read streams `bytes` of device memory with 16-byte non-temporal
+ * loads, four in flight per lane, and writes one 16-byte sum per lane (the
+ * access pattern of the history sums): the measured read bandwidth, which
+ * the read-dominated kernels are compared against.  It allocates its own
+ * small buffer of per-lane sums (16 MiB). */
 #ifndef FRM_DIAG_H
 #define FRM_DIAG_H
 #include <stddef.h>
@@ -17,6 +23,7 @@ extern "C" {
 
 int frm_diag_copy(frm_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, int reps,
                   double* avg_ms);
+int frm_diag_read(frm_ctx* ctx, const void* d_src, size_t bytes, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

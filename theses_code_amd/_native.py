@@ -116,6 +116,7 @@ SIGNATURES = {
     "frm_lod2d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
     "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
+    "frm_diag_read": (_I, [_V, _V, _S, _I, _P]),
     "frm_lod2d_fit_error": (_I, [_V, _P]),
     "frm_lod2d_sweep_shape": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_weights": (_I, [_V, ct.c_int32, _P]),
@@ -225,6 +226,17 @@ class Context:
             self.free(src)
             self.free(dst)
         return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+    def read_bandwidth(self, nbytes: int = 4 << 30, reps: int = 10) -> float:
+        """Measured device read bandwidth in GB/s (a non-temporal 16-byte
+        stream, four loads in flight per lane, one small write per lane)."""
+        src = self.malloc(nbytes)
+        ms = _D()
+        try:
+            _check(lib().frm_diag_read(self.h, src, nbytes, reps, ct.byref(ms)), self.h)
+        finally:
+            self.free(src)
+        return nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if getattr(self, "h", None):
