@@ -110,6 +110,17 @@ typedef struct frm_memterm_desc {
     int32_t accumulate; /* 0: out = ..., 1: out += ... */
 } frm_memterm_desc;
 int frm_memterm(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double* d_out);
+/* frm_memterm for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  The same kernels and results as
+ * frm_memterm, bit for bit, but the partial sums of its time chunks go to a
+ * static per-GPU array of 16 MiB (2^21 FP64 values) instead of the store's
+ * workspace, and the call allocates nothing and records no timing events.
+ * FRM_E_RANGE when time chunks x ntargets x padded nodes exceed 2^21 (the
+ * one-chunk case fits up to 2^20 nodes with two targets, 2^21 with one).
+ * Every call on one GPU shares the array: calls that overlap on one GPU
+ * (other streams or contexts) race. */
+int frm_memterm_static(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double* d_out);
 
 /* ------------------------------------------------ batched tridiagonal ----
  * a[k] x[k-1] + b[k] x[k] + c[k] x[k+1] = d[k], k = 0..n-1 (a[0], c[n-1]

@@ -107,6 +107,7 @@ SIGNATURES = {
     "frm_hist_depth": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_hist_clear": (_I, [_V]),
     "frm_memterm": (_I, [_V, _V, ct.POINTER(MemtermDesc), _V]),
+    "frm_memterm_static": (_I, [_V, _V, ct.POINTER(MemtermDesc), _V]),
     "frm_tridiag": (_I, [_V, ct.POINTER(TridiagDesc), _V, _V, _V, _V, _V]),
     "frm_lod2d_create": (_I, [_V, ct.POINTER(Lod2dDesc), ct.POINTER(_V)]),
     "frm_lod2d_destroy": (_I, [_V]),

@@ -13,10 +13,39 @@
 // chunk order (deterministic).  The weights are uniform across a wavefront,
 // so they come through the scalar cache.
 #include <algorithm>
+#include <mutex>
 
 #include "frm_history.hpp"
 
 namespace frm {
+
+// Partial-sum workspace of frm_memterm_static: a file-scope device array of
+// kStaticScratchDoubles FP64 values (16 MiB), one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself (on the host that
+// name is a stand-in, not a device address): static_scratch_setup looks up
+// each device's address once, with that device current, and the entry
+// point hands the kernels that address.
+constexpr int64_t kStaticScratchDoubles = int64_t(1) << 21;
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+int static_scratch_setup(frm_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return set_error(ctx, FRM_E_INVALID, "static scratch: device %d beyond %d", ctx->device,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return check_hip(ctx, e, "hipGetSymbolAddress(scratch)");
+        if (!p) return set_error(ctx, FRM_E_HIP, "hipGetSymbolAddress(scratch): null address");
+        g_scratch_ptr[ctx->device] = p;
+    }
+    ctx->static_scratch = g_scratch_ptr[ctx->device];
+    return FRM_OK;
+}
 
 // ---------------------------------------------------------------- kernels
 // part[(c*NT + t)*ld + i] = sum_{s in chunk c} w[t*ldw + depth-1-s] * Delta_s[i]
@@ -217,45 +246,55 @@ static int pick_chunks(const frm_hist* h, int depth) {
     return std::max(1, std::min(nch, h->max_chunks));
 }
 
+int memterm_chunks(const frm_hist* h, int first) {
+    const int depth = h->depth;
+    first = std::max(0, std::min(first, depth));
+    return pick_chunks(h, depth - first);
+}
+
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
-                     int* nchunks_out, int first, const float* d_w32) {
+                     int* nchunks_out, int first, const float* d_w32, double* part) {
     const int depth = h->depth;
     first = std::max(0, std::min(first, depth));
     const int span = depth - first;  // slots first .. depth-1, lags depth-first .. 1
     const int nchunks = pick_chunks(h, span);
     *nchunks_out = nchunks;
     if (nchunks == 0) return FRM_OK;
+    // a workspace given by the caller: no timing events (recording them may
+    // create events)
+    frm_ctx* tctx = part ? nullptr : ctx;
+    if (!part) part = h->part;
     const int chunk = (span + nchunks - 1) / nchunks;
     const int threads = kMemtermThreads;
     dim3 grid((unsigned)(h->ld / 2 / threads), (unsigned)nchunks);
     if (h->precision == FRM_PREC_F32C) {
         FRM_REQUIRE(ctx, ntargets == 1 && d_w32, "memterm_partials: FP32 store takes one target");
-        LaunchTimer timer(ctx, FRM_K_MEMTERM,
+        LaunchTimer timer(tctx, FRM_K_MEMTERM,
                           (double)span * h->n_nodes * sizeof(float) +
                               (double)h->n_nodes * sizeof(double));
         if (h->ld >= kF32WideLd) {
             // ld is a multiple of 4 * 256: the last workgroup may be half idle
             dim3 g4((unsigned)((h->ld / 4 + threads - 1) / threads), (unsigned)nchunks);
             hipLaunchKernelGGL(k_memterm_f32c<4>, g4, dim3(threads), 0, ctx->stream,
-                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, part);
         } else {
             hipLaunchKernelGGL(k_memterm_f32c<2>, grid, dim3(threads), 0, ctx->stream,
-                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, h->part);
+                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, part);
         }
         FRM_HIP(ctx, hipGetLastError());
         return FRM_OK;
     }
     FRM_REQUIRE(ctx, h->precision == FRM_PREC_F64, "memterm_partials: precision %d",
                 h->precision);
-    LaunchTimer timer(ctx, FRM_K_MEMTERM,
+    LaunchTimer timer(tctx, FRM_K_MEMTERM,
                       (double)span * h->n_nodes * sizeof(double) +
                           (double)ntargets * h->n_nodes * sizeof(double));
     if (ntargets == 1)
         hipLaunchKernelGGL(k_memterm_f64<1>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, h->part);
+                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, part);
     else
         hipLaunchKernelGGL(k_memterm_f64<2>, grid, dim3(threads), 0, ctx->stream,
-                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, h->part);
+                           (const double*)h->data, h->ld, depth, first, d_w, ldw, chunk, part);
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
 }
@@ -330,6 +369,44 @@ int frm_memterm(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double*
     hipLaunchKernelGGL(k_memterm_reduce, grid, dim3(threads), 0, ctx->stream,
                        (const double*)h->part, h->ld, nchunks, desc->ntargets, h->n_nodes,
                        desc->scale, desc->accumulate, d_out);
+    FRM_HIP(ctx, hipGetLastError());
+    return FRM_OK;
+}
+
+// frm_memterm with the partial sums in this device's static array instead of
+// the store's workspace; allocates nothing and records no timing events.
+// Every call on one GPU shares the array (each call writes every partial it
+// reads before reading it, so nothing needs clearing): calls that overlap
+// on one GPU, from different streams or contexts, race.
+int frm_memterm_static(frm_ctx* ctx, frm_hist* h, const frm_memterm_desc* desc, double* d_out) {
+    FRM_REQUIRE(ctx, ctx && h && desc && d_out, "frm_memterm_static: null argument");
+    FRM_REQUIRE(ctx, desc->ntargets == 1 || desc->ntargets == 2, "frm_memterm_static: ntargets %d",
+                desc->ntargets);
+    FRM_REQUIRE(ctx, h->depth == 0 || (desc->d_w && desc->ldw >= h->depth),
+                "frm_memterm_static: weights shorter than the history depth");
+    FRM_REQUIRE(ctx, h->precision == FRM_PREC_F64, "frm_memterm_static: precision %d",
+                h->precision);
+    if (!ctx->static_scratch)
+        return set_error(ctx, FRM_E_STATE, "frm_memterm_static: no static workspace on device %d",
+                         ctx->device);
+    const int nch = memterm_chunks(h);
+    const int64_t need = (int64_t)nch * desc->ntargets * h->ld;
+    if (need > kStaticScratchDoubles)
+        return set_error(ctx, FRM_E_RANGE,
+                         "frm_memterm_static: %d time chunks x %d targets x %lld padded nodes need "
+                         "%lld partial sums; the static workspace holds %lld (16 MiB): use "
+                         "frm_memterm",
+                         nch, desc->ntargets, (long long)h->ld, (long long)need,
+                         (long long)kStaticScratchDoubles);
+    int nchunks = 0;
+    int rc = memterm_partials(ctx, h, desc->ntargets, desc->d_w, desc->ldw, &nchunks, 0, nullptr,
+                              ctx->static_scratch);
+    if (rc) return rc;
+    const int threads = 256;
+    dim3 grid((unsigned)((h->n_nodes + threads - 1) / threads), (unsigned)desc->ntargets);
+    hipLaunchKernelGGL(k_memterm_reduce, grid, dim3(threads), 0, ctx->stream,
+                       (const double*)ctx->static_scratch, h->ld, nchunks, desc->ntargets,
+                       h->n_nodes, desc->scale, desc->accumulate, d_out);
     FRM_HIP(ctx, hipGetLastError());
     return FRM_OK;
 }

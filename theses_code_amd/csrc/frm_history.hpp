@@ -27,9 +27,14 @@ int hist_alloc(frm_ctx* ctx, int64_t n_nodes, int32_t capacity, int32_t precisio
 // Launches the split-K partial sums for the current depth into h->part,
 // over slots first .. depth-1 (the newest depth - first lags); returns the
 // number of chunks written (0 when there is nothing to sum).  An FP32 store
-// (FRM_PREC_F32C) takes one target and its FP32 weights in d_w32.
+// (FRM_PREC_F32C) takes one target and its FP32 weights in d_w32.  With
+// `part` given, the partials go there instead (nchunks * ntargets * ld
+// doubles, see memterm_chunks) and no timing events are recorded.
 int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w, int ldw,
-                     int* nchunks, int first = 0, const float* d_w32 = nullptr);
+                     int* nchunks, int first = 0, const float* d_w32 = nullptr,
+                     double* part = nullptr);
+// Time chunks memterm_partials splits slots first .. depth-1 into.
+int memterm_chunks(const frm_hist* h, int first = 0);
 // out[i] = sum_c part[c*ld + i] for one target (chunk order, deterministic)
 int memterm_reduce1(frm_ctx* ctx, const frm_hist* h, int nchunks, double* d_out);
 

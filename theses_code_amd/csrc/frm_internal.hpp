@@ -31,11 +31,16 @@ struct frm_ctx {
     int64_t launches[FRM_K_COUNT] = {};
     double total_ms[FRM_K_COUNT] = {};
     double total_bytes[FRM_K_COUNT] = {};
+    double* static_scratch = nullptr;  // this device's copy of the static partial-sum array
 };
 
 namespace frm {
 
 int set_error(frm_ctx* ctx, int code, const char* fmt, ...);
+// Looks up this device's address of the static partial-sum array of
+// frm_memterm_static (once per device; the device must be current) and
+// stores it in ctx->static_scratch.
+int static_scratch_setup(frm_ctx* ctx);
 int check_hip(frm_ctx* ctx, hipError_t e, const char* what);
 
 // Brackets one kernel launch of class `kclass` with timing events when

@@ -109,6 +109,14 @@ int frm_ctx_create(int device, frm_ctx** out) {
         delete ctx;
         return rc;
     }
+    // this device's address of frm_memterm_static's array (device current)
+    int rc = frm::static_scratch_setup(ctx);
+    if (rc) {
+        (void)frm::set_error(nullptr, rc, "frm_ctx_create: %s", ctx->last_error.c_str());
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return FRM_OK;
 }
