@@ -193,6 +193,17 @@ typedef struct frm_lod2d_desc {
                              summation (dt_alg = 2, fr_d_t_g_2d.cpp:3356-3381): each
                              direction's history sum, newest first, stops at the
                              first weight below trunc_eps; 0: the full history */
+    int32_t g_func;       /* time function g of the kernel (g(t) - g(s))^-alpha:
+                             0 = t (default); 3 = t^g_pow (TG_solver_<3, ...>,
+                             fr_d_t_g_2d.cpp:2823-2834).  With 3 the weights are
+                             not Toeplitz and the current-step weights change every
+                             step: every step rebuilds the history weights on the
+                             GPU (_kb_row series, 2937-2963, int_alg 1) and the
+                             sweeps' Thomas coefficients; FRM_PREC_F64 and the
+                             Caputo kernel only */
+    double g_pow;         /* g_func 3: the power (0 < g_pow <= 4) */
+    double int_err;       /* g_func 3: the series' term tolerance (the reference's
+                             int_err; 0: 1e-6, its default, 5322) */
 } frm_lod2d_desc;
 int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* desc, frm_lod2d** out);
 int frm_lod2d_destroy(frm_lod2d* s);
@@ -211,7 +222,9 @@ int frm_lod2d_state_ptr(frm_lod2d* s, double** d_C);
 int frm_lod2d_fit_error(const frm_lod2d* s, double* max_rel);
 /* Host copies of the weights in use: out[0] = row current weight, out[1] =
  * column current weight, out[2] = half-step correction, then the row and
- * column history weights for lags 1..n (2 + 1 + 2n values). */
+ * column history weights for lags 1..n (2 + 1 + 2n values).  With g_func 3
+ * the weights change every step: those of step n, computed on the host as
+ * the step computes them (lag k multiplies the increment of step n - k). */
 int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out);
 /* Launch shapes of the sweeps in use: out[0..3] = elements per lane and
  * wavefronts per line of the row sweep, then of the column sweep. */

@@ -38,6 +38,14 @@ int frm_host_ab_weights(double a, double tau, int32_t maxl, int32_t maxkiter, do
  * (3 + 2D values). */
 int frm_host_caputo_lod_weights(double alpha, double tau, int32_t D, double* out);
 
+/* Caputo LOD weights of step n for g(t) = t^p (TG_solver_<3, ...>::kb,
+ * int_alg 1: the _kb_row series of fr_d_t_g_2d.cpp:2937-2963 to int_err):
+ * the same 3 + 2n values as frm_host_caputo_lod_weights for step n, lag k
+ * multiplying the increment of step n - k (the weights change every step).
+ * FRM_E_STATE when a series reaches its safety bound. */
+int frm_host_lod_weights_pow(double alpha, double tau, double p, double int_err, int32_t n,
+                             double* out);
+
 /* Sum-of-exponentials fit of lag-indexed weights w[t*K + k-1] (targets
  * t < ntargets, lags k = k0..K) by L decays q_l = exp(-r_l tau) with
  * log-spaced rates and FP64 least squares in relative weighting:

@@ -184,3 +184,53 @@ def test_lod2d_restricted_sum_matches_oracle(ctx, sN, M, alpha, steps, eps):
         g.calc_step(chunk)
         assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
     g.close()
+
+
+# g(t) = t^p (TG_solver_<3, ...>): weights rebuilt every step on the GPU,
+# Thomas coefficients rebuilt every step for the changing current weights
+def test_lod2d_gpow_matches_reference_golden(ctx, golden):
+    m, z = golden("tg_psi")
+    dL, dLx = S.tg_ref_spacing(m["sN"], m["M"])
+    coefs = S.coefs_ref(z["V2"], m["dv"], dL, dLx)
+    s = S.TGSolver(ctx, m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], coefs, z["C0"],
+                   capacity=m["nsteps"], C1=m["C1"], S=m["S"], g_func=3, g_pow=m["fpow"],
+                   int_err=m["int_err"])
+    ks = [k for k in range(1, m["nsteps"] + 1) if k % m["every"] == 0 or k == m["nsteps"]]
+    t = 0
+    for q, k in enumerate(ks):
+        s.calc_step(k - t)
+        t = k
+        assert relerr(s.b_C, z["snaps"][q]) <= TOL, k
+    # the solver's weights of the last step are the reference's kb table
+    n = m["nsteps"]
+    kr, kc, kh, wr, wc = s.weights(n)
+    kb = z["kb"]
+    np.testing.assert_allclose([kr, kc, kh], [kb[0, n], kb[3, n], kb[1, n]], rtol=1e-14)
+    np.testing.assert_allclose(wr, kb[4, n - 1 - np.arange(n)], rtol=1e-14)
+    s.close()
+
+
+@pytest.mark.parametrize("p,alpha,sN,M,steps", [(0.7, 0.6, 40, 33, 120), (1.3, 0.8, 70, 50, 90),
+                                                (0.5, 0.7, 129, 65, 60)])
+def test_lod2d_gpow_matches_oracle(ctx, p, alpha, sN, M, steps):
+    rng = np.random.default_rng(int(p * 100) + sN)
+    coefs, C0 = _adv_case(rng, sN, M, D=2e-3)
+    tau, sigma, C1, Ssrc, ie = 1e-3, 1.0, 0.2, 0.05, 1e-7
+    o = O.TGSolver(sN, M, alpha, tau, sigma, C1, Ssrc, *coefs, C0, func=3, fpow=p, int_err=ie)
+    g = S.TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=steps, C1=C1, S=Ssrc,
+                   g_func=3, g_pow=p, int_err=ie)
+    for chunk in (1, steps // 2 - 1, steps - steps // 2):
+        o.step(chunk)
+        g.calc_step(chunk)
+        assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    g.close()
+
+
+def test_lod2d_gpow_rejects_other_forms(ctx):
+    coefs, C0 = _adv_case(np.random.default_rng(1), 20, 20)
+    for kw in (dict(precision=N.PREC_SOE), dict(precision=N.PREC_MFMA_BF16),
+               dict(kernel=N.KERNEL_AB), dict(trunc_eps=1e-3), dict(g_pow=0.0)):
+        args = dict(g_func=3, g_pow=0.7)
+        args.update(kw)
+        with pytest.raises(N.FrmError):
+            S.TGSolver(ctx, 20, 20, 0.6, 1e-3, 1.0, coefs, C0, capacity=10, **args)

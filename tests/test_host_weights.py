@@ -121,3 +121,37 @@ def test_soe_fit_rejects_bad_arguments():
     w = np.ones((1, 10))
     with pytest.raises(RuntimeError):
         N.host_soe_fit(w, 0, 8, 1e-3)  # lag 0 does not exist
+
+
+def test_lod_weights_pow_match_the_reference_kb_table(golden):
+    # g(t) = t^p: the step-n weights against the kb table the reference
+    # itself wrote at step nsteps (tg_psi golden; rows kb(s,1/2,0,1) = variant
+    # 4, columns kb(s,1,0,1) = 5, currents variants 0, 3 and 1)
+    m, z = golden("tg_psi")
+    n = m["nsteps"]
+    kr, kc, kh, wr, wc = N.host_lod_weights_pow(m["alpha"], m["tau"], m["fpow"], m["int_err"], n)
+    kb = z["kb"]
+    assert kr == pytest.approx(kb[0, n], rel=1e-14)
+    assert kc == pytest.approx(kb[3, n], rel=1e-14)
+    assert kh == pytest.approx(kb[1, n], rel=1e-14)
+    s = n - 1 - np.arange(n)  # lag k+1 multiplies step s
+    np.testing.assert_allclose(wr, kb[4, s], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(wc, kb[5, s], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("p,alpha,n", [(0.7, 0.6, 40), (1.3, 0.8, 25), (1.0, 0.5, 30)])
+def test_lod_weights_pow_match_the_oracle(p, alpha, n):
+    from oracle import oracle as O
+
+    tau, ie = 1e-3, 1e-7
+    kr, kc, kh, wr, wc = N.host_lod_weights_pow(alpha, tau, p, ie, n)
+    kb = O.tg_kb_table(3, p, alpha, tau, n, ie)
+    s = n - 1 - np.arange(n)
+    np.testing.assert_allclose([kr, kc, kh], [kb[0, n], kb[3, n], kb[1, n]], rtol=1e-14)
+    np.testing.assert_allclose(wr, kb[4, s], rtol=1e-14)
+    np.testing.assert_allclose(wc, kb[5, s], rtol=1e-14)
+    if p == 1.0:
+        # g(t) = t: the series is the closed form of the Caputo weights
+        _, _, _, cr, cc = N.host_caputo_lod_weights(alpha, tau, n)
+        np.testing.assert_allclose(wr, cr, rtol=1e-12)
+        np.testing.assert_allclose(wc, cc, rtol=1e-12)

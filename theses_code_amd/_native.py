@@ -44,7 +44,8 @@ class Lod2dDesc(ct.Structure):
                 ("kernel", ct.c_int32), ("ab_maxl", ct.c_int32), ("ab_maxkiter", ct.c_int32),
                 ("ab_eps", _D), ("ab_exact", ct.c_int32), ("soe_terms", ct.c_int32),
                 ("soe_f32", ct.c_int32), ("mfma_parts", ct.c_int32), ("sweep_w", ct.c_int32),
-                ("series_terms", ct.c_int32), ("series_tol", _D), ("trunc_eps", _D)]
+                ("series_terms", ct.c_int32), ("series_tol", _D), ("trunc_eps", _D),
+                ("g_func", ct.c_int32), ("g_pow", _D), ("int_err", _D)]
 
 
 class MT1DProblem(ct.Structure):
@@ -89,6 +90,7 @@ SIGNATURES = {
     "frm_host_ab_weights": (_I, [_D, _D, ct.c_int32, ct.c_int32, _D, ct.c_int32, ct.c_int32, _P]),
     "frm_host_caputo_lod_weights": (_I, [_D, _D, ct.c_int32, _P]),
     "frm_host_soe_fit": (_I, [_P, ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, _D, _P, _P, _P]),
+    "frm_host_lod_weights_pow": (_I, [_D, _D, _D, _D, ct.c_int32, _P]),
     "frm_ctx_create": (_I, [_I, ct.POINTER(_V)]),
     "frm_ctx_destroy": (_I, [_V]),
     "frm_sync": (_I, [_V]),
@@ -302,6 +304,14 @@ def host_caputo_lod_weights(alpha, tau, D):
     out = np.empty(3 + 2 * D)
     _check(lib().frm_host_caputo_lod_weights(alpha, tau, D, _dp(out)))
     return out[0], out[1], out[2], out[3:3 + D], out[3 + D:]
+
+
+def host_lod_weights_pow(alpha, tau, p, int_err, n):
+    """Step-n LOD weights for g(t) = t^p: (row current, column current,
+    half-step correction, row weights, column weights), lags 1..n."""
+    out = np.empty(3 + 2 * n)
+    _check(lib().frm_host_lod_weights_pow(alpha, tau, p, int_err, n, _dp(out)))
+    return out[0], out[1], out[2], out[3:3 + n], out[3 + n:]
 
 
 def host_soe_fit(w: np.ndarray, k0: int, L: int, tau: float):

@@ -4,6 +4,7 @@
 
 #include "frm.h"
 #include "frm_host.h"
+#include "frm_gpow.hpp"
 #include "frm_soe.hpp"
 #include "frm_weights.hpp"
 
@@ -55,6 +56,23 @@ int frm_host_caputo_lod_weights(double alpha, double tau, int32_t D, double* out
         out[3 + D + k - 1] = caputo_kb(alpha, tau, k + 1.0, 1.0);
     }
     return FRM_OK;
+}
+
+int frm_host_lod_weights_pow(double alpha, double tau, double p, double int_err, int32_t n,
+                             double* out) {
+    if (!out || n < 0 || !(alpha > 0 && alpha < 1) || !(tau > 0) || !(p > 0 && p <= 4) ||
+        !(int_err > 0))
+        return FRM_E_INVALID;
+    const double Da = 1.0 / gamma_cody(1.0 - alpha);
+    bool capped = false;
+    out[0] = Da * kb_pow(alpha, tau, p, int_err, n, n, 0.5, 0.0, 0.5, &capped);
+    out[1] = Da * kb_pow(alpha, tau, p, int_err, n, n, 1.0, 0.5, 0.5, &capped);
+    out[2] = Da * kb_pow(alpha, tau, p, int_err, n, n, 1.0, 0.0, 0.5, &capped);
+    for (int k = 0; k < n; ++k) {
+        out[3 + k] = Da * kb_pow(alpha, tau, p, int_err, n, n - 1 - k, 0.5, 0.0, 1.0, &capped);
+        out[3 + n + k] = Da * kb_pow(alpha, tau, p, int_err, n, n - 1 - k, 1.0, 0.0, 1.0, &capped);
+    }
+    return capped ? FRM_E_STATE : FRM_OK;
 }
 
 int frm_host_soe_fit(const double* w, int32_t ntargets, int32_t K, int32_t k0, int32_t L,

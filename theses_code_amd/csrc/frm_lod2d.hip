@@ -27,6 +27,7 @@
 #include <cstring>
 #include <vector>
 
+#include "frm_gpow.hpp"
 #include "frm_history.hpp"
 #include "frm_stream.hpp"
 #include "frm_mfma.hpp"
@@ -85,6 +86,12 @@ struct frm_lod2d {
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
     float* F = nullptr;        // [128][ld]
     double* G = nullptr;       // [16][ld] mid part of the current 8-step sub-block
+    // g(t) = t^p (g_func 3): the weights and the sweeps' Thomas coefficients
+    // are rebuilt every step
+    int g_func = 0;
+    double g_pow = 1.0, int_err = 1e-6, Da = 0.0;
+    double* lineAB = nullptr;  // off-diagonals per line: rows A, B [M-1][NX], cols A, B [sN-1][NY]
+    int* flag = nullptr;       // set when a weight series reached its safety bound
 };
 
 namespace frm {
@@ -341,6 +348,52 @@ __global__ __launch_bounds__((col_width<CM, W>() * kWave * W)) void k_lod_cols(C
                     p.slot[q] = d;
             }
         }
+    }
+}
+
+// ------------------------------------------------------------- g(t) = t^p
+// History weights of step n, lag-indexed (w[t*cap + k] multiplies the
+// increment of full step s = n-1-k): rows kb(s, 1/2, 0, 1), columns
+// kb(s, 1, 0, 1), each by the _kb_row series (frm_gpow.hpp), one thread per
+// weight; the reference rebuilds them for every s <= n each step
+// (fr_d_t_g_2d.cpp:3059-3239).
+__global__ __launch_bounds__(256) void k_gpow_weights(int n, double alpha, double tau, double p,
+                                                      double int_err, double Da, int cap,
+                                                      double* __restrict__ w,
+                                                      int* __restrict__ flag) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= 2 * n) return;
+    const int t = idx / n, k = idx % n;
+    bool capped = false;
+    const double v = kb_pow(alpha, tau, p, int_err, n, n - 1 - k, t ? 1.0 : 0.5, 0.0, 1.0, &capped);
+    w[(int64_t)t * cap + k] = Da * v;
+    if (capped) atomicOr(flag, 1);
+}
+
+// Thomas alpha coefficients (a, f, al) of every line for the diagonal
+// sgk + A + B, in the reference's order (al1, fr_d_t_g_2d.cpp:3267-3304) --
+// line_coefs on the GPU, one thread per line, written in the interleaved
+// layout of load_coefs for chunk length cm and nl lanes per line.
+__global__ __launch_bounds__(64) void k_line_coefs(const double* __restrict__ A,
+                                                   const double* __restrict__ B, int lines, int n,
+                                                   double sgk, double al_first, int cm, int nl,
+                                                   double* __restrict__ out) {
+    const int L = blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= lines) return;
+    const double* a = A + (int64_t)L * n;
+    const double* b = B + (int64_t)L * n;
+    double Al = al_first;
+    for (int e = 0; e < n; ++e) {
+        const double ae = a[e], be = b[e];
+        const double R = sgk + ae + be;
+        const double den = R - ae * Al;
+        const double Aln = be / den;
+        const double f = (Aln == 0.0 && be == 0.0) ? 1.0 / den : Aln / be;
+        const int g = e / cm, k = e % cm;
+        out[(((int64_t)L * 3 + 0) * cm + k) * nl + g] = ae;
+        out[(((int64_t)L * 3 + 1) * cm + k) * nl + g] = f;
+        out[(((int64_t)L * 3 + 2) * cm + k) * nl + g] = Aln;
+        Al = Aln;
     }
 }
 
@@ -641,6 +694,45 @@ static int tune_sweeps(frm_lod2d* s, int force) {
     return rc;
 }
 
+// g(t) = t^p: the current-step weights of step n on the host -- rows
+// kb(n, 1/2, 0, 1/2), columns kb(n, 1, 1/2, 1/2), the column half-step term
+// kb(n, 1, 0, 1/2) -- as TG_solver::kb computes them (int_alg 1)
+static void gpow_current(const frm_lod2d* s, int n, double* kr, double* kc, double* kh,
+                         bool* capped) {
+    const double a = s->alpha, t = s->tau, p = s->g_pow, ie = s->int_err;
+    *kr = s->Da * kb_pow(a, t, p, ie, n, n, 0.5, 0.0, 0.5, capped);
+    *kc = s->Da * kb_pow(a, t, p, ie, n, n, 1.0, 0.5, 0.5, capped);
+    *kh = s->Da * kb_pow(a, t, p, ie, n, n, 1.0, 0.0, 0.5, capped);
+}
+
+// g(t) = t^p: the history weights of step n (GPU) and both directions'
+// Thomas coefficients for its current weights
+static int gpow_prepare(frm_lod2d* s, int n) {
+    frm_ctx* ctx = s->ctx;
+    bool capped = false;
+    gpow_current(s, n, &s->kr0, &s->kc0, &s->kch, &capped);
+    if (capped)
+        return set_error(ctx, FRM_E_STATE, "frm_lod2d_step: kb series did not reach int_err at step %d", n);
+    if (n > 0) {
+        LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+        hipLaunchKernelGGL(k_gpow_weights, dim3((unsigned)((2 * n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, n, s->alpha, s->tau, s->g_pow, s->int_err, s->Da,
+                           s->capacity, s->w, s->flag);
+        FRM_HIP(ctx, hipGetLastError());
+    }
+    const int64_t nr = (int64_t)(s->M - 1) * s->NX, nc = (int64_t)(s->sN - 1) * s->NY;
+    LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+    hipLaunchKernelGGL(k_line_coefs, dim3((unsigned)((s->M - 1 + 63) / 64)), dim3(64), 0,
+                       ctx->stream, s->lineAB, s->lineAB + nr, s->M - 1, s->NX, s->sg * s->kr0,
+                       0.0, s->rsh.cm, kWave * s->rsh.w, s->rcoef);
+    FRM_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_line_coefs, dim3((unsigned)((s->sN - 1 + 63) / 64)), dim3(64), 0,
+                       ctx->stream, s->lineAB + 2 * nr, s->lineAB + 2 * nr + nc, s->sN - 1, s->NY,
+                       s->sg * s->kc0, 1.0, s->csh.cm, kWave * s->csh.w, s->ccoef);
+    FRM_HIP(ctx, hipGetLastError());
+    return FRM_OK;
+}
+
 static void lod_free(frm_lod2d* s) {
     if (!s) return;
     if (s->hist) frm_hist_destroy(s->hist);
@@ -660,6 +752,8 @@ static void lod_free(frm_lod2d* s) {
     (void)hipFree(s->atab);
     (void)hipFree(s->F);
     (void)hipFree(s->G);
+    (void)hipFree(s->lineAB);
+    (void)hipFree(s->flag);
     delete s;
 }
 
@@ -698,6 +792,14 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: mfma_parts %d (2 or 3)", d->mfma_parts);
     FRM_REQUIRE(ctx, d->kernel == FRM_KERNEL_CAPUTO || d->kernel == FRM_KERNEL_AB,
                 "frm_lod2d_create: kernel %d", d->kernel);
+    FRM_REQUIRE(ctx, d->g_func == 0 || d->g_func == 3, "frm_lod2d_create: g_func %d (0 or 3)",
+                d->g_func);
+    FRM_REQUIRE(ctx,
+                d->g_func == 0 ||
+                    (d->precision == FRM_PREC_F64 && d->kernel == FRM_KERNEL_CAPUTO &&
+                     d->trunc_eps == 0 && d->g_pow > 0 && d->g_pow <= 4 && d->int_err >= 0),
+                "frm_lod2d_create: g_func 3 takes FRM_PREC_F64, the Caputo kernel, the full "
+                "history, 0 < g_pow <= 4 and int_err >= 0");
     FRM_REQUIRE(ctx, pick_cm(d->sN - 1, 4) && pick_cm(d->M - 1, 4),
                 "frm_lod2d_create: lines longer than %d unknowns", 4 * 64 * 32);
     FRM_REQUIRE(ctx, d->sweep_w == 0 || d->sweep_w == 1 || d->sweep_w == 2 || d->sweep_w == 4,
@@ -720,6 +822,10 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     s->C1 = d->C1;
     s->S = d->S;
     s->sg = d->sigma / d->tau;
+    s->g_func = d->g_func;
+    s->g_pow = d->g_func == 3 ? d->g_pow : 1.0;
+    s->int_err = d->int_err > 0 ? d->int_err : 1e-6;
+    s->Da = 1.0 / frm::gamma_cody(1.0 - d->alpha);
     // Caputo: the kernel's integrals over the LOD sub-intervals, as
     // TG_solver::kb (rows target t_{n+1/2}, columns t_{n+1}).
     // Atangana-Baleanu: the kernel is bounded, so a half-interval current
@@ -745,6 +851,14 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     s->kr0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
     s->kc0 = is_ab ? abI(1) : caputo_kb(d->alpha, d->tau, 0.5, 0.5);
     s->kch = is_ab ? 0.0 : caputo_kb(d->alpha, d->tau, 1.0, 0.5);
+    if (s->g_func == 3) {
+        bool capped = false;
+        gpow_current(s, 0, &s->kr0, &s->kc0, &s->kch, &capped);
+        if (capped) {
+            lod_free(s);
+            return set_error(ctx, FRM_E_INVALID, "frm_lod2d_create: kb series did not reach int_err");
+        }
+    }
     const int64_t nodes = (int64_t)s->NX * s->NY;
     const size_t nn = (size_t)s->ld * (d->M + 2);
     s->precision = d->precision;
@@ -779,7 +893,7 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     // lag-indexed history weights: rows kb(s,1/2,0,1) at lag k = n-s is
     // d = k+1/2, l = 1; columns kb(s,1,0,1) is d = k+1, l = 1
     std::vector<double> w(2 * (size_t)d->capacity);
-    for (int k = 1; k <= d->capacity; ++k) {
+    for (int k = 1; k <= d->capacity && s->g_func == 0; ++k) {
         if (is_ab) {
             w[k - 1] = w[d->capacity + k - 1] = 0.5 * abI(k + 1);
         } else {
@@ -817,6 +931,28 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         double* oa = cc3.data() + (int64_t)(i - 1) * s->NY;
         line_coefs(s->NY, d->colA + base, d->colB + base, s->ld, s->sg * s->kc0, 1.0, oa,
                    oa + lcols * s->NY, oa + 2 * lcols * s->NY);
+    }
+    if (s->g_func == 3) {
+        // the lines' off-diagonals for the per-step coefficient kernel
+        const int64_t nr = lrows * s->NX, nc = lcols * s->NY;
+        std::vector<double> ab((size_t)2 * (nr + nc));
+        for (int j = 1; j < d->M; ++j)
+            for (int i = 1; i < d->sN; ++i) {
+                const int64_t q = (int64_t)j * s->ld + i;
+                ab[(j - 1) * s->NX + (i - 1)] = d->rowA[q];
+                ab[nr + (j - 1) * s->NX + (i - 1)] = d->rowB[q];
+                ab[2 * nr + (i - 1) * s->NY + (j - 1)] = d->colA[q];
+                ab[2 * nr + nc + (i - 1) * s->NY + (j - 1)] = d->colB[q];
+            }
+        e = hipMalloc(&s->lineAB, ab.size() * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&s->flag, sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpy(s->lineAB, ab.data(), ab.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(s->flag, 0, sizeof(int));
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMalloc(lod2d g = t^p)");
+        }
     }
     s->rplain = std::move(rc3);
     s->cplain = std::move(cc3);
@@ -973,6 +1109,10 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     for (int it = 0; it < nsteps; ++it) {
         const int n = s->tstep;
         int nch = 0;
+        if (s->g_func == 3) {
+            const int rc = gpow_prepare(s, n);
+            if (rc) return rc;
+        }
         if (soe) {
             nch = 1;
             const int64_t ld = s->hist->ld;
@@ -1049,7 +1189,13 @@ int frm_lod2d_get_state(frm_lod2d* s, double* h_C) {
     frm_ctx* ctx = s->ctx;
     FRM_HIP(ctx, hipMemcpyAsync(h_C, s->C, (size_t)s->ld * (s->M + 2) * sizeof(double),
                                 hipMemcpyDeviceToHost, ctx->stream));
+    int flag = 0;
+    if (s->flag)
+        FRM_HIP(ctx, hipMemcpyAsync(&flag, s->flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (flag)
+        return set_error(ctx, FRM_E_STATE,
+                         "frm_lod2d_get_state: a history-weight series did not reach int_err");
     return FRM_OK;
 }
 
@@ -1068,6 +1214,18 @@ int frm_lod2d_fit_error(const frm_lod2d* s, double* max_rel) {
 int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out) {
     if (!s || !out || n < 0 || n > s->capacity)
         return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_weights: bad arguments");
+    if (s->g_func == 3) {
+        // the weights of step n, computed on the host as the step computes them
+        bool capped = false;
+        gpow_current(s, n, &out[0], &out[1], &out[2], &capped);
+        for (int k = 0; k < n; ++k) {
+            out[3 + k] = s->Da * kb_pow(s->alpha, s->tau, s->g_pow, s->int_err, n, n - 1 - k, 0.5,
+                                        0.0, 1.0, &capped);
+            out[3 + n + k] = s->Da * kb_pow(s->alpha, s->tau, s->g_pow, s->int_err, n, n - 1 - k,
+                                            1.0, 0.0, 1.0, &capped);
+        }
+        return capped ? set_error(nullptr, FRM_E_STATE, "frm_lod2d_weights: series bound") : FRM_OK;
+    }
     out[0] = s->hw[0];
     out[1] = s->hw[1];
     out[2] = s->hw[2];

@@ -48,21 +48,23 @@ def coefs_advection(vx: np.ndarray, vy: np.ndarray, D: float, h: float):
 
 
 class TGSolver:
-    """TG_solver::calc_step on the MI355X (dt_alg = 0, g(t) = t)."""
+    """TG_solver::calc_step on the MI355X (dt_alg = 0; g(t) = t, or t^g_pow
+    with g_func = 3)."""
 
     def __init__(self, ctx: N.Context, sN: int, M: int, alpha: float, tau: float, sigma: float,
                  coefs, C0: np.ndarray, capacity: int, C1: float = 1.0, S: float = 0.0,
                  precision: int = N.PREC_F64, kernel: int = N.KERNEL_CAPUTO, ab=(50, 1000, 1e-10),
                  ab_exact: bool = False, soe_terms: int = 32, soe_f32: bool = False,
                  mfma_parts: int = 0, sweep_w: int = 0, series_terms: int = 0,
-                 series_tol: float = 0.0, trunc_eps: float = 0.0):
+                 series_tol: float = 0.0, trunc_eps: float = 0.0, g_func: int = 0,
+                 g_pow: float = 1.0, int_err: float = 0.0):
         self.ctx, self.sN, self.M = ctx, sN, M
         nn = (sN + 2) * (M + 2)
         self._coefs = [np.ascontiguousarray(c, dtype=np.float64).reshape(nn) for c in coefs]
         d = N.Lod2dDesc(sN, M, capacity, precision, alpha, tau, sigma, C1, S,
                         *[N._dp(c) for c in self._coefs], kernel, ab[0], ab[1], ab[2],
                         int(ab_exact), soe_terms, int(soe_f32), mfma_parts, sweep_w,
-                        series_terms, series_tol, trunc_eps)
+                        series_terms, series_tol, trunc_eps, g_func, g_pow, int_err)
         self.capacity = capacity
         h = N._V()
         N._check(N.lib().frm_lod2d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
