@@ -132,6 +132,13 @@ double orc_tg_kb(int func, double p, double alpha, double tau, double int_err, i
     return 0.0;
 }
 
+/* dt_alg 4 approximation lists (see the dt_alg 4 section below) */
+#define DT4_AK 10
+#define DT4_MAXAK 20
+typedef struct { double min; int nterms; double c[DT4_MAXAK]; } dt4_ia;
+typedef struct { double min, max, b; int nterms, old; double c[DT4_MAXAK]; } dt4_ibk;
+typedef struct dt4_apps { dt4_ia* ia; int nia, cia; dt4_ibk* ibk; int nibk, cibk; } dt4_apps;
+
 /* forward declaration from the AB section below */
 static double orc_tg_ab_weight(orc_tg* t, double d, double l);
 
@@ -158,6 +165,9 @@ struct orc_tg {
     double *H1, *H2, *r2;
     int *r2n;
     int cap_r2;
+    /* dt_alg 4: per node and direction the ia/ibk approximation lists */
+    int dt4;
+    struct dt4_apps* app4; /* [2][nn] */
 };
 
 static double* dup_arr(const double* src, size_t n) {
@@ -214,6 +224,11 @@ void orc_tg_destroy(orc_tg* t) {
     free(t->ab_w);
     if (t->ab_cache) { orc_mf_cache_free(t->ab_cache); free(t->ab_cache); }
     free(t->H1); free(t->H2); free(t->r2); free(t->r2n);
+    if (t->app4) {
+        size_t nn = (size_t)t->ld * (t->M + 2);
+        for (size_t q = 0; q < 2 * nn; ++q) { free(t->app4[q].ia); free(t->app4[q].ibk); }
+        free(t->app4);
+    }
     free(t);
 }
 
@@ -320,6 +335,274 @@ void orc_tg_set_dt3(orc_tg* t, int terms) {
     t->H2 = (double*)calloc(nn * terms, sizeof(double));
 }
 
+/* ---- dt_alg 4: "long" (ia) and "short" (ibk) series approximations of the
+ * history sum (fr_d_t_g_2d.cpp:246-741; used by Om1 at 3438-3469).  Each
+ * node keeps, per direction, a list of ia approximations -- expansions of
+ * g(x, r) = (x - r + a)^A - (x - r + b)^A in powers x^(A-k), valid for
+ * x >= min -- and of ibk approximations -- Taylor expansions in (x - b_c)
+ * around a centre b_c, valid on [min, max] -- whose coefficients collect the
+ * increments F[r] block by block (blocks of one step here, bs = 1, the only
+ * size Om1 uses); A = 1 - alpha, (a, b) = (3/2, 1/2) for rows and (2, 1) for
+ * columns.  Restated for constant steps (Ts = NULL, the path Om1 takes). */
+static const double dt4_approx_eps = 1e-7, dt4_eps = 1e-09;
+
+
+static void dt4_push_ia(dt4_apps* s, const dt4_ia* v) {
+    if (s->nia == s->cia) {
+        s->cia = s->cia ? 2 * s->cia : 8;
+        s->ia = (dt4_ia*)realloc(s->ia, s->cia * sizeof(dt4_ia));
+    }
+    s->ia[s->nia++] = *v;
+}
+static void dt4_push_ibk(dt4_apps* s, const dt4_ibk* v) {
+    if (s->nibk == s->cibk) {
+        s->cibk = s->cibk ? 2 * s->cibk : 16;
+        s->ibk = (dt4_ibk*)realloc(s->ibk, s->cibk * sizeof(dt4_ibk));
+    }
+    s->ibk[s->nibk++] = *v;
+}
+static void dt4_erase_ibk(dt4_apps* s, int j) {
+    memmove(s->ibk + j, s->ibk + j + 1, (s->nibk - j - 1) * sizeof(dt4_ibk));
+    s->nibk--;
+}
+
+/* ia_mini (262-275): x beyond which the AK-th ia term is below approx_eps */
+static double dt4_ia_mini(double A, double r, int nterms, double a, double b) {
+    double ak = 1;
+    for (int i = 1; i < nterms; i++) ak *= (A - i + 1) / i;
+    return pow(fabs(ak * (pow(a - r, (double)nterms) - pow(b - r, (double)nterms)) / dt4_approx_eps),
+               1.0 / (nterms - A));
+}
+/* ia_approximation(A, rmin, rmax, F, NULL, -1, nt, a, b) (287-325) */
+static void dt4_ia_new(dt4_ia* o, double A, double rmin, double rmax, double Fv, int nt,
+                       double a, double b) {
+    double ak = 1, krk;
+    o->c[0] = 0;
+    for (int i = 1; i < nt; i++) {
+        ak *= (A - i + 1) / i;
+        krk = 0.0;
+        /* blocks of one step: j = rmin only, F[j] = Fv */
+        for (int j = rmin; j < rmax; j++) krk += Fv * (pow(a - (double)j, (double)i) - pow(b - (double)j, (double)i));
+        o->c[i] = ak * krk;
+    }
+    for (int i = nt; i < DT4_MAXAK; i++) o->c[i] = 0.0;
+    o->min = ceil(dt4_ia_mini(A, rmax, nt, a, b));
+    if (!isfinite(o->min)) o->min = rmax;
+    o->nterms = nt;
+}
+static double dt4_ia_calc(const dt4_ia* o, double A, double x) {
+    double ret = 0.0;
+    for (int i = 0; i < o->nterms; i++) ret += o->c[i] * pow(x, A - (double)i);
+    return ret;
+}
+/* ibk_i2 (352-384, Ts = NULL): half-width around b where the ibk series
+ * with nterms terms stays below approx_eps */
+static double dt4_ibk_i2(double A, double r, double b, int nterms, double a, double c) {
+    double ak = 1;
+    for (int i = 1; i < nterms; i++) ak *= (A - i + 1) / i;
+    if (fabs(b - r) < 1e-10) return pow(fabs(dt4_approx_eps / ak), 1.0 / nterms);
+    double p1 = pow((a + b - r), A) / pow((a + b - r), nterms);
+    double p2 = pow((c + b - r), A) / pow((c + b - r), nterms);
+    return pow(fabs(dt4_approx_eps / (ak * (p1 - p2))), 1.0 / nterms);
+}
+/* ibk_approximation(A, rmin, rmax, F, NULL, bc, i2, nt, a, b) (402-437) */
+static void dt4_ibk_new(dt4_ibk* o, double A, double rmin, double rmax, double Fv, double bc,
+                        double i2, int nt, double a, double b) {
+    double ak = 1, krk;
+    o->b = bc;
+    for (int i = 0; i < nt; i++) {
+        if (i != 0) ak *= (A - i + 1) / i;
+        krk = 0.0;
+        for (int j = rmin; j < rmax; j++)
+            krk += Fv * (pow(a + bc - (double)j, A - (double)i) - pow(b + bc - (double)j, A - (double)i));
+        o->c[i] = ak * krk;
+    }
+    for (int i = nt; i < DT4_MAXAK; i++) o->c[i] = 0.0;
+    if (i2 != -1)
+        o->min = i2;
+    else
+        o->min = bc - floor(dt4_ibk_i2(A, rmax, bc, nt, a, b));
+    o->max = bc;
+    o->nterms = nt;
+    o->old = 0;
+}
+static double dt4_ibk_calc(const dt4_ibk* o, double x) {
+    double ret = 0.0, xbk = 1;
+    for (int i = 0; i < o->nterms; i++) {
+        ret += o->c[i] * xbk;
+        xbk *= (x - o->b);
+    }
+    return ret;
+}
+/* The centre of the next ibk of a series starting at rr: the fixed point
+ * bb = rr + ibk_i2(r, bb) + 1 (the loops at 581-592, 632-642, 708-718). */
+static double dt4_next_centre(double A, double rr, double r, double a, double b) {
+    double bb0 = rr, bb = rr;
+    bb = rr + 1;
+    do {
+        bb0 = bb;
+        bb = rr + dt4_ibk_i2(A, r, bb, DT4_AK, a, b) + 1;
+    } while ((bb - bb0) > dt4_eps);
+    if (!isfinite(bb)) {
+        bb = bb0;
+        if (!isfinite(bb)) bb = rr;
+    }
+    return bb;
+}
+
+/* calc_va(A, bs = 1, block, F, V, NULL, ias, ibks, a, b) (490-741): V = the
+ * approximated sum for r <= block + 1 at x = block + 1, then the block's
+ * increment Fv = F[block + 1] enters the approximations. */
+static void dt4_calc_va(double A, int block, double Fv, double* V, dt4_apps* ap, double _a,
+                        double _b) {
+    const int bs = 1;
+    double bm1 = -1;
+    double rmax = ((block + 1) * bs + 1);
+    int rmaxidx = rmax;
+    dt4_ibk bibk, i2;
+    for (int i = 0; i < bs; i++) V[i] = 0.0;
+    if (A == 0.0) return;
+    for (int i = block * bs + 1; i < (block + 1) * bs + 1; i++) {
+        double ii = i;
+        for (int j = 0; j < ap->nia; j++)
+            if (ii >= ap->ia[j].min) V[i - (block * bs + 1)] += dt4_ia_calc(&ap->ia[j], A, ii);
+        for (int j = 0; j < ap->nibk; j++)
+            if ((ii >= ap->ibk[j].min) && (ii <= ap->ibk[j].max))
+                V[i - (block * bs + 1)] += dt4_ibk_calc(&ap->ibk[j], ii);
+    }
+    for (int i = block * bs + 1; i < (block + 1) * bs + 1; i++) {
+        double v = 0.0;
+        for (int j = block * bs + 1; j <= i; j++)
+            v += (pow((double)i - (double)j + _a, A) - pow((double)i - (double)j + _b, A)) * Fv;
+        V[i - (block * bs + 1)] += v;
+    }
+    /* 1) remove ibks with max < rmax */
+    for (int j = 0; j < ap->nibk; j++)
+        if (ap->ibk[j].max < rmax) {
+            dt4_erase_ibk(ap, j);
+            j--;
+        }
+    /* 2) add the block to the ibks beyond rmax; split those whose range the
+     * block's own approximation does not cover */
+    int ss = ap->nibk;
+    for (int j = 0; j < ss; j++)
+        if (ap->ibk[j].min >= rmax)
+            if (ap->ibk[j].old == 0) {
+                int spl = 0;
+                dt4_ibk_new(&bibk, A, block * bs + 1, (block + 1) * bs + 1, Fv, ap->ibk[j].b, -1.0,
+                            DT4_AK, _a, _b);
+                if (bibk.min > ap->ibk[j].min) {
+                    int a;
+                    for (a = DT4_AK; a < DT4_MAXAK; a++)
+                        if ((ap->ibk[j].b - dt4_ibk_i2(A, rmaxidx, ap->ibk[j].b, a, _a, _b)) <
+                            ap->ibk[j].min)
+                            break;
+                    if (a == DT4_MAXAK)
+                        spl = 1;
+                    else
+                        dt4_ibk_new(&bibk, A, block * bs + 1, (block + 1) * bs + 1, Fv, ap->ibk[j].b,
+                                    -1.0, a, _a, _b);
+                    if (spl) i2 = ap->ibk[j];
+                }
+                for (int i = 0; i < bibk.nterms; i++) ap->ibk[j].c[i] += bibk.c[i];
+                if (ap->ibk[j].nterms < bibk.nterms) ap->ibk[j].nterms = bibk.nterms;
+                if (spl) {
+                    if (bibk.min > ap->ibk[j].min) {
+                        i2.max = ceil(bibk.min);
+                        ap->ibk[j].min = i2.max + 1;
+                        i2.old = 1;
+                        if (ap->ibk[j].min > ap->ibk[j].max) {
+                            i2.max = ap->ibk[j].max;
+                            dt4_erase_ibk(ap, j);
+                            j--;
+                            ss--;
+                        }
+                        dt4_push_ibk(ap, &i2);
+                        /* a series of ibks for [i2.min, i2.max] from this block */
+                        double rr = i2.min, bb;
+                        do {
+                            bb = dt4_next_centre(A, rr, i2.min, _a, _b);
+                            dt4_ibk_new(&bibk, A, block * bs + 1, (block + 1) * bs + 1, Fv, floor(bb),
+                                        rr, DT4_AK, _a, _b);
+                            rr = floor(bb) + 1;
+                            if (floor(bb) > i2.max) bibk.max = i2.max;
+                            dt4_push_ibk(ap, &bibk);
+                        } while (floor(bb) < i2.max);
+                    }
+                }
+            }
+    /* 3) new ibks from this block for the range of the ibk that straddles rmax */
+    for (int j = 0; j < ap->nibk; j++)
+        if (ap->ibk[j].max >= rmax)
+            if (ap->ibk[j].min < rmax)
+                if (ap->ibk[j].max > bm1)
+                    if (ap->ibk[j].old == 0) {
+                        bm1 = ap->ibk[j].max;
+                        ap->ibk[j].old = 1;
+                    }
+    if (bm1 != -1) {
+        for (int j = 0; j < ap->nibk; j++)
+            if (ap->ibk[j].max < bm1) ap->ibk[j].old = 1;
+        double rr = rmax, bb;
+        do {
+            bb = dt4_next_centre(A, rr, rmaxidx, _a, _b);
+            dt4_ibk_new(&bibk, A, block * bs + 1, (block + 1) * bs + 1, Fv, floor(bb), rr, DT4_AK,
+                        _a, _b);
+            rr = floor(bb) + 1;
+            if (floor(bb) > bm1) bibk.max = bm1;
+            dt4_push_ibk(ap, &bibk);
+        } while (floor(bb) < bm1);
+    }
+    /* 4) the block's ia joins the last ia if its validity starts earlier */
+    dt4_ia bia;
+    int have_bia = 1;
+    dt4_ia_new(&bia, A, block * bs + 1, (block + 1) * bs + 1, Fv, DT4_AK, _a, _b);
+    if (ap->nia)
+        if (bia.min < ap->ia[ap->nia - 1].min) {
+            dt4_ia* last = &ap->ia[ap->nia - 1];
+            for (int i = 0; i < bia.nterms; i++) last->c[i] += bia.c[i];
+            if (last->nterms < bia.nterms) last->nterms = bia.nterms;
+            have_bia = 0;
+        }
+    /* 5) otherwise ibks bridge the gap up to the new ia's minimum */
+    if (have_bia) {
+        bm1 = rmax;
+        for (int j = 0; j < ap->nibk; j++)
+            if ((ap->ibk[j].max + 1) > bm1) bm1 = ap->ibk[j].max + 1;
+        double rr = bm1, bb;
+        do {
+            bb = dt4_next_centre(A, rr, rmaxidx, _a, _b);
+            dt4_ibk_new(&bibk, A, block * bs + 1, (block + 1) * bs + 1, Fv, floor(bb), rr, DT4_AK,
+                        _a, _b);
+            dt4_push_ibk(ap, &bibk);
+            rr = floor(bb) + 1;
+        } while (floor(bb) < ceil(bia.min));
+        bia.min = rr;
+        dt4_push_ia(ap, &bia);
+    }
+}
+
+void orc_tg_set_dt4(orc_tg* t) {
+    size_t nn = (size_t)t->ld * (t->M + 2);
+    t->dt4 = 1;
+    t->app4 = (dt4_apps*)calloc(2 * nn, sizeof(dt4_apps));
+}
+
+/* dt_alg 4 memory term of node id at step n for direction dir (3438-3469,
+ * without the column half-step term): the calc_va call of the newest block
+ * n - 1 at x = n, scaled by Da tau^(1-a) / (1-a) */
+static double tg_dt4_sum(orc_tg* t, int dir, size_t id, int n) {
+    if (n < 1) return 0.0;
+    const size_t nn = (size_t)t->ld * (t->M + 2);
+    dt4_apps* ap = t->app4 + (size_t)dir * nn + id;
+    double ts;
+    /* F[block + 1] with block = n - 1: Delta_{n-1} = C_n - C_{n-1} */
+    const double Fv = t->old[2 * n][id] - t->old[2 * n - 2][id];
+    dt4_calc_va(1.0 - t->alpha, n - 1, Fv, &ts, ap, dir == 0 ? 1.5 : 2.0, dir == 0 ? 0.5 : 1.0);
+    ts *= (1.0 / orc_gamma(1.0 - t->alpha)) * (pow(t->tau, 1.0 - t->alpha) / (1.0 - t->alpha));
+    return ts;
+}
+
 /* weight variant index: bit0 ja==1, bit1 sa==1/2, bit2 l==1 (fr_d_t_g_2d.cpp:3077-3080) */
 #define TG_KB(t, v, s) ((t)->kb[(size_t)(v) * (t)->cap_kb + (s)])
 
@@ -372,7 +655,13 @@ static void tg_line(orc_tg* t, int dir, int line, int n) {
     for (int i = 1; i < N; ++i) {
         size_t id = NODE(i);
         double ts = 0;
-        if (t->nold >= 2) {
+        if (t->dt4) {
+            ts = tg_dt4_sum(t, dir, id, n);
+            if (dir == 1) { /* the column half-step term, as below */
+                double kv = TG_KB(t, 1, (t->nold - 1) / 2);
+                ts += kv * (t->old[t->nold - 2][id] - t->old[t->nold - 1][id]);
+            }
+        } else if (t->nold >= 2) {
             if (t->nold >= 3 && !t->dt3)
                 for (int q = ((t->nold - 3) / 2) * 2; q >= 0; q -= 2) {
                     /* full-step increments, newest first: kb(s,1/2,0,1) or kb(s,1,0,1) */

@@ -61,6 +61,9 @@ void orc_tg_set_ab(orc_tg* t, int maxl, int maxkiter, double eps);
 void orc_tg_set_dt3(orc_tg* t, int terms);
 /* dt_alg 2: the newest-first history sum stops at the first weight below eps */
 void orc_tg_set_dt2(orc_tg* t, double eps);
+/* dt_alg 4: the ia/ibk series approximations of the history per node and
+ * direction (fr_d_t_g_2d.cpp:246-741, 3438-3469), before the first step */
+void orc_tg_set_dt4(orc_tg* t);
 /* Grid spacings the reference constructor derives (fr_d_t_g_2d.cpp:2337-2388). */
 void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx);
 

@@ -47,6 +47,7 @@ def lib():
         L.orc_tg_set_ab.argtypes = [_V, _I, _I, _D]
         L.orc_tg_set_dt3.argtypes = [_V, _I]
         L.orc_tg_set_dt2.argtypes = [_V, _D]
+        L.orc_tg_set_dt4.argtypes = [_V]
         L.orc_tg_coefs_ref.argtypes = [_I, _I, _P, _D, _D, _D, _P, _P, _P, _P]
         L.orc_tg_ref_spacing.argtypes = [_I, _I, _P, _P]
         L.orc_mt_create.restype = _V
@@ -123,10 +124,11 @@ def tg_coefs_ref(sN, M, V2, dv, dL, dLx):
 
 class TGSolver:
     """Oracle of TG_solver::calc_step (fr_d_t_g_2d.cpp:2552), dt_alg = 0, dt_alg
-    = 2 with the cut-off `dt2`, or dt_alg = 3 with `dt3` series terms."""
+    = 2 with the cut-off `dt2`, dt_alg = 3 with `dt3` series terms, or dt_alg
+    = 4 (the ia/ibk approximations) with `dt4`."""
 
     def __init__(self, sN, M, alpha, tau, sigma, C1, S, rowA, rowB, colA, colB, C0,
-                 func=0, fpow=1.0, int_err=1e-6, ab=None, dt3=0, dt2=0.0):
+                 func=0, fpow=1.0, int_err=1e-6, ab=None, dt3=0, dt2=0.0, dt4=False):
         self.sN, self.M = sN, M
         nn = (sN + 2) * (M + 2)
         arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(nn)
@@ -139,6 +141,8 @@ class TGSolver:
             lib().orc_tg_set_dt3(self._h, int(dt3))
         if dt2:
             lib().orc_tg_set_dt2(self._h, float(dt2))
+        if dt4:
+            lib().orc_tg_set_dt4(self._h)
 
     def step(self, n=1):
         for _ in range(n):

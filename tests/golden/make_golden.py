@@ -179,6 +179,10 @@ def main():
     # binomial-series history (dt_alg 3), 25 and 10 terms
     tg_case("tg_dt3_a", 14, 0, 12, 10, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 30, 10, dt_alg=3, dt_eps=25)
     tg_case("tg_dt3_b", 15, 0, 9, 14, 0.5, 1e-3, 1.0, 0.2, 0.0, 0.0, 40, 20, dt_alg=3, dt_eps=10)
+    # the ia/ibk series approximations of the history (dt_alg 4), across the
+    # reference's every-20-steps compaction of its increment lists
+    tg_case("tg_dt4_a", 18, 0, 12, 10, 0.7, 1e-2, 0.3, 0.05, 1.0, 0.1, 60, 20, dt_alg=4)
+    tg_case("tg_dt4_b", 19, 0, 9, 14, 0.4, 1e-3, 1.0, 0.2, 0.0, 0.0, 90, 30, dt_alg=4)
     mt_traj("mt_flux", "ref_mt", 21, 10, 4, 0.8, 0.01, 1.0, 1e-3, 0.0, -20.0, 1, 60, 20,
             lambda x, r: -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * r.normal(size=x.size))
     mt_traj("mt_dir_cv2", "ref_mt", 22, 10, 3, 0.6, 0.02, 1.0, 2e-3, -0.5, 0.0, 0, 50, 25,

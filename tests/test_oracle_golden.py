@@ -262,3 +262,27 @@ def test_ab_weights_against_quadrature():
         f = np.array([O.mittag_leffler(a, 1.0, -lam * ui ** a, 1e-15) for ui in u])
         q = 0.5 * np.sum(wg * f * 2 * tau * v)
         assert w[d - 1] == pytest.approx(q, rel=rel), d
+
+
+@pytest.mark.parametrize("name", ["tg_dt4_a", "tg_dt4_b"])
+def test_tg_ia_ibk_approximation_matches_reference(golden, name):
+    # dt_alg 4: per node and direction the "long" (ia) and "short" (ibk)
+    # series approximations of the history sum; the trajectories differ from
+    # the full history by 3e-7 (alpha 0.7) to 7e-5 (alpha 0.4), so
+    # agreement to 1e-14 pins the approximation structure itself
+    m, z = golden(name)
+    dL, dLx = O.tg_ref_spacing(m["sN"], m["M"])
+    coefs = O.tg_coefs_ref(m["sN"], m["M"], z["V2"], m["dv"], dL, dLx)
+    s = O.TGSolver(m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], m["C1"], m["S"], *coefs,
+                   z["C0"], int_err=m["int_err"], dt4=True)
+    full = O.TGSolver(m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], m["C1"], m["S"], *coefs,
+                      z["C0"], int_err=m["int_err"])
+    t = 0
+    for q, k in enumerate(_snap_steps(m)):
+        s.step(k - t)
+        full.step(k - t)
+        t = k
+        ref = z["snaps"][q]
+        assert np.max(np.abs(s.state() - ref)) <= 1e-14 * np.max(np.abs(ref))
+        d = np.max(np.abs(full.state() - ref)) / np.max(np.abs(ref))
+        assert 1e-9 < d < 1e-3, d
