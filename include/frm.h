@@ -38,8 +38,12 @@ enum {
     FRM_PREC_F32C = 1,     /* FP32 history, compensated (Kahan) FP32 sums */
     FRM_PREC_MFMA_BF16 = 2, /* blocked Toeplitz on MFMA, split hi+lo BF16, FP32 acc */
     FRM_PREC_SOE = 3,       /* sum-of-exponentials recurrences (series history) */
-    FRM_PREC_BINOMIAL = 4   /* the reference's single-series expansion of the history
+    FRM_PREC_BINOMIAL = 4,  /* the reference's single-series expansion of the history
                                (TG_solver dt_alg = 3, 2D LOD Caputo only) */
+    FRM_PREC_IAIBK = 5      /* the reference's per-node "long" (ia) and "short" (ibk)
+                               series approximations of the history (TG_solver
+                               dt_alg = 4, fr_d_t_g_2d.cpp:246-741; 2D LOD Caputo,
+                               g(t) = t) */
 };
 
 /* kernel-class ids for frm_timing_get */

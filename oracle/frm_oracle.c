@@ -582,6 +582,20 @@ static void dt4_calc_va(double A, int block, double Fv, double* V, dt4_apps* ap,
     }
 }
 
+/* Test hook: the dt_alg 4 sums of one node in direction dir (0: rows, (a, b)
+ * = (3/2, 1/2); 1: columns, (2, 1)) for steps n = 1..nsteps, increments
+ * F[k] = Delta_k (k < nsteps): V[n-1] is calc_va's V at x = n before
+ * scaling by Da tau^(1-a)/(1-a). */
+void orc_dt4_series(double alpha, int dir, const double* F, int nsteps, double* V) {
+    dt4_apps ap;
+    memset(&ap, 0, sizeof ap);
+    for (int n = 1; n <= nsteps; ++n)
+        dt4_calc_va(1.0 - alpha, n - 1, F[n - 1], &V[n - 1], &ap, dir == 0 ? 1.5 : 2.0,
+                    dir == 0 ? 0.5 : 1.0);
+    free(ap.ia);
+    free(ap.ibk);
+}
+
 void orc_tg_set_dt4(orc_tg* t) {
     size_t nn = (size_t)t->ld * (t->M + 2);
     t->dt4 = 1;

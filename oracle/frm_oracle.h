@@ -64,6 +64,9 @@ void orc_tg_set_dt2(orc_tg* t, double eps);
 /* dt_alg 4: the ia/ibk series approximations of the history per node and
  * direction (fr_d_t_g_2d.cpp:246-741, 3438-3469), before the first step */
 void orc_tg_set_dt4(orc_tg* t);
+/* dt_alg 4 test hook: one node's approximated sums V[n-1] at x = n for
+ * n = 1..nsteps (direction 0: rows, 1: columns), increments F[0..nsteps) */
+void orc_dt4_series(double alpha, int dir, const double* F, int nsteps, double* V);
 /* Grid spacings the reference constructor derives (fr_d_t_g_2d.cpp:2337-2388). */
 void orc_tg_ref_spacing(int sN, int M, double* dL, double* dLx);
 

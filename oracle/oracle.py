@@ -48,6 +48,7 @@ def lib():
         L.orc_tg_set_dt3.argtypes = [_V, _I]
         L.orc_tg_set_dt2.argtypes = [_V, _D]
         L.orc_tg_set_dt4.argtypes = [_V]
+        L.orc_dt4_series.argtypes = [_D, _I, _P, _I, _P]
         L.orc_tg_coefs_ref.argtypes = [_I, _I, _P, _D, _D, _D, _P, _P, _P, _P]
         L.orc_tg_ref_spacing.argtypes = [_I, _I, _P, _P]
         L.orc_mt_create.restype = _V
@@ -93,6 +94,15 @@ def gamma(x: float) -> float:
 
 def tg_kb(func, fpow, alpha, tau, n, s, ja, sa, l, int_err=1e-6) -> float:
     return lib().orc_tg_kb(func, fpow, alpha, tau, int_err, n, s, ja, sa, l)
+
+
+def dt4_series(alpha, direction, F):
+    """calc_va's approximated sums of one node for steps 1..len(F) (the
+    dt_alg 4 history, unscaled)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    V = np.empty(F.size)
+    lib().orc_dt4_series(alpha, direction, _p(F), F.size, _p(V))
+    return V
 
 
 def tg_kb_table(func, fpow, alpha, tau, n, int_err=1e-6) -> np.ndarray:

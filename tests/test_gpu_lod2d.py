@@ -234,3 +234,41 @@ def test_lod2d_gpow_rejects_other_forms(ctx):
         args.update(kw)
         with pytest.raises(N.FrmError):
             S.TGSolver(ctx, 20, 20, 0.6, 1e-3, 1.0, coefs, C0, capacity=10, **args)
+
+
+# dt_alg 4: the reference's per-node ia/ibk series approximations of the
+# history, planned on the host (data-independent structure) and applied to
+# every node's coefficient slots on the GPU
+@pytest.mark.parametrize("name", ["tg_dt4_a", "tg_dt4_b"])
+def test_lod2d_iaibk_matches_reference_golden(ctx, golden, name):
+    m, z = golden(name)
+    dL, dLx = S.tg_ref_spacing(m["sN"], m["M"])
+    coefs = S.coefs_ref(z["V2"], m["dv"], dL, dLx)
+    s = S.TGSolver(ctx, m["sN"], m["M"], m["alpha"], m["tau"], m["sigma"], coefs, z["C0"],
+                   capacity=m["nsteps"], C1=m["C1"], S=m["S"], precision=N.PREC_IAIBK)
+    ks = [k for k in range(1, m["nsteps"] + 1) if k % m["every"] == 0 or k == m["nsteps"]]
+    t = 0
+    for q, k in enumerate(ks):
+        s.calc_step(k - t)
+        t = k
+        assert relerr(s.b_C, z["snaps"][q]) <= TOL, (name, k)
+    s.close()
+
+
+@pytest.mark.parametrize("sN,M,alpha,steps", [(65, 33, 0.6, 200), (97, 49, 0.35, 120)])
+def test_lod2d_iaibk_matches_oracle(ctx, sN, M, alpha, steps):
+    rng = np.random.default_rng(sN + M)
+    coefs, C0 = _adv_case(rng, sN, M, D=2e-3)
+    tau, sigma, C1, Ssrc = 1e-3, 1.0, 0.2, 0.05
+    o = O.TGSolver(sN, M, alpha, tau, sigma, C1, Ssrc, *coefs, C0, dt4=True)
+    g = S.TGSolver(ctx, sN, M, alpha, tau, sigma, coefs, C0, capacity=steps, C1=C1, S=Ssrc,
+                   precision=N.PREC_IAIBK)
+    for chunk in (1, steps // 2 - 1, steps - steps // 2):
+        o.step(chunk)
+        g.calc_step(chunk)
+        assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    # and after a restart from the initial field
+    g.set_state(C0)
+    g.calc_step(steps)
+    assert relerr(g.b_C, o.state().reshape(M + 2, sN + 2)) <= TOL
+    g.close()

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfrm.so")
 
 FRM_OK, FRM_E_INVALID, FRM_E_HIP, FRM_E_NOMEM, FRM_E_RANGE, FRM_E_STATE = 0, -1, -2, -3, -4, -5
-PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE, PREC_BINOMIAL = 0, 1, 2, 3, 4
+PREC_F64, PREC_F32C, PREC_MFMA_BF16, PREC_SOE, PREC_BINOMIAL, PREC_IAIBK = 0, 1, 2, 3, 4, 5
 KERNEL_CAPUTO, KERNEL_AB = 0, 1
 K_MEMTERM, K_ROWS, K_COLS, K_SWEEP1D, K_OTHER, K_BLOCKFAR = 0, 1, 2, 3, 4, 5
 

@@ -29,6 +29,7 @@
 
 #include "frm_gpow.hpp"
 #include "frm_history.hpp"
+#include "frm_iaibk.hpp"
 #include "frm_stream.hpp"
 #include "frm_mfma.hpp"
 #include "frm_wavescan.hpp"
@@ -92,6 +93,15 @@ struct frm_lod2d {
     double g_pow = 1.0, int_err = 1e-6, Da = 0.0;
     double* lineAB = nullptr;  // off-diagonals per line: rows A, B [M-1][NX], cols A, B [sN-1][NY]
     int* flag = nullptr;       // set when a weight series reached its safety bound
+    // FRM_PREC_IAIBK: per-node coefficient slots [2][slots][MAXAK][ld] and
+    // both directions' operation plans
+    double* ia_H = nullptr;
+    frm::IaOp* ia_ops[2] = {nullptr, nullptr};
+    double* ia_vals[2] = {nullptr, nullptr};
+    std::vector<int> ia_off[2];
+    int ia_slots = 0;
+    double ia_ak[frm::kIaMaxAk] = {};
+    double ia_scale = 0.0;     // Da tau^(1-alpha) / (1-alpha)
 };
 
 namespace frm {
@@ -397,6 +407,59 @@ __global__ __launch_bounds__(64) void k_line_coefs(const double* __restrict__ A,
     }
 }
 
+// ------------------------------------------------------------- ia/ibk memory term
+// The reference's dt_alg 4 (calc_va, fr_d_t_g_2d.cpp:490-741) for every node:
+// each direction's plan step (frm_iaibk.cpp) evaluates the node's valid
+// approximations at x = n, adds the newest block exactly, then folds the
+// newest increment Delta_{n-1} into the coefficient slots.  One lane per
+// node; the operations are wavefront-uniform.
+struct IaArgs {
+    double* H;                 // [2][slots][MAXAK][ld]
+    const double* ring;        // two-slot increment ring
+    int64_t ld;
+    int n, slots;
+    const IaOp* ops[2];
+    int o0[2], o1[2];          // this step's operation range per direction
+    const double* vals[2];
+    double ak[kIaMaxAk];
+    double scale;
+    double* part;              // [2][ld]
+};
+
+__global__ __launch_bounds__(256) void k_iaibk_memterm(IaArgs p) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.ld) return;
+    const int64_t ld = p.ld;
+    const double f = p.ring[(int64_t)((p.n - 1) % 2) * ld + i];  // the two-slot ring
+#pragma unroll 1
+    for (int t = 0; t < 2; ++t) {
+        double* H = p.H + (int64_t)t * p.slots * kIaMaxAk * ld + i;
+        const IaOp* ops = p.ops[t];
+        const double* vals = p.vals[t];
+        double V = 0.0;
+        for (int o = p.o0[t]; o < p.o1[t]; ++o) {
+            const IaOp op = ops[o];
+            double* c = H + (int64_t)op.slot * kIaMaxAk * ld;
+            const double* w = vals + op.off;
+            if (op.kind == kIaEvalIa || op.kind == kIaEvalIbk) {
+                double r = 0.0;
+                for (int k = 0; k < op.nt; ++k) r += c[(int64_t)k * ld] * w[k];
+                V += r;
+            } else if (op.kind == kIaExact) {
+                V += w[0] * f;
+            } else if (op.kind == kIaCreate) {
+                for (int k = 0; k < kIaMaxAk; ++k) c[(int64_t)k * ld] = k < op.nt ? p.ak[k] * (f * w[k]) : 0.0;
+            } else if (op.kind == kIaAdd) {
+                for (int k = 0; k < op.nt; ++k) c[(int64_t)k * ld] += p.ak[k] * (f * w[k]);
+            } else if (op.kind == kIaCopy) {
+                const double* src = H + (int64_t)op.src * kIaMaxAk * ld;
+                for (int k = 0; k < kIaMaxAk; ++k) c[(int64_t)k * ld] = src[(int64_t)k * ld];
+            }
+        }
+        p.part[(int64_t)t * ld + i] = V * p.scale;
+    }
+}
+
 // ------------------------------------------------------------- SOE memory term
 // History weights for lags k >= 2 replaced by sum_l c_{t,l} q_l^(k-2): per
 // node, the mode states H_l = sum_{s <= n-2} q_l^(n-2-s) Delta_s follow
@@ -608,7 +671,7 @@ static ColArgs col_args(frm_lod2d* s, int nch, int n) {
     ColArgs a;
     a.C = s->C;
     a.Cprev = s->Cprev;
-    const bool ring = soe || s->precision == FRM_PREC_BINOMIAL;
+    const bool ring = soe || s->precision == FRM_PREC_BINOMIAL || s->precision == FRM_PREC_IAIBK;
     a.slot = (double*)s->hist->data +
              (int64_t)(ring ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
     a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
@@ -754,6 +817,11 @@ static void lod_free(frm_lod2d* s) {
     (void)hipFree(s->G);
     (void)hipFree(s->lineAB);
     (void)hipFree(s->flag);
+    (void)hipFree(s->ia_H);
+    for (int t = 0; t < 2; ++t) {
+        (void)hipFree(s->ia_ops[t]);
+        (void)hipFree(s->ia_vals[t]);
+    }
     delete s;
 }
 
@@ -775,7 +843,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: coefficient arrays missing");
     FRM_REQUIRE(ctx,
                 d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE ||
-                    d->precision == FRM_PREC_MFMA_BF16 || d->precision == FRM_PREC_BINOMIAL,
+                    d->precision == FRM_PREC_MFMA_BF16 || d->precision == FRM_PREC_BINOMIAL ||
+                    d->precision == FRM_PREC_IAIBK,
                 "frm_lod2d_create: precision %d not built", d->precision);
     FRM_REQUIRE(ctx,
                 d->trunc_eps == 0 ||
@@ -794,6 +863,10 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
                 "frm_lod2d_create: kernel %d", d->kernel);
     FRM_REQUIRE(ctx, d->g_func == 0 || d->g_func == 3, "frm_lod2d_create: g_func %d (0 or 3)",
                 d->g_func);
+    FRM_REQUIRE(ctx,
+                d->precision != FRM_PREC_IAIBK ||
+                    (d->kernel == FRM_KERNEL_CAPUTO && d->g_func == 0 && d->trunc_eps == 0),
+                "frm_lod2d_create: FRM_PREC_IAIBK takes the Caputo kernel with g(t) = t");
     FRM_REQUIRE(ctx,
                 d->g_func == 0 ||
                     (d->precision == FRM_PREC_F64 && d->kernel == FRM_KERNEL_CAPUTO &&
@@ -865,7 +938,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     // the SOE and binomial forms keep only a two-slot ring of increments,
     // the blocked MFMA form FP32 increments
     const bool mf = d->precision == FRM_PREC_MFMA_BF16;
-    const bool ring = d->precision == FRM_PREC_SOE || d->precision == FRM_PREC_BINOMIAL;
+    const bool ring = d->precision == FRM_PREC_SOE || d->precision == FRM_PREC_BINOMIAL ||
+                      d->precision == FRM_PREC_IAIBK;
     int rc = hist_alloc(ctx, nodes, ring ? kSoeLag0 : d->capacity,
                         mf ? FRM_PREC_MFMA_BF16 : FRM_PREC_F64, &s->hist);
     if (rc) {
@@ -1012,6 +1086,32 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
             return check_hip(ctx, e, "hipMalloc(lod2d binomial)");
         }
     }
+    if (d->precision == FRM_PREC_IAIBK) {
+        // both directions' plans for the whole capacity, (a, b) = (3/2, 1/2)
+        // for rows and (2, 1) for columns (Om1, fr_d_t_g_2d.cpp:3443-3458)
+        const double A = 1.0 - d->alpha;
+        frm::IaPlan pl[2] = {frm::iaibk_plan(A, 1.5, 0.5, d->capacity),
+                             frm::iaibk_plan(A, 2.0, 1.0, d->capacity)};
+        s->ia_slots = std::max(1, std::max(pl[0].nslots, pl[1].nslots));
+        for (int k = 0; k < frm::kIaMaxAk; ++k) s->ia_ak[k] = pl[0].ak[k];
+        s->ia_scale = (1.0 / gamma_cody(1.0 - d->alpha)) * (std::pow(d->tau, 1.0 - d->alpha) / (1.0 - d->alpha));
+        e = hipMalloc(&s->ia_H, sizeof(double) * 2 * (size_t)s->ia_slots * frm::kIaMaxAk * s->hist->ld);
+        for (int t = 0; t < 2 && e == hipSuccess; ++t) {
+            s->ia_off[t] = pl[t].step_off;
+            e = hipMalloc(&s->ia_ops[t], sizeof(frm::IaOp) * std::max<size_t>(1, pl[t].ops.size()));
+            if (e == hipSuccess) e = hipMalloc(&s->ia_vals[t], sizeof(double) * std::max<size_t>(1, pl[t].vals.size()));
+            if (e == hipSuccess && !pl[t].ops.empty())
+                e = hipMemcpy(s->ia_ops[t], pl[t].ops.data(), sizeof(frm::IaOp) * pl[t].ops.size(),
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess && !pl[t].vals.empty())
+                e = hipMemcpy(s->ia_vals[t], pl[t].vals.data(), sizeof(double) * pl[t].vals.size(),
+                              hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) {
+            lod_free(s);
+            return check_hip(ctx, e, "hipMalloc(lod2d ia/ibk)");
+        }
+    }
     if (mf) {
         // BF16 parts of the FP64 weights: w = p0 + p1 + p2 + O(2^-24 w)
         // default parts: two for the Caputo weights (whose products are
@@ -1103,6 +1203,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
     const bool soe = s->precision == FRM_PREC_SOE;
     const bool mf = s->precision == FRM_PREC_MFMA_BF16;
     const bool bin = s->precision == FRM_PREC_BINOMIAL;
+    const bool iab = s->precision == FRM_PREC_IAIBK;
     if (s->tstep + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_lod2d_step: %d steps exceed capacity %d",
                          s->tstep + nsteps, s->capacity);
@@ -1133,6 +1234,30 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                 else launch(k_soe_memterm<double, false>, (double*)s->soe_H);
             }
             FRM_HIP(ctx, hipGetLastError());
+        } else if (iab) {
+            // nothing to sum before the first increment
+            nch = n > 0 ? 1 : 0;
+            if (n > 0) {
+                IaArgs a;
+                a.H = s->ia_H;
+                a.ring = (const double*)s->hist->data;
+                a.ld = s->hist->ld;
+                a.n = n;
+                a.slots = s->ia_slots;
+                for (int t = 0; t < 2; ++t) {
+                    a.ops[t] = s->ia_ops[t];
+                    a.vals[t] = s->ia_vals[t];
+                    a.o0[t] = s->ia_off[t][n - 1];
+                    a.o1[t] = s->ia_off[t][n];
+                }
+                for (int k = 0; k < kIaMaxAk; ++k) a.ak[k] = s->ia_ak[k];
+                a.scale = s->ia_scale;
+                a.part = s->hist->part;
+                LaunchTimer t(ctx, FRM_K_MEMTERM, 0.0);
+                hipLaunchKernelGGL(k_iaibk_memterm, dim3((unsigned)(a.ld / 256)), dim3(256), 0,
+                                   ctx->stream, a);
+                FRM_HIP(ctx, hipGetLastError());
+            }
         } else if (bin) {
             nch = 1;
             const int64_t ld = s->hist->ld;
@@ -1178,7 +1303,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_COLS, 0.0);
             FRM_HIP(ctx, launch_cols(s->csh, ctx->stream, col_args(s, nch, n)));
         }
-        if (!soe && !bin) s->hist->depth += 1;
+        if (!soe && !bin && !iab) s->hist->depth += 1;
         s->tstep = n + 1;
     }
     return FRM_OK;
