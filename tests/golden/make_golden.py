@@ -63,6 +63,60 @@ def tg_case(name, seed, func, sN, M, alpha, tau, sigma, dv, C1, S, nsteps, every
          kb=kb)
 
 
+# ---------------------------------------------------------------- fitting input reader
+def fitio_case(name, files, args):
+    """Runs the reference's main (mt_minimal.cpp) in fitting mode on the given
+    input files (their text is stored with the fixture) and stores what its
+    reader (1955-2043) handed to fit_and_solve."""
+    with tempfile.TemporaryDirectory() as d:
+        for fn, text in files.items():
+            with open(os.path.join(d, fn), "w") as f:
+                f.write(text)
+        fo = os.path.join(d, "out.bin")
+        argv = [os.path.join(REF, "ref_fitio"), fo, "Fit", "1", "BS", "2", "NB", "2", "Tm", "1"]
+        for a in args:
+            argv.append(os.path.join(d, a[1:]) if a.startswith("@") else a)
+        r = subprocess.run(argv, capture_output=True, text=True, cwd=d)
+        if r.returncode != 0:
+            raise RuntimeError(f"ref_fitio failed ({r.returncode}): {r.stderr[-500:]}")
+        out = np.fromfile(fo)
+    meta = dict(source="mt_minimal.cpp main, fitting input reader (1955-2043)", args=args,
+                files=sorted(files))
+    save(name, meta, parsed=out, **{"file_" + k.replace(".", "_"): np.array(v)
+                                    for k, v in files.items()})
+
+
+def fitio_all():
+    rng = np.random.default_rng(31)
+
+    def rows(n, c, lo=-5, hi=5, trail=""):
+        a = rng.uniform(lo, hi, (n, c)).round(7)
+        return "".join(" ".join("%.17g" % x for x in r) + trail + "\n" for r in a)
+
+    head = "1 3 2 2 2 1 4 2 0.25\n30 0.7 1.4 1.6 1e-6 200\n"
+    body = rows(3, 3) + rows(2, 2) + rows(2, 2) + rows(1, 2) + rows(2, 3)
+    ev = rows(4, 4, 0, 1e-6, " ")
+    # one file, 25 parameter values (the reader takes 20), default H_mult 10
+    fitio_case("fitio_one_file", {"fit_input.txt": head + body + ev +
+                                  " ".join("%.10g" % x for x in rng.uniform(0, 1, 25)) + "\n"},
+               ["fi_file", "@fit_input.txt"])
+    # evapotranspiration and parameters from their own files, H_mult 1
+    fitio_case("fitio_files", {"fi.txt": head + body, "et.txt": ev,
+                               "pv.txt": " ".join("%.10g" % x for x in rng.uniform(0, 1, 7))},
+               ["fi_file", "@fi.txt", "et_file", "@et.txt", "pv_file", "@pv.txt", "H_mult", "1"])
+    # empty sections and no parameters (one evapotranspiration column: the
+    # reader prints EV_F[0][0], so the reference needs nEV_F >= 1)
+    fitio_case("fitio_empty", {"fit_input.txt": "0 2 1 0 0 0 2 1 1\n5 0.5 1 1 0.001 3\n" +
+                               rows(2, 3) + rows(1, 2) + rows(2, 3, 0, 1)},
+               ["fi_file", "@fit_input.txt"])
+    # a file that ends inside the values to fit: the reader gives up
+    fitio_case("fitio_short", {"fit_input.txt": head + rows(2, 3)}, ["fi_file", "@fit_input.txt"])
+    # exactly 19 parameter values: the reference's counter reports 20
+    fitio_case("fitio_19", {"fit_input.txt": head + body + ev +
+                            " ".join("%.10g" % x for x in rng.uniform(0, 1, 19)) + "\n"},
+               ["fi_file", "@fit_input.txt"])
+
+
 # ---------------------------------------------------------------- 1D H_solver
 def mt_traj(name, binary, seed, BS, NB, alpha, tau, L, k, Ev, Hb, bc, nsteps, every, U0fn,
             beta=None):
@@ -196,6 +250,7 @@ def main():
                 [1, 2, 5, 17, 64, 200])
     mf_all("mf")
     mf_series("mf_series")
+    fitio_all()
 
 
 if __name__ == "__main__":

@@ -149,3 +149,59 @@ def test_result_rows_format(checker, tmp_path):
     rows = lines[-3:]
     U, w = [-1.5, -0.25, 0.0], [0.3, 0.35, 0.4]
     assert rows == ["%g %g %g %g " % (1000.0, i * 0.05, U[i], w[i]) for i in range(3)]
+
+
+# ---- against the reference's own reader (tests/golden/fitio_*.npz: the
+# reference's main in fitting mode, its fit_and_solve interposed to capture
+# the parsed fields; see oracle/ref/harness_fitio.cpp)
+def _unpack_reference(parsed):
+    it = iter(parsed.tolist())
+    called = next(it)
+    if not called:
+        return None
+    head = [next(it) for _ in range(15)]
+    to_fit, nv, ni, nc, npc, nir, net, nef = (int(x) for x in head[:8])
+    f = dict(counts=[to_fit, nv, ni, nc, npc, nir, net, nef], ev_mu=[head[8]],
+             pso=[head[9], head[10], head[11], head[12], head[13], head[14]])
+    take = lambda n: [next(it) for _ in range(n)]  # noqa: E731
+    for k, n in (("values_T", nv), ("values_Z", nv), ("values_F", nv), ("init_Z", ni),
+                 ("init_F", ni), ("perc_T", npc), ("perc_A", npc), ("irr_T", nir), ("irr_A", nir),
+                 ("check_T", nc), ("check_Z", nc), ("check_F", nc), ("EV_T", net), ("LAI", net)):
+        f[k] = take(n)
+    f["EV_F"] = [take(net) for _ in range(nef)]
+    npar = int(next(it))
+    f["nparam_values"] = npar
+    f["params"] = take(npar)
+    assert next(it, None) is None
+    return f
+
+
+@pytest.mark.parametrize("name", ["fitio_one_file", "fitio_files", "fitio_empty", "fitio_short",
+                                  "fitio_19"])
+def test_reader_matches_reference_reader(checker, golden, tmp_path, name):
+    m, z = golden(name)
+    for fn in m["files"]:
+        (tmp_path / fn).write_text(str(z["file_" + fn.replace(".", "_")]))
+    opts = dict(zip(m["args"][::2], m["args"][1::2]))
+    path = lambda k: str(tmp_path / opts[k][1:]) if k in opts else "-"  # noqa: E731
+    hmult = float(opts.get("H_mult", 10.0))  # the reference's default Hmult (mt_minimal.cpp:209)
+    got, lines = run(checker, path("fi_file"), hmult, path("et_file"), path("pv_file"))
+    ref = _unpack_reference(z["parsed"])
+    if ref is None:
+        assert got is None  # the reference's reader gave up on this file too
+        return
+    assert got is not None
+    c = [int(x) for x in lines[1].split()[1:]]
+    assert c == ref["counts"]
+    assert got["ev_mu"] == ref["ev_mu"]
+    pso = lines[3].split()[1:]
+    assert [float(x) for x in pso] == ref["pso"]
+    for k in ("values_T", "values_Z", "values_F", "init_Z", "init_F", "perc_T", "perc_A", "irr_T",
+              "irr_A", "check_T", "check_Z", "check_F", "EV_T", "LAI", "EV_F"):
+        assert got.get(k, []) == ref[k], k
+    if name == "fitio_19":
+        # 19 values: the reference's counter says 20 with the last one unset;
+        # this reader returns the 19 it read
+        assert ref["nparam_values"] == 20 and got["params"] == ref["params"][:19]
+    else:
+        assert got["params"] == ref["params"] and len(got["params"]) == ref["nparam_values"]

@@ -169,3 +169,17 @@ def test_mt1d_graph_matches_eager(ctx, kind, picard):
         assert np.array_equal(e.ab_error(), g.ab_error())
     e.close()
     g.close()
+
+
+def test_mt1d_graph_long_call_matches_eager(ctx):
+    # 400 steps in one call: the step graphs are replayed back to back with
+    # new arguments on their nodes every step (depth, time chunks, slots),
+    # with no synchronisation in between
+    prob, u0 = W.mt1d_c0_case()
+    e = S.MT1DBatch(ctx, [prob], 1e-3, 400, [u0])
+    g = S.MT1DBatch(ctx, [prob], 1e-3, 400, [u0], graph=True)
+    e.calc_step(400)
+    g.calc_step(400)
+    assert np.array_equal(e.state()[0], g.state()[0])
+    e.close()
+    g.close()
