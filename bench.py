@@ -9,18 +9,20 @@ on MFMA with split BF16 products; the 32-term exponential sum) each run the
 whole 4000 steps from t = 0 on the same inputs; their fields at step 4000
 give the max / RMS relative error against FP64.  The K = --steps timed
 steps (default 64) are a window at history depth T/2 = 2000 inside those
-runs, starting on a 64-step block boundary (1984), so the window pays for
-the far product of its block; the W = --warmup steps just before it are the
-last untimed ones.  `value` is the window rate of the fastest form within
-1e-6 of FP64; the whole-run rates stand beside it.
+runs, starting on a 64-step block boundary (1984); the MFMA form forms the
+next block's far product in 8 node slices, one every 8 steps, so a window
+of K steps pays for ceil(K/8) slices.  The W = --warmup steps just before it
+are the last untimed ones.  `value` is the window rate of the fastest form
+within 1e-6 of FP64; the whole-run rates stand beside it.
 
     python bench.py [--gpus 1] [--steps 64] [--warmup 5] [--config c2]
 
 Prints one JSON line.  `roofline` describes the chosen form's dominant
-kernel, the far product k_block_far: algorithmic bytes per launch = 4 B x
-settled slots x nodes (read) + 4 B x 128 rows x nodes (written), over its
-HIP-event-timed duration on the solver's stream inside the window.  The
-plain path's memory-term kernel has its own roofline object.
+kernel -- for the MFMA form the far product k_block_far: algorithmic bytes
+per launch = 4 B x settled slots x slice nodes (read) + 4 B x 128 rows x
+slice nodes (written), over its HIP-event-timed duration on the solver's
+stream inside the window.  The plain path's memory-term kernel has its own
+roofline object.
 `cpu_baseline` times the oracle (a port of the reference's CPU path) on
 LOD steps of the same workload at the window's history depth.  The other
 configurations (--config c0|c1|c3|c4|ab1d|ab1d_series|timesum3d|soe_step)
@@ -491,8 +493,8 @@ def run_lod_forms(args, cfg_name):
     against the chosen forms on the same inputs.  Every run goes the whole
     configured length T from t = 0; the K = --steps timed steps are a window
     at history depth T/2 (the run average, since per-step cost is linear in
-    depth), started on a 64-step block boundary so that the window pays for
-    the MFMA form's far product of its block.  The line's value is the
+    depth), started on a 64-step block boundary (the MFMA form's far
+    product is formed in 8 slices, one every 8 steps).  The line's value is the
     window rate of the fastest form within 1e-6 of FP64 (error taken at step
     T); the whole-run rates are reported beside it."""
     from theses_code_amd import _native as N
@@ -533,11 +535,12 @@ def run_lod_forms(args, cfg_name):
         forms.remove("binomial")
     # events in the window on the dominant kernel of each form only: the
     # far product (one launch per block) and the plain memory term (ms long)
-    fmask = {"mfma": 1 << N.K_BLOCKFAR, "soe": 0, "binomial": 0}
+    fmask = {"mfma": 1 << N.K_BLOCKFAR, "soe": 1 << N.K_MEMTERM, "binomial": 0}
     tot_nodes, t_plain, tw_plain, kt_plain = 0, 0.0, 0.0, {}
     plain_win = {"launches": 0, "ms": 0.0, "bytes": 0.0}
     res = {f: dict(seconds=0.0, window=0.0, errs=[], rms=[], fits=[], kernels_ms={},
-                   far={"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0}, shapes=[]) for f in forms}
+                   far={"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0},
+                   mem={"launches": 0, "ms": 0.0, "bytes": 0.0}, shapes=[]) for f in forms}
     for r in runs:
         sN, M = r["case"][0], r["case"][1]
         nodes = (sN - 1) * (M - 1)
@@ -563,16 +566,21 @@ def run_lod_forms(args, cfg_name):
             q["window"] += w0["seconds"]
             q["shapes"].append(w0["sweep_shape"])
             if f == "mfma":
-                n_, ms_, _ = w0["kernels"]["blockfar"]
+                # the far slices in the window: algorithmic bytes from the
+                # library (every settled slot read once, 4 B per node; the
+                # 128 F rows written once, 4 B per node and row); each slice
+                # covers 1/8 of the nodes, so the useful flops (2 x 128 per
+                # slot and node) are 64 x (bytes - 512 B x nodes / 8)
+                n_, ms_, by_ = w0["kernels"]["blockfar"]
                 q["far"]["launches"] += n_
                 q["far"]["ms"] += ms_
-                # algorithmic bytes and useful flops of the far products in the window: every
-                # settled slot read once (4 B per node), the 128 F rows
-                # written once (4 B per node and row)
-                for b in range(start // BLOCK_STEPS * BLOCK_STEPS, start + K, BLOCK_STEPS):
-                    if b >= start and b > 0:
-                        q["far"]["bytes"] += (4.0 * b + 4.0 * 2 * BLOCK_STEPS) * nodes
-                        q["far"]["flops"] += 2.0 * 2 * BLOCK_STEPS * b * nodes
+                q["far"]["bytes"] += by_
+                q["far"]["flops"] += 64.0 * (by_ - 512.0 * nodes * n_ / 8)
+            if f == "soe":
+                n_, ms_, by_ = w0["kernels"]["memterm"]
+                q["mem"]["launches"] += n_
+                q["mem"]["ms"] += ms_
+                q["mem"]["bytes"] += by_
             # per-kernel breakdown from a second, event-timed run
             if f != "soe" or args.breakdown_soe:
                 for k, v in _lod_run(*common, timing=True, **kw, **fkw[f])[2].items():
@@ -641,15 +649,15 @@ def run_lod_forms(args, cfg_name):
                                   "history_depth": [start, start + K - 1],
                                   "note": "the K timed steps lie at history depth T/2 of "
                                           "whole runs from t = 0, starting on a 64-step block "
-                                          "boundary (the window pays for its block's far "
-                                          "product)"},
+                                          "boundary; the MFMA form's far product is formed "
+                                          "in 8 node slices, one every 8 steps"},
                        "parallelism": "1 GPU"},
             "chosen_form": dict(chosen, name=best),
             "forms": out_forms,
             "plain_fp64": plain,
             "speedup_vs_plain_fp64": tw_plain / chosen["window_seconds"],
             "measured_read_GBs": read_gbs}
-    if "mfma" in res and res["mfma"]["far"]["launches"]:
+    if best == "mfma" and res["mfma"]["far"]["launches"]:
         far = res["mfma"]["far"]
         a = far["bytes"] / (far["ms"] * 1e-3) / 1e9
         ratio, src = _stored_ratio(f"blockfar_lod2d_{'1024' if cfg_name == 'c2' else 'ab'}"
@@ -661,13 +669,26 @@ def run_lod_forms(args, cfg_name):
             "frac_of_measured_read": a / read_gbs if read_gbs else None,
             "traffic": ratio * far["bytes"] / far["launches"] if ratio else None,
             "traffic_source": f"stored PMC ratio ({src}) x algorithmic bytes" if ratio else None,
-            "kernel": f"k_block_far<{nprod}> (far product of the MFMA form)",
+            "kernel": f"k_block_far<{nprod}> (far product of the MFMA form, one node slice of 8 "
+                      "per launch, every 8 steps)",
             "launches": far["launches"], "avg_launch_ms": far["ms"] / far["launches"],
             "bytes_per_launch": far["bytes"] / far["launches"],
-            "bytes_model": "4 B x settled slots x nodes read + 4 B x 128 rows x nodes written",
+            "bytes_model": "4 B x settled slots x slice nodes read + 4 B x 128 rows x slice "
+                           "nodes written",
             "useful_tflops": far["flops"] / (far["ms"] * 1e-3) / 1e12,
             "split_products": nprod,
             "mfma_tflops_bf16": nprod * far["flops"] / (far["ms"] * 1e-3) / 1e12}
+    elif best == "soe" and res["soe"]["mem"]["launches"]:
+        mem = res["soe"]["mem"]
+        a = mem["bytes"] / (mem["ms"] * 1e-3) / 1e9
+        line["roofline"] = {
+            "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": a / HBM_PEAK_GBS, "frac_of_measured_read": a / read_gbs if read_gbs else None,
+            "traffic": None, "kernel": "k_soe_memterm (mode recurrences of the SOE form)",
+            "launches": mem["launches"], "avg_launch_ms": mem["ms"] / mem["launches"],
+            "bytes_per_launch": mem["bytes"] / mem["launches"],
+            "bytes_model": "mode planes read and written (8 or 4 B x modes x 2) + ring and "
+                           "partials, per node"}
     else:
         line["roofline"] = plain["roofline"]
     if args.parity_steps > 0:

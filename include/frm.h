@@ -53,8 +53,9 @@ enum {
     FRM_K_COLS = 2,    /* column half-step sweep (+ history push) */
     FRM_K_SWEEP1D = 3, /* batched 1D sweep */
     FRM_K_OTHER = 4,
-    FRM_K_BLOCKFAR = 5, /* blocked Toeplitz far-part product on MFMA; its
-                           "bytes" are the product's flops (2*128*K*nodes) */
+    FRM_K_BLOCKFAR = 5, /* blocked Toeplitz far-part product on MFMA (algorithmic
+                           bytes: 4 B per settled slot and node read, 4 B per F row
+                           and node written) */
     FRM_K_COUNT = 6
 };
 
@@ -281,9 +282,12 @@ typedef struct frm_mt1d_desc {
     int32_t picard;    /* Picard iterations per step, >= 1 */
     double tau;
     const frm_mt1d_problem* probs; /* host array of nprob */
-    int32_t graph;     /* 1: a frm_mt1d_step call of several steps records them
-                          into one HIP graph and launches it once (not while
-                          per-launch timing events are on) */
+    int32_t graph;     /* 1: every step runs as a HIP graph: one graph per
+                          distinct sequence of kernels (a handful per run), built
+                          and instantiated the first time, then replayed with
+                          this step's arguments and grids set on its nodes
+                          (hipGraphExecKernelNodeSetParams); not while per-launch
+                          timing events are on */
 } frm_mt1d_desc;
 
 int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* desc, frm_mt1d** out);

@@ -1,5 +1,5 @@
-"""The chosen forms at the full size and length of the three configurations
-that quote them, against the plain FP64 GPU path on the same seeded inputs:
+"""The chosen forms at the full size and length of the configurations that
+quote them, against the plain FP64 GPU path on the same seeded inputs:
 
 * 2D Caputo LOD, 1024 x 1024 interior nodes, alpha 0.6, 4000 steps: the
   blocked Toeplitz MFMA form (two BF16 parts, FP32 increments, far products
@@ -12,7 +12,8 @@ that quote them, against the plain FP64 GPU path on the same seeded inputs:
   restatement on whole runs of two problems;
 * the largest Atangana-Baleanu parameter set, 2048 x 2048, 2000 steps: the
   64-term exponential sum with FP32 mode states (1 GiB of planes, streamed)
-  and the three-part MFMA form.
+  and the three-part MFMA form;
+* the 512 x 512 workload, alpha 0.7, over its whole 5000 steps: both forms.
 
 Bar: max-norm and RMS relative error <= 1e-6 against FP64 for the chosen
 forms; <= 1e-12 for FP64 against the oracle."""
@@ -131,3 +132,19 @@ def test_c3_batch_full_run(ctx):
         err = float(np.max(np.abs(U64[i] - ref)) / np.max(np.abs(ref)))
         print(f"c3 problem {i}: FP64 vs oracle {err:.3e}", flush=True)
         assert err <= 1e-12
+
+
+def test_c1_chosen_forms_full_depth(ctx):
+    # the 512^2, alpha 0.7 workload over its whole 5000 steps (history depth
+    # up to 4999): both chosen forms against the plain FP64 path
+    cfg = W.C1_WORKLOAD
+    case = W.lod2d_case(cfg["n_interior"], cfg["seed"], vmax=cfg["vmax"], D=cfg["D"])
+    K, alpha = cfg["steps"], cfg["alpha"]
+    print(f"c1: 512^2, alpha {alpha}, {K} steps, plain FP64", flush=True)
+    ref, _ = _run(ctx, case, alpha, cfg, K)
+    for name, kw in (("MFMA", dict(precision=N.PREC_MFMA_BF16)),
+                     ("SOE", dict(precision=N.PREC_SOE, soe_terms=32))):
+        got, _ = _run(ctx, case, alpha, cfg, K, **kw)
+        mx, rms = errs(got, ref)
+        print(f"c1 {name}: max {mx:.3e} rms {rms:.3e}", flush=True)
+        assert mx <= 1e-6 and rms <= 1e-6

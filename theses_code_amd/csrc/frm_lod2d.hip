@@ -85,7 +85,7 @@ struct frm_lod2d {
     void* atab = nullptr;      // weight-operand fragments (atab_build)
     int atab_chunks = 0;
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
-    float* F = nullptr;        // [128][ld]
+    float* F = nullptr;        // [2][128][ld]: far parts of this block and the next
     double* G = nullptr;       // [16][ld] mid part of the current 8-step sub-block
     // g(t) = t^p (g_func 3): the weights and the sweeps' Thomas coefficients
     // are rebuilt every step
@@ -1132,7 +1132,8 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk) + 1;
         const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
-        if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * kBlockSteps * s->hist->ld);
+        // two far buffers: the current block's and the next one's, formed in slices
+        if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * 2 * kBlockSteps * s->hist->ld);
         if (e == hipSuccess) e = hipMalloc(&s->G, sizeof(double) * 2 * kSubSteps * s->hist->ld);
         if (e == hipSuccess)
             e = hipMemcpy(s->atab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
@@ -1268,27 +1269,44 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                                s->hist->part);
             FRM_HIP(ctx, hipGetLastError());
         } else if (mf) {
+            // block b = n / 64 uses the far part of the slots before P0 =
+            // N0 - 64 (F of parity b), formed during block b - 1; this
+            // block's steps N0 + 8q form, slice q of 8, the next block's far
+            // part (slots before N0, all settled) into F of parity b + 1
             const int64_t ld = s->hist->ld;
             const int N0 = n / kBlockSteps * kBlockSteps;
+            const int P0 = std::max(0, N0 - kBlockSteps);
             const float* h32 = (const float*)s->hist->data;
-            if (N0 > 0 && n == N0) {
-                LaunchTimer t(ctx, FRM_K_BLOCKFAR, 2.0 * 2 * kBlockSteps * N0 * (double)s->hist->n_nodes);
-                if (block_far(ctx->stream, h32, ld, N0, s->atab, s->atab_chunks, s->mfma_nprod, s->F))
-                    return check_hip(ctx, hipGetLastError(), "block_far");
+            const size_t fsz = (size_t)2 * kBlockSteps * ld;
+            float* Fcur = s->F + (size_t)((N0 / kBlockSteps) & 1) * fsz;
+            float* Fnext = s->F + (size_t)(((N0 / kBlockSteps) + 1) & 1) * fsz;
+            const int q = (n - N0) / kSubSteps;
+            if (N0 > 0 && N0 + kBlockSteps < s->capacity && (n - N0) % kSubSteps == 0) {
+                const int groups = block_far_groups(ld);
+                const int per = (groups + kBlockSteps / kSubSteps - 1) / (kBlockSteps / kSubSteps);
+                const int wg0 = q * per, nwg = std::min(per, groups - wg0);
+                if (nwg > 0) {
+                    const double nodes = (double)s->hist->n_nodes * nwg / groups;
+                    LaunchTimer t(ctx, FRM_K_BLOCKFAR, nodes * (4.0 * N0 + 4.0 * 2 * kBlockSteps));
+                    if (block_far(ctx->stream, h32, ld, N0 + kBlockSteps, N0, s->atab, s->atab_chunks,
+                                  s->mfma_nprod, Fnext, wg0, nwg))
+                        return check_hip(ctx, hipGetLastError(), "block_far");
+                }
             }
-            // the settled part of the block, every 8 steps, in FP64
+            // the slots from P0 up to the sub-block start, every 8 steps, in FP64
             const int N1 = n / kSubSteps * kSubSteps;
-            if (N1 > N0 && n == N1) {
-                LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * ((N1 - N0) * 4.0 + 2 * kSubSteps * 8));
-                if (block_mid(ctx->stream, h32, ld, N0, N1, s->w, s->capacity, s->G))
+            if (N1 > P0 && n == N1) {
+                LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * ((N1 - P0) * 4.0 + 2 * kSubSteps * 8));
+                if (block_mid(ctx->stream, h32, ld, P0, N1, s->w, s->capacity, s->G))
                     return check_hip(ctx, hipGetLastError(), "block_mid");
             }
             nch = 1;
+            const bool useF = P0 > 0, useG = N1 > P0;
             LaunchTimer t(ctx, FRM_K_MEMTERM,
-                          (double)ld * ((n - N1) * 4.0 + (N0 > 0 ? 2 * 4 : 0) + (N1 > N0 ? 2 * 8 : 0) +
+                          (double)ld * ((n - N1) * 4.0 + (useF ? 2 * 4 : 0) + (useG ? 2 * 8 : 0) +
                                         2 * 8));
-            if (block_near(ctx->stream, h32, ld, N0, N1, n, s->w, s->capacity, s->F, N0 > 0, s->G,
-                           N1 > N0, s->hist->part))
+            if (block_near(ctx->stream, h32, ld, N0, N1, n, s->w, s->capacity, Fcur, useF, s->G, useG,
+                           s->hist->part))
                 return check_hip(ctx, hipGetLastError(), "block_near");
         } else {
             int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch,

@@ -2,17 +2,20 @@
 //
 // For Toeplitz weights the history sums of a block of future steps share
 // their settled past: for the steps n = N0 + j, j < 64, of a block starting
-// at N0,
-//   S_t(n) = sum_{s < N0} w_t(N0 + j - s) Delta_s  +  sum_{N0 <= s < n} w_t(n - s) Delta_s,
+// at N0, and P0 = N0 - 64,
+//   S_t(n) = sum_{s < P0} w_t(N0 + j - s) Delta_s  +  sum_{P0 <= s < n} w_t(n - s) Delta_s,
 // and the first ("far") part for all 64 j and both LOD targets t is one
-// dense product  F[t*64 + j][i] = sum_{s < N0} W[t*64 + j][s] Delta_s[i]  with
-// M = 128, K = N0, N = nodes -- read once per 64 steps instead of every step.
+// dense product  F[t*64 + j][i] = sum_{s < P0} W[t*64 + j][s] Delta_s[i]  with
+// M = 128, K = P0, N = nodes -- read once per 64 steps instead of every step.
+// Its slots are settled one block before it is needed, so it is formed
+// during the previous block in 8 node slices, one every 8 steps: the cost is
+// spread evenly over the steps instead of falling on the block's first.
 // The reference forms every step's sum from scratch (fr_d_t_g_2d.cpp:
 // 3340-3437).  W and Delta are split into two or three BF16 parts whose
 // cross products accumulate in FP32 on v_mfma_f32_16x16x32_bf16.  The rest
-// of the block is summed in FP64: every 8 steps the settled slots of the
-// block (k_block_mid), every step the F row, the mid row and at most 7
-// recent slots (k_block_near).
+// is summed in FP64: every 8 steps the settled slots from P0 on
+// (k_block_mid, up to 120 slots), every step the F row, the mid row and at
+// most 7 recent slots (k_block_near).
 #include <cstring>
 
 #include "frm_mfma.hpp"
@@ -61,8 +64,8 @@ static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block i
 // spills since the loader stores 16-byte pieces).
 template <int NP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_block_far(
-    const float* __restrict__ hist, int64_t ld, int N0, const bf16x8* __restrict__ atab,
-    float* __restrict__ F) {
+    const float* __restrict__ hist, int64_t ld, int N0, int K, const bf16x8* __restrict__ atab,
+    float* __restrict__ F, int wg0) {
     constexpr int NS = NP == 6 ? 3 : 2;    // BF16 parts per operand
     constexpr int CT = kNt / 16;           // column tiles per workgroup
     static_assert(kNt == 128 && kKc == 32, "the loader map takes 2 x 64 nodes, 4 x 8 slots");
@@ -70,14 +73,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int kg = lane >> 4, lc = lane & 15;
     const int r0 = (wave & 1) + 4 * (wave >> 1);  // row tiles r0, r0 + 2
-    const int64_t i0 = (int64_t)blockIdx.x * kNt;
+    const int64_t i0 = (int64_t)(blockIdx.x + wg0) * kNt;
     f32x4 acc[2][CT];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < CT; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
     // loader map: slots 8*sq .. 8*sq+7 of the chunk for nodes ng and ng + 64.
-    // N0 is a multiple of two chunk lengths (block_far checks it), so every
+    // K is a multiple of two chunk lengths (block_far checks it), so every
     // chunk is whole and the loads need no guard: a guarded load compiles to
     // a select between global and private addresses and a flat load, which
     // serialises the prefetch.
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     load(y, kKc);
     __builtin_amdgcn_sched_barrier(0);
     int s0 = 0;
-    for (; s0 + 2 * kKc < N0; s0 += 2 * kKc) {
+    for (; s0 + 2 * kKc < K; s0 += 2 * kKc) {
         chunk(x, s0 + kKc, s0 + 2 * kKc);
         chunk(y, s0 + 2 * kKc, s0 + 3 * kKc);
     }
@@ -181,10 +184,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 }
 
 // Mid part of a 64-step block, refreshed every kSubSteps steps: for the
-// sub-block starting at N1 (N0 < N1 < N0 + 64),
-//   G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i],  j < 8,
+// sub-block starting at N1 and the first slot P0 the far part leaves out
+// (P0 < N1 <= P0 + 120),
+//   G[t*8 + j][i] = sum_{P0 <= s < N1} w_t(N1 + j - s) Delta_s[i],  j < 8,
 // in FP64 (two nodes per thread, 8-byte loads of the FP32 slots; the
-// weights are wavefront-uniform).  N1 - N0 is a multiple of 8: the slots go
+// weights are wavefront-uniform).  N1 - P0 is a multiple of 8: the slots go
 // in batches of 8 whose loads are all issued before the first product, and
 // a batch's 8 x 8 products touch only the 15 lags N1 - s0 - 7 .. N1 - s0 + 7.
 // Terms are added in increasing s, as a plain loop over s would add them.
@@ -301,20 +305,24 @@ __global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hi
 #undef FRM_NEAR_CASE
 }
 
-int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, const void* atab,
-              int atab_chunks, int nprod, float* F) {
-    // the table must reach d = N0 + 32 (the carried fragment of the first chunk)
-    if (ld % kNt || N0 <= 0 || N0 % kBlockSteps || N0 / kKc + 1 > atab_chunks ||
-        (nprod != 3 && nprod != 6))
+int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, int K, const void* atab,
+              int atab_chunks, int nprod, float* F, int wg0, int nwg) {
+    // the table must reach d = N0 + 32 (the carried fragment of the first
+    // chunk); the last chunk's lag base is N0 - K + 32 >= 32
+    if (ld % kNt || N0 <= 0 || N0 % kBlockSteps || K <= 0 || K % kBlockSteps || K > N0 ||
+        N0 / kKc + 1 > atab_chunks || (nprod != 3 && nprod != 6) || wg0 < 0 || nwg <= 0 ||
+        (int64_t)wg0 + nwg > ld / kNt)
         return -1;
-    const dim3 grid((unsigned)(ld / kNt)), block(256);
+    const dim3 grid((unsigned)nwg), block(256);
     const bf16x8* a = (const bf16x8*)atab;
     if (nprod == 6)
-        hipLaunchKernelGGL(k_block_far<6>, grid, block, 0, st, hist, ld, N0, a, F);
+        hipLaunchKernelGGL(k_block_far<6>, grid, block, 0, st, hist, ld, N0, K, a, F, wg0);
     else
-        hipLaunchKernelGGL(k_block_far<3>, grid, block, 0, st, hist, ld, N0, a, F);
+        hipLaunchKernelGGL(k_block_far<3>, grid, block, 0, st, hist, ld, N0, K, a, F, wg0);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+int block_far_groups(int64_t ld) { return (int)(ld / kNt); }
 
 // Host: the weight-operand fragments for d = 32, 64, ..., 32*chunks.  Entry
 // [(d/32 - 1)*8 + rt][p][lane] holds, for row tile rt (target t = rt/4, steps
@@ -344,7 +352,7 @@ std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks) {
 
 int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, const double* w,
               int ldw, double* G) {
-    if (ld % 2 || N1 <= N0 || N1 - N0 >= kBlockSteps || (N1 - N0) % kSubSteps) return -1;
+    if (ld % 2 || N0 < 0 || N1 <= N0 || N1 - N0 >= 2 * kBlockSteps || (N1 - N0) % kSubSteps) return -1;
     const int64_t threads = ld / 2;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
     hipLaunchKernelGGL(k_block_mid, grid, block, 0, st, hist, ld, N0, N1, w, ldw, G);

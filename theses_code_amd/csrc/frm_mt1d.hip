@@ -18,8 +18,10 @@
 //      the push of the new increment.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "frm_history.hpp"
@@ -798,6 +800,39 @@ __global__ __launch_bounds__(256) void k_fit_after(FitArgs a) {
 
 using namespace frm;
 
+namespace frm {
+// A kernel launch of one step: function, shape and a copy of its argument
+// values (hipKernelNodeParams takes pointers to them).
+struct KRec {
+    const void* func = nullptr;
+    dim3 grid, block;
+    alignas(16) unsigned char buf[448];
+    size_t used = 0;
+    size_t off[16];
+    int n = 0;
+    template <class T>
+    void arg(const T& v) {
+        static_assert(sizeof(T) <= 448, "kernel argument too large");
+        used = (used + alignof(T) - 1) / alignof(T) * alignof(T);
+        std::memcpy(buf + used, &v, sizeof(T));
+        off[n++] = used;
+        used += sizeof(T);
+    }
+    void ptrs(void** p) {
+        for (int i = 0; i < n; ++i) p[i] = buf + off[i];
+    }
+};
+// One step's launches as an instantiated graph (a chain of kernel nodes),
+// reused for every step that issues the same kernels in the same order: only
+// the nodes' arguments and grids change between steps.
+struct StepGraph {
+    std::vector<const void*> funcs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+};
+}  // namespace frm
+
 struct frm_mt1d {
     frm_ctx* ctx = nullptr;
     int nprob = 0, kind = 0, capacity = 0, precision = 0, picard = 1;
@@ -811,8 +846,9 @@ struct frm_mt1d {
     MtProbDev* probs = nullptr;
     int* gprob = nullptr;
     int* gvalid = nullptr;    // real nodes per group (0 for padding groups)
-    int graph = 0;                  // capture multi-step calls in a HIP graph
-    hipGraphExec_t gexec = nullptr; // the last captured sequence (launched)
+    int graph = 0;                  // steps as HIP graphs, built once per launch sequence
+    std::vector<frm::StepGraph> graphs;  // one instantiated graph per launch sequence
+    std::vector<frm::KRec>* rec = nullptr;  // while set, launches are recorded, not issued
     int64_t hnodes = 0, hnodes_mt = 0;  // real history nodes (all; full-history problems)
     double* U = nullptr;
     void* hist = nullptr;     // slots * ldh (FP64 or FP32)
@@ -847,12 +883,36 @@ struct frm_mt1d {
     std::vector<int> fit_off;        // [capacity+1]: observations of step k at fit_idx[off[k-1] ..]
 };
 
+// Issues a kernel launch on the context's stream, or records it for the
+// step graph while s->rec is set; arguments are converted to the kernel's
+// parameter types.
+template <typename... KArgs, typename... Args>
+static hipError_t mt_launch(frm_mt1d* s, void (*kern)(KArgs...), dim3 grid, dim3 block,
+                            Args&&... args) {
+    static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
+    frm::KRec r;
+    r.func = (const void*)kern;
+    r.grid = grid;
+    r.block = block;
+    (r.arg(static_cast<std::decay_t<KArgs>>(args)), ...);
+    if (s->rec) {
+        s->rec->push_back(r);
+        return hipSuccess;
+    }
+    void* p[16];
+    r.ptrs(p);
+    return hipLaunchKernel(r.func, grid, block, p, 0, s->ctx->stream);
+}
+
 static void mt_free(frm_mt1d* s) {
     if (!s) return;
     (void)hipFree(s->probs);
     (void)hipFree(s->gprob);
     (void)hipFree(s->gvalid);
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    for (auto& g : s->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     (void)hipFree(s->U);
     (void)hipFree(s->hist);
     (void)hipFree(s->part);
@@ -1226,17 +1286,15 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             fa.err = s->fit_err;
             fa.dead = s->fit_dead;
             LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
-            hipLaunchKernelGGL(k_fit_before, dim3((unsigned)((s->nprob + 255) / 256)), dim3(256), 0,
-                               ctx->stream, fa);
-            FRM_HIP(ctx, hipGetLastError());
+            FRM_HIP(ctx, mt_launch(s, k_fit_before, dim3((unsigned)((s->nprob + 255) / 256)),
+                                   dim3(256), fa));
         }
         // the step's fitting error, once U holds the new state
         auto fit_after = [&]() -> int {
             if (fcnt > 0) {
                 LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
-                hipLaunchKernelGGL(k_fit_after, dim3((unsigned)((s->nprob + 255) / 256)), dim3(256),
-                                   0, ctx->stream, fa);
-                FRM_HIP(ctx, hipGetLastError());
+                FRM_HIP(ctx, mt_launch(s, k_fit_after, dim3((unsigned)((s->nprob + 255) / 256)),
+                                       dim3(256), fa));
             }
             s->tt += s->tau;
             return FRM_OK;
@@ -1245,9 +1303,8 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             const int threads = 256;
             dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
             LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
-            hipLaunchKernelGGL(k_psi_weights, grid, dim3(threads), 0, ctx->stream, s->probs,
-                               s->nprob, k, depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad);
-            FRM_HIP(ctx, hipGetLastError());
+            FRM_HIP(ctx, mt_launch(s, k_psi_weights, grid, dim3(threads), s->probs, s->nprob, k,
+                                   depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad));
         }
         int nch = 0;
         if (depth > 0 && s->ser_count < s->nprob) {
@@ -1261,14 +1318,14 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_MEMTERM,
                           (double)depth * (s->gprob_mt ? s->hnodes_mt : s->hnodes) * esz);
             if (f32)
-                hipLaunchKernelGGL(k_mt_memterm_f32c, grid, dim3(kMemtermThreads), 0, ctx->stream,
-                                   (const float*)s->hist, s->ldh, depth, s->w32, s->ldw, s->gprob,
-                                   s->gvalid, chunk, s->part);
+                FRM_HIP(ctx, mt_launch(s, k_mt_memterm_f32c, grid, dim3(kMemtermThreads),
+                                       (const float*)s->hist, s->ldh, depth, s->w32, s->ldw,
+                                       s->gprob, s->gvalid, chunk, s->part));
             else
-                hipLaunchKernelGGL(k_mt_memterm_f64, grid, dim3(kMemtermThreads), 0, ctx->stream,
-                                   (const double*)s->hist, s->ldh, depth, s->w, s->ldw,
-                                   s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid, chunk, s->part);
-            FRM_HIP(ctx, hipGetLastError());
+                FRM_HIP(ctx, mt_launch(s, k_mt_memterm_f64, grid, dim3(kMemtermThreads),
+                                       (const double*)s->hist, s->ldh, depth, s->w, s->ldw,
+                                       s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid, chunk,
+                                       s->part));
         }
         if (s->kind == FRM_MT_AB && s->ser_count > 0 && k >= 2) {
             AbSeriesArgs c;
@@ -1285,9 +1342,8 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             c.i = k;
             c.bad = s->bad;
             LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
-            hipLaunchKernelGGL(k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)), dim3(256), 0,
-                               ctx->stream, c);
-            FRM_HIP(ctx, hipGetLastError());
+            FRM_HIP(ctx, mt_launch(s, k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)), dim3(256),
+                                   c));
         }
         if (s->kind == FRM_MT_AB) {
             AbSweepArgs b;
@@ -1309,12 +1365,13 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             {
                 LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
                 const dim3 g((unsigned)((s->nprob + 3) / 4));
+                hipError_t e;
                 switch (mt_pick_cm(s->maxN)) {
-                    case 4: hipLaunchKernelGGL(k_ab1d_sweep<4>, g, dim3(256), 0, ctx->stream, b); break;
-                    case 8: hipLaunchKernelGGL(k_ab1d_sweep<8>, g, dim3(256), 0, ctx->stream, b); break;
-                    default: hipLaunchKernelGGL(k_ab1d_sweep<16>, g, dim3(256), 0, ctx->stream, b);
+                    case 4: e = mt_launch(s, k_ab1d_sweep<4>, g, dim3(256), b); break;
+                    case 8: e = mt_launch(s, k_ab1d_sweep<8>, g, dim3(256), b); break;
+                    default: e = mt_launch(s, k_ab1d_sweep<16>, g, dim3(256), b);
                 }
-                FRM_HIP(ctx, hipGetLastError());
+                FRM_HIP(ctx, e);
             }
             s->depth = depth + 1;
             s->tstep = k + 1;
@@ -1335,60 +1392,97 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         {
             LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
             const dim3 g((unsigned)((s->nprob + 3) / 4));
+            hipError_t e;
             switch (mt_pick_cm(s->maxN)) {
-                case 4: hipLaunchKernelGGL(k_mt_sweep<4>, g, dim3(256), 0, ctx->stream, a); break;
-                case 8: hipLaunchKernelGGL(k_mt_sweep<8>, g, dim3(256), 0, ctx->stream, a); break;
-                default: hipLaunchKernelGGL(k_mt_sweep<16>, g, dim3(256), 0, ctx->stream, a);
+                case 4: e = mt_launch(s, k_mt_sweep<4>, g, dim3(256), a); break;
+                case 8: e = mt_launch(s, k_mt_sweep<8>, g, dim3(256), a); break;
+                default: e = mt_launch(s, k_mt_sweep<16>, g, dim3(256), a);
             }
-            FRM_HIP(ctx, hipGetLastError());
+            FRM_HIP(ctx, e);
         }
         if (k >= 3) s->depth = depth + 1;
         s->tstep = k + 1;
         if (int rc = fit_after()) return rc;
         return FRM_OK;
     };
-    // With graph capture the whole sequence of steps is recorded into one
-    // HIP graph (every step's arguments as it is queued) and launched once;
-    // not while per-launch timing events are on.
-    const bool cap = s->graph && nsteps > 1 && ctx->timing_mask == 0;
-    const int tstep0 = s->tstep, depth0 = s->depth;
-    const double tt0 = s->tt;
-    // a capture that fails queues nothing: the solver stays at its step
-    auto uncapture = [&]() {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(ctx->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        s->tstep = tstep0;
-        s->depth = depth0;
-        s->tt = tt0;
-    };
-    if (cap) {
-        if (s->gexec) {  // the previous call's graph has been launched
-            FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            (void)hipGraphExecDestroy(s->gexec);
-            s->gexec = nullptr;
-        }
-        FRM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    // Graph mode: each step's launches are recorded, matched with the graph
+    // built for the same kernel sequence (a chain of kernel nodes,
+    // instantiated once), the nodes get this step's arguments and grids
+    // (hipGraphExecKernelNodeSetParams) and the graph is launched.  Steps
+    // differ only in arguments and grids (depth, time-chunk count, slots), so
+    // a run uses a handful of graphs.  Not while per-launch timing is on.
+    if (!s->graph || ctx->timing_mask != 0) {
+        for (int it = 0; it < nsteps; ++it)
+            if (int rc = step_once()) return rc;
+        return FRM_OK;
     }
+    std::vector<KRec> recs;
     for (int it = 0; it < nsteps; ++it) {
-        if (int rc = step_once()) {
-            if (cap) uncapture();
-            return rc;
-        }
-    }
-    if (cap) {
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-        if (e == hipSuccess) e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
-        if (g) (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            s->gexec = nullptr;
+        recs.clear();
+        const int tstep0 = s->tstep, depth0 = s->depth;
+        const double tt0 = s->tt;
+        s->rec = &recs;
+        const int rc = step_once();
+        s->rec = nullptr;
+        // a step that is not launched leaves the solver where it was
+        auto unstep = [&](int code) {
             s->tstep = tstep0;
             s->depth = depth0;
             s->tt = tt0;
+            return code;
+        };
+        if (rc) return unstep(rc);
+        StepGraph* g = nullptr;
+        for (auto& c : s->graphs) {
+            if (c.funcs.size() != recs.size()) continue;
+            bool same = true;
+            for (size_t i = 0; i < recs.size() && same; ++i) same = c.funcs[i] == recs[i].func;
+            if (same) {
+                g = &c;
+                break;
+            }
         }
-        FRM_HIP(ctx, e);
-        FRM_HIP(ctx, hipGraphLaunch(s->gexec, ctx->stream));
+        void* ptrs[16];
+        auto params = [&](KRec& r) {
+            r.ptrs(ptrs);
+            hipKernelNodeParams p{};
+            p.blockDim = r.block;
+            p.extra = nullptr;
+            p.func = const_cast<void*>(r.func);
+            p.gridDim = r.grid;
+            p.kernelParams = ptrs;
+            p.sharedMemBytes = 0;
+            return p;
+        };
+        if (!g) {
+            StepGraph ng;
+            FRM_HIP(ctx, hipGraphCreate(&ng.graph, 0));
+            hipError_t e = hipSuccess;
+            for (size_t i = 0; i < recs.size() && e == hipSuccess; ++i) {
+                hipKernelNodeParams p = params(recs[i]);
+                hipGraphNode_t node = nullptr;
+                e = hipGraphAddKernelNode(&node, ng.graph, i ? &ng.nodes.back() : nullptr, i ? 1 : 0,
+                                          &p);
+                ng.nodes.push_back(node);
+                ng.funcs.push_back(recs[i].func);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                if (ng.exec) (void)hipGraphExecDestroy(ng.exec);
+                (void)hipGraphDestroy(ng.graph);
+                return unstep(check_hip(ctx, e, "frm_mt1d_step: step graph"));
+            }
+            s->graphs.push_back(ng);
+            g = &s->graphs.back();
+        } else {
+            for (size_t i = 0; i < recs.size(); ++i) {
+                hipKernelNodeParams p = params(recs[i]);
+                const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &p);
+                if (e != hipSuccess) return unstep(check_hip(ctx, e, "frm_mt1d_step: node update"));
+            }
+        }
+        const hipError_t e = hipGraphLaunch(g->exec, ctx->stream);
+        if (e != hipSuccess) return unstep(check_hip(ctx, e, "frm_mt1d_step: graph launch"));
     }
     return FRM_OK;
 }
