@@ -6,11 +6,12 @@
  * bandwidth the memory-term roofline is compared against, and the known-size
  * kernel that scales rocprof FETCH_SIZE / WRITE_SIZE counters.
  *
- * frm_diag_read streams `bytes` of device memory with 16-byte non-temporal
- * loads, four in flight per lane, and writes one 16-byte sum per lane (the
+ * frm_diag_read streams `bytes` (a multiple of 8 MiB) of device memory as
+ * rows of 8 MiB, each lane summing one 16-byte column over all rows with
+ * non-temporal loads, four rows in flight, and writing its sum once (the
  * access pattern of the history sums): the measured read bandwidth, which
  * the read-dominated kernels are compared against.  It allocates its own
- * small buffer of per-lane sums (16 MiB). */
+ * buffer of per-lane sums (8 MiB). */
 #ifndef FRM_DIAG_H
 #define FRM_DIAG_H
 #include <stddef.h>

@@ -4,30 +4,32 @@
 #include "frm_internal.hpp"
 
 namespace frm {
-// Each lane sums 16-byte pieces of the source, four in flight, with
-// non-temporal loads (the access pattern of the history sums: read once,
-// streamed), and writes its sum once: the written bytes are 16 per lane.
+// The buffer read as rows of kReadCols 16-byte pieces (8 MiB); lane t sums
+// column t over all rows with non-temporal loads, four rows in flight, and
+// writes its sum once: the access pattern of the history sums (one pass per
+// node column over every slot, all wavefronts walking the same rows
+// together, 512-thread workgroups).
+constexpr int64_t kReadCols = int64_t(1) << 19;
 typedef double diag_dv2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(512) void k_read16(const diag_dv2* __restrict__ src,
-                                                diag_dv2* __restrict__ dst, int64_t n) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+                                                diag_dv2* __restrict__ dst, int64_t rows) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const diag_dv2* p = src + t;
     double ax = 0.0, ay = 0.0;
-    int64_t i = t;
-    for (; i + 3 * stride < n; i += 4 * stride) {
+    int64_t r = 0;
+    for (; r + 4 <= rows; r += 4) {
         diag_dv2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(p + (r + u) * kReadCols);
 #pragma unroll
         for (int u = 0; u < 4; ++u) ax += v[u].x, ay += v[u].y;
     }
-    for (; i < n; i += stride) {
-        const diag_dv2 v = __builtin_nontemporal_load(src + i);
+    for (; r < rows; ++r) {
+        const diag_dv2 v = __builtin_nontemporal_load(p + r * kReadCols);
         ax += v.x, ay += v.y;
     }
     dst[t] = diag_dv2{ax, ay};
 }
-
 __global__ __launch_bounds__(256) void k_copy16(const double2* __restrict__ src,
                                                 double2* __restrict__ dst, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -63,12 +65,12 @@ extern "C" int frm_diag_copy(frm_ctx* ctx, void* d_dst, const void* d_src, size_
 
 extern "C" int frm_diag_read(frm_ctx* ctx, const void* d_src, size_t bytes, int reps,
                              double* avg_ms) {
-    FRM_REQUIRE(ctx, ctx && d_src && reps > 0 && bytes % 16 == 0 && bytes >= 16,
-                "frm_diag_read: bad arguments");
-    const int64_t n = (int64_t)(bytes / 16);
-    const dim3 grid(256 * 8), block(512);
+    FRM_REQUIRE(ctx, ctx && d_src && reps > 0 && bytes > 0 && bytes % (16 * frm::kReadCols) == 0,
+                "frm_diag_read: bad arguments (bytes a multiple of 8 MiB)");
+    const int64_t rows = (int64_t)(bytes / (16 * frm::kReadCols));
+    const dim3 grid((unsigned)(frm::kReadCols / 512)), block(512);
     void* sums = nullptr;
-    FRM_HIP(ctx, hipMalloc(&sums, (size_t)grid.x * block.x * 16));
+    FRM_HIP(ctx, hipMalloc(&sums, (size_t)frm::kReadCols * 16));
     hipEvent_t a, b;
     hipError_t e = hipEventCreate(&a);
     if (e == hipSuccess) e = hipEventCreate(&b);
@@ -77,11 +79,11 @@ extern "C" int frm_diag_read(frm_ctx* ctx, const void* d_src, size_t bytes, int 
         return frm::check_hip(ctx, e, "frm_diag_read: hipEventCreate");
     }
     hipLaunchKernelGGL(frm::k_read16, grid, block, 0, ctx->stream, (const frm::diag_dv2*)d_src,
-                       (frm::diag_dv2*)sums, n);  // warm
+                       (frm::diag_dv2*)sums, rows);  // warm
     (void)hipEventRecord(a, ctx->stream);
     for (int r = 0; r < reps; ++r)
         hipLaunchKernelGGL(frm::k_read16, grid, block, 0, ctx->stream,
-                           (const frm::diag_dv2*)d_src, (frm::diag_dv2*)sums, n);
+                           (const frm::diag_dv2*)d_src, (frm::diag_dv2*)sums, rows);
     (void)hipEventRecord(b, ctx->stream);
     e = hipEventSynchronize(b);
     float ms = 0.f;
