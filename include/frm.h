@@ -320,9 +320,10 @@ int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
  * tolerance of its coefficients).  The node-independent coefficients of
  * every step are formed on the host at creation (ab_maxl * 5 doubles per
  * step and distinct (alpha, ab_maxl, ab_maxkiter, ab_eps)).  ab_exact must
- * be 0.  When a series term is not finite the reference replaces the sum by
- * a sorted pyramid sum (600-609); this path is not implemented and
- * frm_mt1d_get_state then returns FRM_E_STATE. */
+ * be 0.  Where a term of MF_intF's sum is not finite the sum restarts from
+ * the sorted pyramid sum of all finite terms (600-609), on the GPU as in the
+ * reference, for ab_maxl up to 4096; above that frm_mt1d_get_state returns
+ * FRM_E_STATE when it happens. */
 int frm_mt1d_ab_error(frm_mt1d* s, double* out);
 
 /* Fitting error of every problem against one set of observations, as the

@@ -978,6 +978,11 @@ void orc_psi_weights(double alpha, double beta, double tau, int A, double* out) 
     psi_cache_free(&c);
 }
 
+/* calc_int's fallback alone: _kb(1, t0, t1, gtj, sing, alpha) */
+double orc_psi_kb_quad(double alpha, double beta, double t0, double t1, double gtj, int sing) {
+    return psi_kb_adapt(beta, 1, t0, t1, gtj, sing, alpha);
+}
+
 /* ===================================================================== */
 /* 1D H_solver (mt_minimal.cpp / mt_psi_caputo.cpp)                       */
 /* ===================================================================== */

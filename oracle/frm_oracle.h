@@ -95,6 +95,8 @@ void orc_mt_history(int N, int psi, double beta, double alpha, double tau, int o
 /* psi weight calc_int(alpha, a, a+1, A, tau, 1) (mt_psi_caputo.cpp:720-740)
  * for a = 0..A-1, g(t) = t^beta. */
 void orc_psi_weights(double alpha, double beta, double tau, int A, double* out);
+/* the adaptive-quadrature fallback of calc_int (_kb, mt_psi_caputo.cpp:476-494) */
+double orc_psi_kb_quad(double alpha, double beta, double t0, double t1, double gtj, int sing);
 
 /* ---- W4/H5/S3: Atangana-Baleanu path of MF_int_series.cpp -------------- */
 double orc_mittag_leffler(double a, double b, double z, double eps);   /* :247-302 */

@@ -61,6 +61,8 @@ def lib():
         L.orc_mt_da.argtypes = [_V]
         L.orc_mt_history.argtypes = [_I, _I, _D, _D, _D, _I, _I, _P, _P]
         L.orc_psi_weights.argtypes = [_D, _D, _D, _I, _P]
+        L.orc_psi_kb_quad.restype = _D
+        L.orc_psi_kb_quad.argtypes = [_D, _D, _D, _D, _D, _I]
         if hasattr(L, "orc_mittag_leffler"):
             L.orc_mittag_leffler.restype = _D
             L.orc_mittag_leffler.argtypes = [_D, _D, _D, _D]
@@ -259,6 +261,11 @@ def psi_weights(alpha, beta, tau, A) -> np.ndarray:
     out = np.empty(A)
     lib().orc_psi_weights(alpha, beta, tau, A, _p(out))
     return out
+
+
+def psi_kb_quad(alpha, beta, t0, t1, gtj, sing) -> float:
+    """calc_int's adaptive-quadrature fallback _kb(1, t0, t1, gtj, sing, alpha)."""
+    return lib().orc_psi_kb_quad(alpha, beta, t0, t1, gtj, int(sing))
 
 
 def set_threads(n: int) -> None:

@@ -21,6 +21,9 @@
 //   2: psi weights only (mt_psi_caputo.cpp:720-740). Input: mode, alpha,
 //      beta, tau, nA, A[nA]. Output for each A: calc_int(alpha,a,a+1,A,tau,1)
 //      for a = 0..A-1.
+//   3: calc_int's quadrature fallback alone (mt_psi_caputo.cpp:476-494).
+//      Input: mode, n, then n x (alpha, beta, t0, t1, gtj, sing).  Output:
+//      _kb(1, t0, t1, gtj, sing, alpha) for each.
 #include "harness_io.h"
 #define main ref_main
 #ifdef REF_PSI
@@ -116,6 +119,21 @@ int main(int argc, char** argv) {
         }
 #else
         std::fprintf(stderr, "mode 2 needs REF_PSI\n");
+        return 1;
+#endif
+    } else if (mode == 3) {
+#ifdef REF_PSI
+        int n = in.i();
+        func_in_kernel = 3;
+        H_solver* s = make_solver(1, 4, 0.5, 1e-3, 1.0, 1e-3, 0.0, 1e300, 0);
+        for (int q = 0; q < n; ++q) {
+            double alpha = in.d(), beta = in.d(), t0 = in.d(), t1 = in.d(), gtj = in.d();
+            int sing = in.i();
+            func_power = beta;
+            out.d(s->_kb(1, t0, t1, gtj, sing, alpha));
+        }
+#else
+        std::fprintf(stderr, "mode 3 needs REF_PSI\n");
         return 1;
 #endif
     }

@@ -163,6 +163,24 @@ def psi_weights(name, cases, As):
     save(name, meta, **tabs)
 
 
+def psi_kb_quad(name):
+    """calc_int's quadrature fallback _kb(1, t0, t1, gtj, sing, alpha) alone
+    (harness mode 3): intervals away from the target and ending at it
+    (sing), g = t^beta for beta 1 (closed form known), 0.8, 1.4 and 80."""
+    cases = []
+    for alpha in (0.05, 0.3, 0.6, 0.9):
+        for beta in (1.0, 0.8, 1.4):
+            for t0, t1, tj, sing in ((0.0, 0.01, 0.02, 0), (0.3, 0.4, 0.4, 1), (0.0, 1.0, 1.0, 1),
+                                     (0.2, 0.25, 0.9, 0), (1.5, 2.0, 2.0, 1)):
+                cases.append((alpha, beta, t0, t1, tj ** beta, sing))
+        for t0, t1, tj, sing in ((0.0, 0.01, 0.5, 0), (0.49, 0.5, 0.5, 1), (0.0, 0.01, 0.03, 0)):
+            cases.append((alpha, 80.0, t0, t1, tj ** 80.0, sing))
+    cases = np.array(cases)
+    out = run("ref_psi", [3, len(cases), *cases.reshape(-1)])
+    save(name, dict(source="mt_psi_caputo.cpp _kb(1, t0, t1, gtj, sing, alpha)",
+                    columns=["alpha", "beta", "t0", "t1", "gtj", "sing"]), cases=cases, kb=out)
+
+
 # ---------------------------------------------------------------- AB
 def mf_all(name):
     arrays, meta = {}, dict(source="MF_int_series.cpp")
@@ -219,6 +237,27 @@ def mf_series(name):
     save(name, meta, **arrays)
 
 
+def mf_recount(name):
+    """solve_and_check with simple = 0 where MF_intF's power-series terms
+    overflow (T > 1, long series): the sum restarts from the sorted pyramid
+    sum of the finite terms (MF_int_series.cpp:600-609)."""
+    arrays, meta = {}, dict(source="MF_int_series.cpp solve_and_check, simple = 0, MF_intF recount")
+    rng = np.random.default_rng(92)
+    hcases = [(0.3, 0.05, 600, 1000, 1e-10, 80)]
+    for q, c in enumerate(hcases):
+        fvs = rng.normal(0, 1, c[-1] - 1)
+        arrays[f"fvs{q}"] = fvs
+        out = run("ref_mf", [4, *c, *fvs])
+        arrays[f"h{q}"] = out[: c[-1] - 1]
+        arrays[f"choice{q}"] = out[c[-1] - 1:]
+    meta["hist"] = hcases
+    scases = [(0.3, 4.0, 80, 30, 0, 600, 1000, 1e-10, 0), (0.3, 4.0, 80, 30, 0, 600, 1000, 1e-10, 2),
+              (0.25, 3.0, 60, 20, 0, 700, 1000, 1e-10, 1)]
+    arrays["solve_err"] = np.array([run("ref_mf", [2, *c])[0] for c in scases])
+    meta["solve"] = scases
+    save(name, meta, **arrays)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         sys.exit("the reference sources are needed to write golden vectors")
@@ -248,8 +287,17 @@ def main():
     mt_hist("psi_hist", "ref_psi", 25, 10, 2, 0.65, 0.01, 18, 19, beta=1.3)
     psi_weights("psi_w", [(0.6, 0.8, 1e-3), (0.9, 1.4, 1e-3), (0.55, 1.0, 1e-2)],
                 [1, 2, 5, 17, 64, 200])
+    # g = t^beta with large beta and small tau: g(A tau) is so small that
+    # some weight series are not finite and calc_int falls back to _kb
+    psi_kb_quad("psi_kb_quad")
+    psi_weights("psi_w_quad", [(0.05, 80.0, 0.01), (0.3, 40.0, 1e-4), (0.6, 80.0, 1e-4)],
+                [3, 10, 50])
+    mt_traj("psi_quad_flux", "ref_psi", 26, 10, 2, 0.05, 0.01, 1.0, 1e-3, 0.0, -20.0, 1, 40, 20,
+            lambda x, r: -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * r.normal(size=x.size),
+            beta=80.0)
     mf_all("mf")
     mf_series("mf_series")
+    mf_recount("mf_recount")
     fitio_all()
 
 

@@ -144,6 +144,40 @@ def test_ab1d_series_mixed_batch_matches_oracle(ctx):
         assert errs[i] == pytest.approx(e_ref, rel=TOL_ERR), i
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_ab1d_series_recount_matches_reference(ctx, golden, graph):
+    # tau 0.05 to T = 3 and 4 with 600-700 terms: the power-series
+    # accumulators overflow and MF_intF's sum restarts from the sorted pyramid
+    # of the finite terms (MF_int_series.cpp:600-609) at the nodes and steps
+    # after that; one problem of the batch (maxl 50) never overflows.  The
+    # 60-step case is read at step 60, the others at step 80.
+    m, z = golden("mf_recount")
+    cases = m["solve"]
+    tau = cases[0][1] / cases[0][2]
+    probs = [_series_prob(a, mm, maxl, F, maxk, eps)
+             for a, T, n, mm, simple, maxl, maxk, eps, F in cases]
+    probs.append(_series_prob(0.3, 25, 50, 1))
+    nmax = max(c[2] for c in cases)
+    b = S.MT1DBatch(ctx, probs, tau, nmax, [np.zeros(p["N"] + 2) for p in probs], kind=N.MT_AB,
+                    graph=graph)
+    done = 0
+    for n in sorted({c[2] for c in cases}):
+        b.calc_step((n - done) // 2)
+        b.calc_step(n - done - (n - done) // 2)
+        done = n
+        errs, states = b.ab_error(), b.state()
+        for i, (a, T, nn, mm, simple, maxl, maxk, eps, F) in enumerate(cases):
+            if nn != n:
+                continue
+            print(f"recount case {i}: a={a} T={T} n={nn} maxl={maxl} F={F}", flush=True)
+            _, Cref = O.ab_solve_series(a, T, nn, mm, maxl, maxk, eps, F)
+            assert relerr(states[i][:mm + 1], Cref) <= TOL, i
+            assert errs[i] == pytest.approx(z["solve_err"][i], rel=TOL_ERR), i
+    b.close()
+    _, Cref = O.ab_solve_series(0.3, tau * nmax, nmax, 25, 50, 1000, 1e-10, 1)
+    assert relerr(states[-1][:26], Cref) <= TOL
+
+
 def test_ab1d_series_restart_clears_accumulators(ctx):
     prob = _series_prob(0.8, 20, 50, 2)
     b = _batch(ctx, [prob], 1.0 / 30, 30)

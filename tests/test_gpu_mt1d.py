@@ -31,7 +31,7 @@ def relerr(a, b):
 
 
 @pytest.mark.parametrize("name,kind", [("mt_flux", N.MT_L1), ("mt_dir_cv2", N.MT_L1),
-                                       ("psi_flux", N.MT_PSI)])
+                                       ("psi_flux", N.MT_PSI), ("psi_quad_flux", N.MT_PSI)])
 def test_mt1d_matches_reference_golden(ctx, golden, name, kind):
     m, z = golden(name)
     prob = dict(N=m["N"], L=m["L"], alpha=m["alpha"], beta=m["beta"] or 1.0, form=N.FORM_CV2,
@@ -109,6 +109,58 @@ def test_mt1d_uneven_batch_matches_oracle(ctx, kind, picard):
         o.step(steps)
         assert relerr(g, o.state()) <= TOL_STRESS, p
     b.close()
+
+
+def test_psi_kb_quad_device_matches_reference(ctx, golden):
+    # the device restatement of calc_int's quadrature fallback (_kb) on the
+    # reference's own values: intervals away from and ending at the target
+    # (sing), g = t^beta for beta 1, 0.8, 1.4, 80.  GPU pow and glibc pow
+    # differ in the last bit.  Away from the target that moves the result by
+    # rounding only.  On the interval ending at the target, ___kb's march
+    # chooses each step from ratios of neighbouring integrand values, so a
+    # last-bit change moves its whole path: perturbing glibc's pow by +-1 ulp
+    # at random moves the reference's own value by up to 1.8e-9 (alpha 0.3,
+    # 7e-11 for beta <= 1.4), 1.3e-6 (alpha 0.6) and 8.7e-3 (alpha 0.9) on
+    # these cases, the size of the rule's error against the closed form at
+    # alpha 0.9 (2e-2).  The bounds below are those spreads with margin.
+    m, z = golden("psi_kb_quad")
+    cases = z["cases"]
+    got = ctx.psi_kb_quad(cases)
+    for r, g, want in zip(cases, got, z["kb"]):
+        alpha, sing = r[0], r[5]
+        tol = 1e-10 if not sing else (3e-8 if alpha < 0.5 else 1e-5 if alpha < 0.8 else 3e-2)
+        assert abs(g - want) <= tol * abs(want), (r, g, want)
+
+
+def _psi_quad_batch():
+    # tau 1e-2 with g = t^80: the series of interval [0, tau] is not finite at
+    # every step, and calc_int falls back to _kb there; one problem with
+    # beta 0.8 takes the series only
+    rng = np.random.default_rng(77)
+    probs, U0 = [], []
+    for n, a, b in ((20, 0.05, 80.0), (33, 0.1, 80.0), (26, 0.6, 0.8), (41, 0.05, 80.0)):
+        probs.append(dict(N=n, L=1.0, alpha=a, beta=b, form=N.FORM_CV2, k=1e-3, Ev=0.0,
+                          top=N.TOP_AUTO, bottom_cond=1, Hbottom=-20.0))
+        x = np.arange(n + 2) / n
+        u = -20.0 + 2.0 * np.sin(np.pi * x) + 0.1 * rng.normal(size=n + 2)
+        u[n + 1] = u[n]
+        U0.append(u)
+    return probs, U0
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_psi_quadrature_fallback_batch_matches_oracle(ctx, graph):
+    probs, U0 = _psi_quad_batch()
+    tau, steps = 1e-2, 40
+    b = S.MT1DBatch(ctx, probs, tau, steps, U0, kind=N.MT_PSI, graph=graph)
+    b.calc_step(17)
+    b.calc_step(steps - 17)
+    got = b.state()
+    b.close()
+    for p, u0, g in zip(probs, U0, got):
+        o = _oracle_for(p, tau, u0, N.MT_PSI, 1)
+        o.step(steps)
+        assert relerr(g, o.state()) <= TOL_STRESS, p
 
 
 def test_mt1d_f32c_within_1e6_of_fp64(ctx):

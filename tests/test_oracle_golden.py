@@ -84,7 +84,8 @@ def test_tg_binomial_series_matches_reference(golden, name):
     assert not np.any(z["kb"][:, :n - 1])
 
 
-@pytest.mark.parametrize("name,psi", [("mt_flux", 0), ("mt_dir_cv2", 0), ("psi_flux", 1)])
+@pytest.mark.parametrize("name,psi", [("mt_flux", 0), ("mt_dir_cv2", 0), ("psi_flux", 1),
+                                      ("psi_quad_flux", 1)])
 def test_mt_matches_reference(golden, name, psi):
     m, z = golden(name)
     s = O.MTSolver(m["N"], m["L"], m["alpha"], m["tau"], z["U0"], psi=psi,
@@ -113,6 +114,38 @@ def test_psi_weights_match_reference(golden):
         for A in m["As"]:
             np.testing.assert_allclose(O.psi_weights(a, b, tau, A), w[off:off + A], rtol=1e-14)
             off += A
+
+
+def test_psi_weights_with_quadrature_fallback_match_reference(golden):
+    # g = t^80 with tau 1e-2 / 1e-4: some series are not finite and calc_int
+    # takes _kb's adaptive quadrature (mt_psi_caputo.cpp:736-737)
+    m, z = golden("psi_w_quad")
+    for q, (a, b, tau) in enumerate(m["cases"]):
+        w, off = z[f"w{q}"], 0
+        for A in m["As"]:
+            np.testing.assert_allclose(O.psi_weights(a, b, tau, A), w[off:off + A], rtol=1e-14)
+            off += A
+
+
+def test_psi_kb_quad_matches_reference(golden):
+    m, z = golden("psi_kb_quad")
+    got = np.array([O.psi_kb_quad(*r[:5], int(r[5])) for r in z["cases"]])
+    np.testing.assert_array_equal(got, z["kb"])
+
+
+def test_psi_kb_quad_closed_form():
+    # beta = 1: the integral of (gtj - t)^-alpha is
+    # ((gtj - t0)^(1-alpha) - (gtj - t1)^(1-alpha)) / (1 - alpha).  The
+    # reference's rule stops where the 1- and 2-panel midpoint values differ
+    # by 1e-5, so the error is that size per leaf, larger on the interval
+    # ending at the singularity (sing) at high alpha.
+    for alpha in (0.05, 0.3, 0.6, 0.9):
+        for t0, t1, tj, sing in ((0.0, 0.01, 0.02, 0), (0.2, 0.25, 0.9, 0), (0.3, 0.4, 0.4, 1),
+                                 (0.0, 1.0, 1.0, 1), (1.5, 2.0, 2.0, 1)):
+            ex = ((tj - t0) ** (1 - alpha) - (tj - t1) ** (1 - alpha)) / (1 - alpha)
+            got = O.psi_kb_quad(alpha, 1.0, t0, t1, tj, sing)
+            bound = 3e-2 if (sing and alpha > 0.8) else 3e-4
+            assert abs(got - ex) <= bound * ex, (alpha, t0, t1, sing, got, ex)
 
 
 def test_gamma_against_math():
@@ -205,6 +238,20 @@ def test_ab_series_history_matches_reference(golden):
         np.testing.assert_allclose(got, z[f"h{q}"], rtol=1e-15, atol=0)
         assert recounts == 0
     assert set(np.concatenate([z[f"choice{q}"] for q in range(len(m["hist"]))])) == {1.0, 2.0}
+
+
+def test_ab_series_recount_matches_reference(golden):
+    # T = 4 and 600 terms: (t tau)^(l+1) overflows, and MF_intF restarts its
+    # sum from the sorted pyramid of the finite terms (MF_int_series.cpp:600-609)
+    m, z = golden("mf_recount")
+    for q, (a, tau, maxl, maxk, eps, nmax) in enumerate(m["hist"]):
+        got, choice, recounts = O.ab_series_history(a, tau, maxl, maxk, eps, z[f"fvs{q}"])
+        np.testing.assert_array_equal(choice, z[f"choice{q}"])
+        np.testing.assert_array_equal(got, z[f"h{q}"])
+        assert recounts > 0
+    for q, (a, T, n, mm, simple, maxl, maxk, eps, F) in enumerate(m["solve"]):
+        err, _ = O.ab_solve_series(a, T, n, mm, maxl, maxk, eps, F)
+        assert err == z["solve_err"][q]
 
 
 @pytest.mark.parametrize("q", [1, 3, 4])

@@ -120,6 +120,7 @@ SIGNATURES = {
     "frm_lod2d_state_ptr": (_I, [_V, ct.POINTER(_V)]),
     "frm_diag_copy": (_I, [_V, _V, _V, _S, _I, _P]),
     "frm_diag_read": (_I, [_V, _V, _S, _I, _P]),
+    "frm_diag_psi_kb_quad": (_I, [_V, ct.c_int32, _P, _P]),
     "frm_lod2d_fit_error": (_I, [_V, _P]),
     "frm_lod2d_sweep_shape": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_weights": (_I, [_V, ct.c_int32, _P]),
@@ -240,6 +241,14 @@ class Context:
         finally:
             self.free(src)
         return nbytes / (ms.value * 1e-3) / 1e9
+
+    def psi_kb_quad(self, cases) -> np.ndarray:
+        """The device quadrature fallback of the psi weights on rows (alpha,
+        beta, t0, t1, gtj, sing)."""
+        c = np.ascontiguousarray(cases, dtype=np.float64).reshape(-1, 6)
+        out = np.empty(len(c))
+        _check(lib().frm_diag_psi_kb_quad(self.h, len(c), _dp(c), _dp(out)), self.h)
+        return out
 
     def close(self):
         if getattr(self, "h", None):

@@ -24,6 +24,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "frm_diag.h"
 #include "frm_history.hpp"
 #include "frm_wavescan.hpp"
 #include "frm_weights.hpp"
@@ -310,12 +311,119 @@ __device__ double psi_calc_int(double alpha, double beta, int a, int A, double t
     return psi_kb_row(beta, a * tau, (a + 1) * tau, q.gtj, alpha, q.v_first);
 }
 
+// The reference's fallback where a series is not finite: _kb(1, t0, t1,
+// gtj, sing, alpha) (mt_psi_caputo.cpp:476-494), the integral of
+// (gtj - g(t))^-alpha over [t0, t1] by midpoint rules on 1 and 2 panels,
+// halving the interval while they differ by more than global_eps; below
+// global_eps in width the two rules' mean, or on the interval that ends at
+// the target (sing) the adaptive 3/8-rule march of ___kb (447-473).  The
+// recursion runs on an explicit stack of frames, left half first, each
+// interval's value the sum of its halves' as in the reference.  Returns NaN
+// if the subdivision would pass kPsiQuadDepth levels or the march
+// kPsiQuadMaxIt steps (the reference has no such bounds; widths below
+// global_eps end the subdivision after log2(width / 1e-5) levels).
+constexpr int kPsiQuadDepth = 48;
+constexpr int kPsiQuadMaxIt = 1 << 20;
+
+__device__ inline double psi_kb_mid(double beta, int n, double t0, double t1, double gtj,
+                                    double alpha) {
+#pragma clang fp contract(off)
+    double ret = 0.0;
+    const double step = (t1 - t0) / n;
+    for (int i = 0; i < n; i++) ret += step * pow(gtj - pow(t0 + (i + 0.5) * step, beta), -alpha);
+    return ret;
+}
+
+__device__ double psi_kb_sing(double beta, int n, double t0, double t1, double gtj, double alpha) {
+#pragma clang fp contract(off)
+    double f = pow(gtj - pow(t0, beta), -alpha), f1, f2, f3, f4;
+    double sum = 0.0, v, t = t0, step = (t1 - t0) / n;
+    int it = 0;
+    do {
+        f1 = pow(gtj - pow(t + 0.5 * step, beta), -alpha);
+        f3 = pow(gtj - pow(t + step / 3.0, beta), -alpha);
+        f4 = pow(gtj - pow(t + 2.0 * step / 3.0, beta), -alpha);
+        f2 = pow(gtj - pow(t + step, beta), -alpha);
+        v = (f + 3 * f3 + 3 * f4 + f2) * step / 8.0;
+        if (!isfinite(v)) break;
+        sum += v;
+        t += step;
+        if ((f1 / f) < (f2 / f1)) step /= (f2 / f1) / (f1 / f);
+        // a finite step reaches 0 within 1100 halvings; an infinite one
+        // (t1 - t0 overflowing) is cut off here
+        for (int hv = 0; (t + step) > t1; ++hv) {
+            if (hv > 1100) return NAN;
+            step /= 2.0;
+        }
+        f = f2;
+        if (++it > kPsiQuadMaxIt) return NAN;
+    } while ((t1 - t) > kPsiEps * kPsiEps * kPsiEps);
+    return sum;
+}
+
+__device__ double psi_kb_quad(double beta, double t0, double t1, double gtj, bool sing,
+                              double alpha) {
+#pragma clang fp contract(off)
+    struct Frame {
+        double t0, t1, left;
+        int phase;  // 0: not yet evaluated, 1: left half pending, 2: right half pending
+    };
+    Frame fr[kPsiQuadDepth];
+    int sp = 0;
+    fr[0] = {t0, t1, 0.0, 0};
+    double ret = 0.0;
+    for (;;) {
+        Frame& F = fr[sp];
+        if (F.phase == 0) {
+            const double v1 = psi_kb_mid(beta, 1, F.t0, F.t1, gtj, alpha);
+            const double v2 = psi_kb_mid(beta, 2, F.t0, F.t1, gtj, alpha);
+            if ((F.t1 - F.t0) < kPsiEps) {
+                ret = sing ? psi_kb_sing(beta, 16, F.t0, F.t1, gtj, alpha) : 0.5 * (v1 + v2);
+            } else if (fabs(v2 - v1) > kPsiEps) {
+                if (sp + 1 >= kPsiQuadDepth) return NAN;
+                F.phase = 1;
+                fr[sp + 1] = {F.t0, 0.5 * (F.t0 + F.t1), 0.0, 0};
+                ++sp;
+                continue;
+            } else {
+                ret = v2;
+            }
+        } else if (F.phase == 1) {  // the left half's value arrived
+            F.left = ret;
+            F.phase = 2;
+            fr[sp + 1] = {0.5 * (F.t0 + F.t1), F.t1, 0.0, 0};
+            ++sp;
+            continue;
+        } else {  // the right half's value arrived
+            ret = F.left + ret;
+        }
+        if (sp == 0) return ret;
+        --sp;
+    }
+}
+
+// Stores the value of interval x of problem p at step k as a history weight
+// (x < depth) or the current-step factor d0.
+__device__ inline void psi_store(const MtProbDev& P, int p, int x, int depth, double v,
+                                 double* __restrict__ w, int ldw, float* __restrict__ w32,
+                                 double* __restrict__ d0) {
+    if (x < depth) {
+        const double wl = v * (1 - P.alpha);
+        const int l = depth - (x + 1);
+        w[(int64_t)p * ldw + l] = wl;
+        if (w32) w32[(int64_t)p * ldw + l] = (float)wl;
+    } else {
+        d0[p] = P.Da * v * (1 - P.alpha);
+    }
+}
+
 // Step k (= tstep): w[p*ldw + l] = (1-alpha) kb_time[depth-l] for l < depth
 // (depth = k-3 history terms) and d0[p] = Da (1-alpha) kb_time[k].
 // Thread x of row p handles interval a = x+1 (x < depth) or a = k (x = depth).
 __global__ void k_psi_weights(const MtProbDev* __restrict__ pr, int nprob, int k, int depth,
                               double tau, double* __restrict__ w, int ldw, float* __restrict__ w32,
-                              double* __restrict__ d0, int* __restrict__ bad) {
+                              double* __restrict__ d0, int* __restrict__ bad,
+                              int2* __restrict__ queue, int qcap, int* __restrict__ qcount) {
     const int p = blockIdx.y;
     if (p >= nprob) return;  // the whole workgroup: one problem per grid row
     const MtProbDev P = pr[p];
@@ -326,15 +434,48 @@ __global__ void k_psi_weights(const MtProbDev* __restrict__ pr, int nprob, int k
     if (x > depth) return;
     const int a = (x < depth) ? x + 1 : k;
     const double v = psi_calc_int(P.alpha, P.beta, a, k + 1, tau, q);
-    if (!isfinite(v)) atomicOr(bad, 1);
-    if (x < depth) {
-        const double wl = v * (1 - P.alpha);
-        const int l = depth - a;
-        w[(int64_t)p * ldw + l] = wl;
-        if (w32) w32[(int64_t)p * ldw + l] = (float)wl;
-    } else {
-        d0[p] = P.Da * v * (1 - P.alpha);
+    if (!isfinite(v)) {
+        // calc_int falls back to _kb here: k_psi_quad computes and stores it
+        const int slot = atomicAdd(&qcount[k & 1], 1);
+        if (slot < qcap)
+            queue[slot] = make_int2(p, x);
+        else
+            atomicOr(bad, 4);
+        return;
     }
+    psi_store(P, p, x, depth, v, w, ldw, w32, d0);
+}
+
+// One thread per queued (problem, interval) whose series was not finite:
+// calc_int's _kb fallback, stored as k_psi_weights stores a weight.  A value
+// that is still not finite sets bad bit 1.  Workgroup 0 clears the queue
+// length that step k + 1 appends to (step k - 1's k_psi_quad, its last
+// reader, has finished).
+__global__ __launch_bounds__(64) void k_psi_quad(const MtProbDev* __restrict__ pr, int k,
+                                                 int depth, double tau, double* __restrict__ w,
+                                                 int ldw, float* __restrict__ w32,
+                                                 double* __restrict__ d0, int* __restrict__ bad,
+                                                 const int2* __restrict__ queue, int qcap,
+                                                 int* __restrict__ qcount) {
+    const int nq = min(qcount[k & 1], qcap);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) qcount[(k + 1) & 1] = 0;
+    for (int q = t; q < nq; q += gridDim.x * blockDim.x) {
+        const int2 e = queue[q];
+        const MtProbDev P = pr[e.x];
+        const int a = (e.y < depth) ? e.y + 1 : k;
+        const double gtj = pow((k + 1) * tau, P.beta);
+        const double v = psi_kb_quad(P.beta, a * tau, (a + 1) * tau, gtj, a == k, P.alpha);
+        if (!isfinite(v)) atomicOr(bad, 1);
+        psi_store(P, e.x, e.y, depth, v, w, ldw, w32, d0);
+    }
+}
+
+__global__ void k_diag_psi_kb_quad(const double* __restrict__ c, int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* r = c + 6 * (int64_t)i;
+    out[i] = psi_kb_quad(r[1], r[2], r[3], r[4], r[5] != 0.0, r[0]);
 }
 
 // ---------------------------------------------------------------- sweep
@@ -630,8 +771,14 @@ struct AbSeriesArgs {
     signed char* lflag;     // [maxL][ldh] 0, or the sign of a log-form Lc2
     double* part;           // out: the series value per history node
     int i;                  // step, >= 2
-    int* bad;               // set when MF_intF meets a non-finite term
+    int* bad;               // bit 2: a non-finite term with ab_maxl above kRecountMaxL
+    int2* queue;            // [ldh] nodes whose MF_intF sum met a non-finite term:
+                            // (node, index of the last such term)
+    int* qcount;            // [2] queue lengths; step i appends to qcount[i & 1]
 };
+
+// The longest series the sorted-pyramid recount holds in LDS.
+constexpr int kRecountMaxL = 4096;
 
 // One thread per node: MF_intF_choice(i, ..., on = i - 2, mn = 1, append_all
 // = 1, Fvalue = F) (MF_int_series.cpp:839-853) on the node's accumulators.
@@ -661,7 +808,7 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
     signed char* __restrict__ lflag = a.lflag + h;
     const int64_t ld = a.ldh;
     double ret = 0.0;
-    bool nonfinite = false;
+    int last_bad = -1;  // the last term whose Lc1 or Kc was not finite
     // one term: the accumulators of term l as loaded (c2, fl, c1)
     auto term = [&](int l, double c2, int fl, double c1) {
         const int64_t q = (int64_t)l * ld;
@@ -686,7 +833,7 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
             if (isfinite(c1) && isfinite(kc))
                 ret += c1 * kc;
             else
-                nonfinite = true;
+                last_bad = l;
         } else {
             const double lk = T[maxl + l];
             double v = 0.0;
@@ -719,8 +866,108 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
         const int64_t q = (int64_t)l * ld;
         term(l, lc2[q], lflag[q], series ? lc1[q] : 0.0);
     }
-    if (nonfinite) atomicOr(a.bad, 2);
     a.part[h] = ret / (1.0 - P.alpha);
+    if (last_bad >= 0) {
+        // MF_intF replaced the running sum by the sorted pyramid sum of all
+        // finite products there: k_ab_recount forms it and overwrites part
+        const int slot = maxl <= kRecountMaxL ? atomicAdd(&a.qcount[a.i & 1], 1) : -1;
+        if (slot >= 0 && slot < a.ldh)
+            a.queue[slot] = make_int2((int)h, last_bad);
+        else
+            atomicOr(a.bad, 2);
+    }
+}
+
+struct AbRecountArgs {
+    const MtProbDev* pr;
+    const int* gprob;
+    int64_t ldh;
+    const double* tab;
+    const double* lc1;      // [maxL][ldh], as k_ab_series left them this step
+    double* part;
+    const int2* queue;
+    int* qcount;
+    int i;
+};
+
+// The non-finite branch of MF_intF (MF_int_series.cpp:594-609 with
+// F_nosorting = 1, where the `else` binds to the finiteness test): each
+// non-finite term replaces the running sum by sum_sort_pyramid (200-218) of
+// all products Lc1[l] Kc[l] (0 where either is not finite), and the finite
+// terms after the last such term are added to it in increasing l.  One
+// workgroup per queued node: the products are sorted by decreasing
+// magnitude in LDS (bitonic), then added pairwise level by level, an odd
+// level's last element folded into the level's last sum.  Elements of equal
+// magnitude and opposite sign may be ordered differently from std::sort;
+// their sums are the same unless such a pair straddles a pair boundary.
+// Workgroup 0 clears the queue length that step i + 1 appends to: step i - 1's
+// recount, the last reader of it, has finished.
+__global__ __launch_bounds__(256) void k_ab_recount(AbRecountArgs a) {
+#pragma clang fp contract(off)
+    __shared__ double v[kRecountMaxL];
+    __shared__ double w[kRecountMaxL / 2];
+    const int tid = threadIdx.x;
+    const int nq = (int)min((int64_t)a.qcount[a.i & 1], a.ldh);
+    if (blockIdx.x == 0 && tid == 0) a.qcount[(a.i + 1) & 1] = 0;
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        const int2 e = a.queue[q];
+        const int64_t h = e.x;
+        const MtProbDev& P = a.pr[a.gprob[h / kGroup]];
+        const int maxl = P.ser_maxl;
+        const double* __restrict__ T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
+        int n2 = 1;
+        while (n2 < maxl) n2 <<= 1;
+        for (int l = tid; l < n2; l += blockDim.x) {
+            double x = 0.0;
+            if (l < maxl) {
+                const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
+                if (isfinite(c1) && isfinite(kc)) x = c1 * kc;
+            }
+            v[l] = x;
+        }
+        __syncthreads();
+        for (int k = 2; k <= n2; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int l = tid; l < n2; l += blockDim.x) {
+                    const int m = l ^ j;
+                    if (m > l) {
+                        const double x = v[l], y = v[m];
+                        const bool desc = (l & k) == 0;
+                        if (desc ? fabs(x) < fabs(y) : fabs(x) > fabs(y)) {
+                            v[l] = y;
+                            v[m] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        // the levels: v -> w -> v ...; the zeros padding v beyond maxl are
+        // never read
+        double* src = v;
+        double* dst = w;
+        int m = maxl;
+        if (m == 1 && tid == 0) v[0] += v[0];  // a single element is doubled
+        while (m > 1) {
+            const int m2 = m >> 1;
+            for (int j = tid; j < m2; j += blockDim.x) dst[j] = src[2 * j] + src[2 * j + 1];
+            __syncthreads();
+            if ((m & 1) && tid == 0) dst[m2 - 1] += src[m - 1];
+            __syncthreads();
+            double* t = src;
+            src = dst;
+            dst = t;
+            m = m2;
+        }
+        if (tid == 0) {
+            double ret = src[0];
+            for (int l = e.y + 1; l < maxl; ++l) {
+                const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
+                if (isfinite(c1) && isfinite(kc)) ret += c1 * kc;
+            }
+            a.part[h] = ret / (1.0 - P.alpha);
+        }
+        __syncthreads();  // v and w are reused by the next queued node
+    }
 }
 
 // ---------------------------------------------------------------- fitting error
@@ -870,6 +1117,13 @@ struct frm_mt1d {
     int* ser_choice = nullptr;
     double* ser_lc = nullptr;        // [2][maxL][ldh]: Lc1, Lc2
     signed char* ser_flag = nullptr; // [maxL][ldh]
+    // kind psi: (problem, interval) pairs whose weight series was not finite
+    int2* psi_queue = nullptr;
+    int* psi_qcount = nullptr;       // [2] queue lengths (by step parity)
+    int psi_qcap = 0;
+    int2* ser_queue = nullptr;       // [ldh] nodes for the pyramid recount of this step
+    int* ser_qcount = nullptr;       // [2] queue lengths (by step parity)
+    int ser_nodes = 0;               // history nodes of the series problems
     std::vector<double> ser_bytes;   // per step i >= 2: algorithmic bytes of k_ab_series
     // fitting error (solve_and_test)
     int nobs = 0;
@@ -934,6 +1188,10 @@ static void mt_free(frm_mt1d* s) {
     (void)hipFree(s->ser_choice);
     (void)hipFree(s->ser_lc);
     (void)hipFree(s->ser_flag);
+    (void)hipFree(s->ser_queue);
+    (void)hipFree(s->psi_queue);
+    (void)hipFree(s->psi_qcount);
+    (void)hipFree(s->ser_qcount);
     delete s;
 }
 
@@ -1064,6 +1322,11 @@ static int ab_setup(frm_ctx* ctx, frm_mt1d* s, const frm_mt1d_desc* d) {
         if (e == hipSuccess) e = hipMalloc(&s->ser_flag, nl);
         if (e == hipSuccess) e = hipMemset(s->ser_lc, 0, sizeof(double) * 2 * nl);
         if (e == hipSuccess) e = hipMemset(s->ser_flag, 0, nl);
+        if (e == hipSuccess) e = hipMalloc(&s->ser_queue, sizeof(int2) * s->ldh);
+        if (e == hipSuccess) e = hipMalloc(&s->ser_qcount, 2 * sizeof(int));
+        if (e == hipSuccess) e = hipMemset(s->ser_qcount, 0, 2 * sizeof(int));
+        for (const MtProbDev& P : s->hprobs)
+            if (P.ser) s->ser_nodes += P.N - 1;
         if (e == hipSuccess)
             e = hipMemcpy(s->probs, s->hprobs.data(), sizeof(MtProbDev) * s->nprob, hipMemcpyHostToDevice);
     }
@@ -1180,6 +1443,13 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         e = hipMalloc(&s->w32, sizeof(float) * (size_t)d->nprob * s->ldw);
     if (e == hipSuccess) e = hipMalloc(&s->d0, sizeof(double) * d->nprob);
     if (e == hipSuccess) e = hipMalloc(&s->bad, sizeof(int));
+    if (e == hipSuccess && d->kind == FRM_MT_PSI) {
+        // one entry per weight of a step, up to 2^20 (more set bad bit 4)
+        s->psi_qcap = (int)std::min<int64_t>((int64_t)d->nprob * (s->capacity + 1), 1 << 20);
+        e = hipMalloc(&s->psi_queue, sizeof(int2) * s->psi_qcap);
+        if (e == hipSuccess) e = hipMalloc(&s->psi_qcount, 2 * sizeof(int));
+        if (e == hipSuccess) e = hipMemset(s->psi_qcount, 0, 2 * sizeof(int));
+    }
     if (e != hipSuccess) {
         mt_free(s);
         return check_hip(ctx, e, "hipMalloc(mt1d)");
@@ -1236,6 +1506,10 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->tstep = 1;
     s->depth = 0;
+    if (s->psi_qcount) {
+        FRM_HIP(ctx, hipMemsetAsync(s->psi_qcount, 0, 2 * sizeof(int), ctx->stream));
+        FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     if (s->ab_err) {
         FRM_HIP(ctx, hipMemsetAsync(s->ab_err, 0, sizeof(double) * s->nprob, ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1250,6 +1524,7 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
         const size_t nl = (size_t)s->ser_maxL * s->ldh;
         FRM_HIP(ctx, hipMemsetAsync(s->ser_lc, 0, sizeof(double) * 2 * nl, ctx->stream));
         FRM_HIP(ctx, hipMemsetAsync(s->ser_flag, 0, nl, ctx->stream));
+        FRM_HIP(ctx, hipMemsetAsync(s->ser_qcount, 0, 2 * sizeof(int), ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return FRM_OK;
@@ -1304,7 +1579,12 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
             LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
             FRM_HIP(ctx, mt_launch(s, k_psi_weights, grid, dim3(threads), s->probs, s->nprob, k,
-                                   depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad));
+                                   depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad,
+                                   s->psi_queue, s->psi_qcap, s->psi_qcount));
+            // a fixed grid: threads beyond the queue's length exit at once
+            FRM_HIP(ctx, mt_launch(s, k_psi_quad, dim3(64), dim3(64), s->probs, k, depth, s->tau,
+                                   s->w, s->ldw, s->w32, s->d0, s->bad, s->psi_queue,
+                                   s->psi_qcap, s->psi_qcount));
         }
         int nch = 0;
         if (depth > 0 && s->ser_count < s->nprob) {
@@ -1341,9 +1621,26 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             c.part = s->part;
             c.i = k;
             c.bad = s->bad;
-            LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
-            FRM_HIP(ctx, mt_launch(s, k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)), dim3(256),
-                                   c));
+            c.queue = s->ser_queue;
+            c.qcount = s->ser_qcount;
+            {
+                LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
+                FRM_HIP(ctx, mt_launch(s, k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)),
+                                       dim3(256), c));
+            }
+            AbRecountArgs r;
+            r.pr = s->probs;
+            r.gprob = s->gprob;
+            r.ldh = s->ldh;
+            r.tab = s->ser_tab;
+            r.lc1 = s->ser_lc;
+            r.part = s->part;
+            r.queue = s->ser_queue;
+            r.qcount = s->ser_qcount;
+            r.i = k;
+            // a fixed grid: workgroups beyond the queue's length exit at once
+            FRM_HIP(ctx, mt_launch(s, k_ab_recount, dim3((unsigned)std::min(s->ser_nodes, 256)),
+                                   dim3(256), r));
         }
         if (s->kind == FRM_MT_AB) {
             AbSweepArgs b;
@@ -1496,13 +1793,17 @@ int frm_mt1d_get_state(frm_mt1d* s, double* h_U) {
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (bad & 1)
         return set_error(ctx, FRM_E_STATE,
-                         "frm_mt1d: a psi weight series did not return a finite value (the "
-                         "reference falls back to adaptive quadrature there)");
+                         "frm_mt1d: a psi weight was not finite after the adaptive quadrature "
+                         "fallback of calc_int (mt_psi_caputo.cpp:736-737)");
+    if (bad & 4)
+        return set_error(ctx, FRM_E_STATE,
+                         "frm_mt1d: more psi weight series than the fallback queue holds (2^20) "
+                         "were not finite in one step");
     if (bad & 2)
         return set_error(ctx, FRM_E_STATE,
-                         "frm_mt1d: a term of the AB power series was not finite (the reference "
-                         "recounts the sum by a sorted pyramid there, MF_int_series.cpp:600-609, "
-                         "which this path does not do)");
+                         "frm_mt1d: a term of the AB power series was not finite with ab_maxl "
+                         "above 4096, the longest series whose sorted-pyramid recount "
+                         "(MF_int_series.cpp:600-609) this path holds in LDS");
     return FRM_OK;
 }
 
@@ -1592,3 +1893,34 @@ int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep) {
 }
 
 }  // extern "C"
+
+extern "C" int frm_diag_psi_kb_quad(frm_ctx* ctx, int32_t n, const double* h_cases,
+                                    double* h_out) {
+    FRM_REQUIRE(ctx, ctx && h_cases && h_out && n > 0 && n <= (1 << 20),
+                "frm_diag_psi_kb_quad: bad arguments (1 <= n <= 2^20)");
+    for (int32_t i = 0; i < n; ++i) {
+        const double* r = h_cases + 6 * (int64_t)i;
+        FRM_REQUIRE(ctx, std::isfinite(r[2]) && std::isfinite(r[3]) && std::isfinite(r[4]) &&
+                             r[1] > 0.0 && r[0] >= 0.0 && r[0] < 1.0 && r[2] <= r[3],
+                    "frm_diag_psi_kb_quad: case %d: need 0 <= alpha < 1, beta > 0, finite "
+                    "t0 <= t1 and gtj", i);
+    }
+    double *dc = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&dc, sizeof(double) * 6 * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * (size_t)n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dc, h_cases, sizeof(double) * 6 * (size_t)n, hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(frm::k_diag_psi_kb_quad, dim3((unsigned)((n + 63) / 64)), dim3(64), 0,
+                           ctx->stream, dc, n, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h_out, dout, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dc);
+    (void)hipFree(dout);
+    return e == hipSuccess ? FRM_OK : check_hip(ctx, e, "frm_diag_psi_kb_quad");
+}
