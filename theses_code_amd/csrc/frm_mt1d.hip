@@ -772,13 +772,83 @@ struct AbSeriesArgs {
     double* part;           // out: the series value per history node
     int i;                  // step, >= 2
     int* bad;               // bit 2: a non-finite term with ab_maxl above kRecountMaxL
-    int2* queue;            // [ldh] nodes whose MF_intF sum met a non-finite term:
-                            // (node, index of the last such term)
-    int* qcount;            // [2] queue lengths; step i appends to qcount[i & 1]
 };
 
 // The longest series the sorted-pyramid recount holds in LDS.
 constexpr int kRecountMaxL = 4096;
+
+// The non-finite branch of MF_intF (MF_int_series.cpp:594-609 with
+// F_nosorting = 1, where the `else` binds to the finiteness test) for node
+// e.x, whose last non-finite term is e.y: each non-finite term replaces the
+// running sum by sum_sort_pyramid (200-218) of all products Lc1[l] Kc[l]
+// (0 where either is not finite), and the finite terms after the last such
+// term are added to it in increasing l.  The whole workgroup works on the
+// node: the products are sorted by decreasing magnitude in LDS (bitonic),
+// then added pairwise level by level, an odd level's last element folded
+// into the level's last sum.  Elements of equal magnitude and opposite sign
+// may be ordered differently from std::sort; their sums are the same unless
+// such a pair straddles a pair boundary.  Every thread of the workgroup
+// calls it with the same node.
+__device__ void ab_block_recount(const AbSeriesArgs& a, int2 e, double* v, double* w) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int64_t h = e.x;
+    const MtProbDev& P = a.pr[a.gprob[h / kGroup]];
+    const int maxl = P.ser_maxl;
+    const double* __restrict__ T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
+    int n2 = 1;
+    while (n2 < maxl) n2 <<= 1;
+    for (int l = tid; l < n2; l += blockDim.x) {
+        double x = 0.0;
+        if (l < maxl) {
+            const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
+            if (isfinite(c1) && isfinite(kc)) x = c1 * kc;
+        }
+        v[l] = x;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int l = tid; l < n2; l += blockDim.x) {
+                const int m = l ^ j;
+                if (m > l) {
+                    const double x = v[l], y = v[m];
+                    const bool desc = (l & k) == 0;
+                    if (desc ? fabs(x) < fabs(y) : fabs(x) > fabs(y)) {
+                        v[l] = y;
+                        v[m] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // the levels: v -> w -> v ...; the zeros padding v beyond maxl are never
+    // read
+    double* src = v;
+    double* dst = w;
+    int m = maxl;
+    if (m == 1 && tid == 0) v[0] += v[0];  // a single element is doubled
+    while (m > 1) {
+        const int m2 = m >> 1;
+        for (int j = tid; j < m2; j += blockDim.x) dst[j] = src[2 * j] + src[2 * j + 1];
+        __syncthreads();
+        if ((m & 1) && tid == 0) dst[m2 - 1] += src[m - 1];
+        __syncthreads();
+        double* t = src;
+        src = dst;
+        dst = t;
+        m = m2;
+    }
+    if (tid == 0) {
+        double ret = src[0];
+        for (int l = e.y + 1; l < maxl; ++l) {
+            const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
+            if (isfinite(c1) && isfinite(kc)) ret += c1 * kc;
+        }
+        a.part[h] = ret / (1.0 - P.alpha);
+    }
+    __syncthreads();  // v and w are reused by the next node
+}
 
 // One thread per node: MF_intF_choice(i, ..., on = i - 2, mn = 1, append_all
 // = 1, Fvalue = F) (MF_int_series.cpp:839-853) on the node's accumulators.
@@ -788,10 +858,12 @@ constexpr int kRecountMaxL = 4096;
 // returns sum Lc1 Kc (598-603), the integral phase returns sum of the
 // products of Lc2 and Kc with either in log form (806-822), terms in
 // increasing l, over (1 - a).  Products and sums are rounded separately, as
-// the reference's x86-64 build does.
-__global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
+// the reference's x86-64 build does.  Returns in *last_bad the last term
+// whose Lc1 or Kc was not finite (-1: none) and in *maxl_out the series
+// length.
+__device__ inline void ab_series_node(const AbSeriesArgs& a, int64_t h, int* last_bad_out,
+                                      int* maxl_out) {
 #pragma clang fp contract(off)
-    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= a.ldh) return;
     const int p = a.gprob[h / kGroup];
     if (p < 0) return;
@@ -867,107 +939,35 @@ __global__ __launch_bounds__(256) void k_ab_series(AbSeriesArgs a) {
         term(l, lc2[q], lflag[q], series ? lc1[q] : 0.0);
     }
     a.part[h] = ret / (1.0 - P.alpha);
+    *last_bad_out = last_bad;
+    *maxl_out = maxl;
+}
+
+// One thread per node (ab_series_node); then the workgroup recounts, one
+// node after another, the sums of its nodes that met a non-finite term
+// (ab_block_recount), overwriting their part.  Three workgroups per CU: the
+// recount's code would otherwise raise the kernel to 175 VGPRs (two waves
+// per SIMD instead of three); held to 168 it spills 36 bytes per lane
+// outside the term loop.
+__global__ __launch_bounds__(256, 3) void k_ab_series(AbSeriesArgs a) {
+    __shared__ double v[kRecountMaxL];
+    __shared__ double w[kRecountMaxL / 2];
+    __shared__ int2 todo[256];
+    __shared__ int ntodo;
+    if (threadIdx.x == 0) ntodo = 0;
+    __syncthreads();
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int last_bad = -1, maxl = 0;
+    ab_series_node(a, h, &last_bad, &maxl);
     if (last_bad >= 0) {
-        // MF_intF replaced the running sum by the sorted pyramid sum of all
-        // finite products there: k_ab_recount forms it and overwrites part
-        const int slot = maxl <= kRecountMaxL ? atomicAdd(&a.qcount[a.i & 1], 1) : -1;
-        if (slot >= 0 && slot < a.ldh)
-            a.queue[slot] = make_int2((int)h, last_bad);
+        if (maxl <= kRecountMaxL)
+            todo[atomicAdd(&ntodo, 1)] = make_int2((int)h, last_bad);
         else
             atomicOr(a.bad, 2);
     }
-}
-
-struct AbRecountArgs {
-    const MtProbDev* pr;
-    const int* gprob;
-    int64_t ldh;
-    const double* tab;
-    const double* lc1;      // [maxL][ldh], as k_ab_series left them this step
-    double* part;
-    const int2* queue;
-    int* qcount;
-    int i;
-};
-
-// The non-finite branch of MF_intF (MF_int_series.cpp:594-609 with
-// F_nosorting = 1, where the `else` binds to the finiteness test): each
-// non-finite term replaces the running sum by sum_sort_pyramid (200-218) of
-// all products Lc1[l] Kc[l] (0 where either is not finite), and the finite
-// terms after the last such term are added to it in increasing l.  One
-// workgroup per queued node: the products are sorted by decreasing
-// magnitude in LDS (bitonic), then added pairwise level by level, an odd
-// level's last element folded into the level's last sum.  Elements of equal
-// magnitude and opposite sign may be ordered differently from std::sort;
-// their sums are the same unless such a pair straddles a pair boundary.
-// Workgroup 0 clears the queue length that step i + 1 appends to: step i - 1's
-// recount, the last reader of it, has finished.
-__global__ __launch_bounds__(256) void k_ab_recount(AbRecountArgs a) {
-#pragma clang fp contract(off)
-    __shared__ double v[kRecountMaxL];
-    __shared__ double w[kRecountMaxL / 2];
-    const int tid = threadIdx.x;
-    const int nq = (int)min((int64_t)a.qcount[a.i & 1], a.ldh);
-    if (blockIdx.x == 0 && tid == 0) a.qcount[(a.i + 1) & 1] = 0;
-    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
-        const int2 e = a.queue[q];
-        const int64_t h = e.x;
-        const MtProbDev& P = a.pr[a.gprob[h / kGroup]];
-        const int maxl = P.ser_maxl;
-        const double* __restrict__ T = a.tab + P.ser_off + (int64_t)(a.i - 2) * kAbSeriesRows * maxl;
-        int n2 = 1;
-        while (n2 < maxl) n2 <<= 1;
-        for (int l = tid; l < n2; l += blockDim.x) {
-            double x = 0.0;
-            if (l < maxl) {
-                const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
-                if (isfinite(c1) && isfinite(kc)) x = c1 * kc;
-            }
-            v[l] = x;
-        }
-        __syncthreads();
-        for (int k = 2; k <= n2; k <<= 1)
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int l = tid; l < n2; l += blockDim.x) {
-                    const int m = l ^ j;
-                    if (m > l) {
-                        const double x = v[l], y = v[m];
-                        const bool desc = (l & k) == 0;
-                        if (desc ? fabs(x) < fabs(y) : fabs(x) > fabs(y)) {
-                            v[l] = y;
-                            v[m] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        // the levels: v -> w -> v ...; the zeros padding v beyond maxl are
-        // never read
-        double* src = v;
-        double* dst = w;
-        int m = maxl;
-        if (m == 1 && tid == 0) v[0] += v[0];  // a single element is doubled
-        while (m > 1) {
-            const int m2 = m >> 1;
-            for (int j = tid; j < m2; j += blockDim.x) dst[j] = src[2 * j] + src[2 * j + 1];
-            __syncthreads();
-            if ((m & 1) && tid == 0) dst[m2 - 1] += src[m - 1];
-            __syncthreads();
-            double* t = src;
-            src = dst;
-            dst = t;
-            m = m2;
-        }
-        if (tid == 0) {
-            double ret = src[0];
-            for (int l = e.y + 1; l < maxl; ++l) {
-                const double c1 = a.lc1[(int64_t)l * a.ldh + h], kc = T[l];
-                if (isfinite(c1) && isfinite(kc)) ret += c1 * kc;
-            }
-            a.part[h] = ret / (1.0 - P.alpha);
-        }
-        __syncthreads();  // v and w are reused by the next queued node
-    }
+    __syncthreads();
+    const int n = ntodo;
+    for (int q = 0; q < n; ++q) ab_block_recount(a, todo[q], v, w);
 }
 
 // ---------------------------------------------------------------- fitting error
@@ -1121,9 +1121,6 @@ struct frm_mt1d {
     int2* psi_queue = nullptr;
     int* psi_qcount = nullptr;       // [2] queue lengths (by step parity)
     int psi_qcap = 0;
-    int2* ser_queue = nullptr;       // [ldh] nodes for the pyramid recount of this step
-    int* ser_qcount = nullptr;       // [2] queue lengths (by step parity)
-    int ser_nodes = 0;               // history nodes of the series problems
     std::vector<double> ser_bytes;   // per step i >= 2: algorithmic bytes of k_ab_series
     // fitting error (solve_and_test)
     int nobs = 0;
@@ -1188,10 +1185,8 @@ static void mt_free(frm_mt1d* s) {
     (void)hipFree(s->ser_choice);
     (void)hipFree(s->ser_lc);
     (void)hipFree(s->ser_flag);
-    (void)hipFree(s->ser_queue);
     (void)hipFree(s->psi_queue);
     (void)hipFree(s->psi_qcount);
-    (void)hipFree(s->ser_qcount);
     delete s;
 }
 
@@ -1322,11 +1317,6 @@ static int ab_setup(frm_ctx* ctx, frm_mt1d* s, const frm_mt1d_desc* d) {
         if (e == hipSuccess) e = hipMalloc(&s->ser_flag, nl);
         if (e == hipSuccess) e = hipMemset(s->ser_lc, 0, sizeof(double) * 2 * nl);
         if (e == hipSuccess) e = hipMemset(s->ser_flag, 0, nl);
-        if (e == hipSuccess) e = hipMalloc(&s->ser_queue, sizeof(int2) * s->ldh);
-        if (e == hipSuccess) e = hipMalloc(&s->ser_qcount, 2 * sizeof(int));
-        if (e == hipSuccess) e = hipMemset(s->ser_qcount, 0, 2 * sizeof(int));
-        for (const MtProbDev& P : s->hprobs)
-            if (P.ser) s->ser_nodes += P.N - 1;
         if (e == hipSuccess)
             e = hipMemcpy(s->probs, s->hprobs.data(), sizeof(MtProbDev) * s->nprob, hipMemcpyHostToDevice);
     }
@@ -1524,7 +1514,6 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
         const size_t nl = (size_t)s->ser_maxL * s->ldh;
         FRM_HIP(ctx, hipMemsetAsync(s->ser_lc, 0, sizeof(double) * 2 * nl, ctx->stream));
         FRM_HIP(ctx, hipMemsetAsync(s->ser_flag, 0, nl, ctx->stream));
-        FRM_HIP(ctx, hipMemsetAsync(s->ser_qcount, 0, 2 * sizeof(int), ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     return FRM_OK;
@@ -1621,26 +1610,9 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             c.part = s->part;
             c.i = k;
             c.bad = s->bad;
-            c.queue = s->ser_queue;
-            c.qcount = s->ser_qcount;
-            {
-                LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
-                FRM_HIP(ctx, mt_launch(s, k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)),
-                                       dim3(256), c));
-            }
-            AbRecountArgs r;
-            r.pr = s->probs;
-            r.gprob = s->gprob;
-            r.ldh = s->ldh;
-            r.tab = s->ser_tab;
-            r.lc1 = s->ser_lc;
-            r.part = s->part;
-            r.queue = s->ser_queue;
-            r.qcount = s->ser_qcount;
-            r.i = k;
-            // a fixed grid: workgroups beyond the queue's length exit at once
-            FRM_HIP(ctx, mt_launch(s, k_ab_recount, dim3((unsigned)std::min(s->ser_nodes, 256)),
-                                   dim3(256), r));
+            LaunchTimer t(ctx, FRM_K_MEMTERM, s->ser_bytes.empty() ? 0.0 : s->ser_bytes[k - 2]);
+            FRM_HIP(ctx, mt_launch(s, k_ab_series, dim3((unsigned)((s->ldh + 255) / 256)), dim3(256),
+                                   c));
         }
         if (s->kind == FRM_MT_AB) {
             AbSweepArgs b;
