@@ -1,4 +1,5 @@
-"""Seeded synthetic inputs for the benchmark configurations.
+"""Seeded synthetic inputs for the benchmark configurations (splitmix64-seeded
+xoshiro256**, rng.py, seed 0x5EED0000 + configuration index).
 
 2D Caputo convection-diffusion on [0,1]^2 (the fr_d_t_g_2d LOD form): a grid
 of sN = M = n+1 intervals gives n x n interior unknowns per half-step (n row
@@ -11,6 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
+from .rng import Xoshiro
 from .solvers import coefs_advection
 
 SEED_BASE = 0x5EED0000
@@ -29,7 +31,7 @@ def gaussian_random_field(n: int, corr: float, h: float, rng) -> np.ndarray:
 def lod2d_case(n_interior: int, seed: int, vmax: float = 0.1, D: float = 1e-3,
                blobs: int = 8, blob_sigma: float = 0.05):
     """Returns (sN, M, coefs, C0) for an n_interior^2 LOD problem."""
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     sN = M = n_interior + 1
     h = 1.0 / sN
     ng = sN + 2  # nodes 0..sN+1 in both directions, layout [j][i]
@@ -62,7 +64,7 @@ C0_WORKLOAD = dict(name="mt1d_caputo_200_fp64", N=200, alpha=0.8, tau=1e-3, step
 
 
 def mt1d_c0_case(seed: int = C0_WORKLOAD["seed"], N: int = 200):
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     x = np.arange(N + 2) / N
     u0 = np.sin(np.pi * x) + 0.01 * rng.normal(size=N + 2)
     u0[0] = 0.0
@@ -83,10 +85,8 @@ C3_WORKLOAD = dict(name="mt1d_psi_batch4096_kexp", nprob=4096, Nmin=64, Nmax=102
 
 def mt1d_c3_case(nprob: int = 4096, seed: int = C3_WORKLOAD["seed"], Nmin: int = 64,
                  Nmax: int = 1024):
-    root = np.random.SeedSequence(seed)
     probs, U0 = [], []
-    for child in root.spawn(nprob):
-        r = np.random.default_rng(child)
+    for r in Xoshiro(seed).spawn(nprob):
         n = int(r.integers(Nmin, Nmax + 1))
         probs.append(dict(N=n, L=1.0, alpha=float(r.uniform(0.5, 0.99)),
                           beta=float(r.uniform(0.5, 1.5)), form=2, k0=float(r.uniform(1e-4, 1e-2)),
@@ -112,8 +112,7 @@ def c4_sets(nsets: int = 16, seed: int = C4_WORKLOAD["seed"], side_min: int = 10
             side_max: int = 2048):
     """Per-set (alpha, n_interior, seed) drawn from seeded sub-streams."""
     out = []
-    for child in np.random.SeedSequence(seed).spawn(nsets):
-        r = np.random.default_rng(child)
+    for r in Xoshiro(seed).spawn(nsets):
         out.append(dict(alpha=float(r.uniform(0.3, 0.9)),
                         n_interior=int(r.integers(side_min, side_max + 1)),
                         seed=int(r.integers(0, 2 ** 31))))
@@ -122,7 +121,7 @@ def c4_sets(nsets: int = 16, seed: int = C4_WORKLOAD["seed"], side_min: int = 10
 
 def ab_diffusion_case(n_interior: int, seed: int, D: float = 1e-3):
     """Pure diffusion (no velocity) with a Gaussian random initial field."""
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     sN = M = n_interior + 1
     h = 1.0 / sN
     ng = sN + 2
@@ -151,7 +150,7 @@ AB1D_SERIES_WORKLOAD = dict(name="ab1d_series_batch1024", nprob=1024, N=100, tau
 
 def ab1d_series_case(nprob: int = AB1D_SERIES_WORKLOAD["nprob"],
                      seed: int = AB1D_SERIES_WORKLOAD["seed"]):
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     alphas = np.round(np.arange(0.9, 0.05, -0.1), 1)
     cfg = AB1D_SERIES_WORKLOAD
     return [dict(N=cfg["N"], alpha=float(alphas[q % alphas.size]), src=int(rng.integers(0, 3)),
@@ -160,7 +159,7 @@ def ab1d_series_case(nprob: int = AB1D_SERIES_WORKLOAD["nprob"],
 
 
 def ab1d_case(nprob: int = AB1D_WORKLOAD["nprob"], seed: int = AB1D_WORKLOAD["seed"]):
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     alphas = np.linspace(0.3, 0.9, 16)
     probs = []
     for _ in range(nprob):
@@ -183,7 +182,7 @@ TIMESUM3D_WORKLOAD = dict(name="timesum3d_128x128x64", sN=126, M=126, K=62, tau=
 
 def timesum3d_levels(n: int, count: int, seed: int = TIMESUM3D_WORKLOAD["seed"]):
     """`count` distinct synthetic time levels of n nodes (smooth in time)."""
-    rng = np.random.default_rng(seed)
+    rng = Xoshiro(seed)
     base = rng.uniform(-1, 1, n)
     rate = rng.uniform(-1, 1, n)
     return [base + rate * np.sqrt(k + 1.0) for k in range(count)]
