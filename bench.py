@@ -89,8 +89,9 @@ def parse():
                     help="wavefronts per sweep line for the chosen forms (0: autotuned)")
     ap.add_argument("--breakdown-plain", action="store_true",
                     help="c2/c4: also time plain FP64 per kernel (a second full run)")
-    ap.add_argument("--graph", action="store_true",
-                    help="c0/c3: also time the FP64 run with its steps captured in one HIP graph")
+    ap.add_argument("--no-patterns", action="store_true",
+                    help="c0/c3: skip the launch-pattern runs (fixed history block length 64, "
+                         "steps replayed as HIP graphs)")
     ap.add_argument("--forms", default="soe,mfma",
                     help="comma-separated forms timed against plain FP64 on the same workload "
                          "(soe, mfma; binomial = the reference's own series history, c1/c2; "
@@ -157,9 +158,10 @@ def run_mt1d(args, cfg_name):
     ctx = N.Context(0)
     results = {}
 
-    def one_run(prec, timing, graph=False):
+    def one_run(prec, timing, graph=False, hist_block=0):
         b = MT1DBatch(ctx, probs, cfg["tau"], K, U0, kind=kind, precision=prec, picard=picard,
-                      graph=graph)
+                      graph=graph, hist_block=hist_block)
+        hb = b.hist_block
         if timing:
             ctx.timing_reset()
             ctx.timing_enable((1 << N.K_MEMTERM) | (1 << N.K_OTHER) | (1 << N.K_SWEEP1D))
@@ -176,6 +178,7 @@ def run_mt1d(args, cfg_name):
                                                        ("sweep", N.K_SWEEP1D))}
         U = np.concatenate(b.state())
         b.close()
+        one_run.hist_block = hb
         return dt, U, kt
 
     # c0's steps are tens of microseconds, so per-launch timing events would
@@ -189,13 +192,27 @@ def run_mt1d(args, cfg_name):
             kt = one_run(prec, True)[2]
         results[prec] = (dt, U, kt)
     dt, U64, kt = results[N.PREC_F64]
-    graph = None
-    if args.graph:
-        # the same FP64 run with all K steps recorded into one HIP graph and
-        # launched once (capture and instantiation inside the timed region)
-        dtg, Ug, _ = one_run(N.PREC_F64, False, graph=True)
-        graph = {"value": nodes * K / dtg, "ms_per_step": dtg * 1e3 / K,
-                 "identical_to_eager": bool(np.array_equal(Ug, U64))}
+    hist_block = one_run.hist_block
+    patterns = None
+    if not args.no_patterns:
+        # the launch patterns of the chosen form on the same run: eager
+        # launches with the autotuned history block length (the line's
+        # value), the same with the former fixed length 64, and every step
+        # replayed as a HIP graph (one graph per launch sequence, built on
+        # first use inside the timed region, node arguments set per step)
+        cp = N.PREC_F32C if N.PREC_F32C in results else N.PREC_F64
+        dte, Ue, _ = results[cp]
+        patterns = {"precision": "f32c" if cp == N.PREC_F32C else "f64",
+                    "eager": {"value": nodes * K / dte, "ms_per_step": dte * 1e3 / K,
+                              "hist_block": hist_block}}
+        if hist_block != 64:
+            dtf, Uf, _ = one_run(cp, False, hist_block=64)
+            patterns["eager_hist_block_64"] = {
+                "value": nodes * K / dtf, "ms_per_step": dtf * 1e3 / K,
+                "max_rel_diff": float(np.max(np.abs(Uf - Ue)) / np.max(np.abs(Ue)))}
+        dtg, Ug, _ = one_run(cp, False, graph=True)
+        patterns["graph_replay"] = {"value": nodes * K / dtg, "ms_per_step": dtg * 1e3 / K,
+                                    "identical_to_eager": bool(np.array_equal(Ug, Ue))}
     per_problem = None
     if cfg_name == "c3" and args.plain_sample > 0:
         # The plain FP64 drop-in as the reference's callers use it: one solver
@@ -240,8 +257,9 @@ def run_mt1d(args, cfg_name):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": (nb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if ms else 0.0,
                         "traffic": traffic, "kernel": "k_mt_memterm_f64"}
-    if graph:
-        line["graph_capture"] = graph
+    line["config"]["hist_block"] = hist_block
+    if patterns:
+        line["launch_patterns"] = patterns
     if cfg_name == "c3" and args.cpu_steps > 0:
         # the oracle (H_solver restatement with the psi weights rebuilt every
         # step, as the reference does) on whole runs of the first problems,

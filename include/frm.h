@@ -288,6 +288,13 @@ typedef struct frm_mt1d_desc {
                           this step's arguments and grids set on its nodes
                           (hipGraphExecKernelNodeSetParams); not while per-launch
                           timing events are on */
+    int32_t hist_block; /* history block length: the shortest time chunk, in
+                           slots, into which the memory term splits the history
+                           when the nodes alone give too few wavefronts (the
+                           sweep adds one partial sum per chunk).  0: chosen at
+                           creation by timing the memory term and sweep at depth
+                           capacity/2 for 16..512 (frm_mt1d_hist_block reports
+                           it); > 0: that length */
 } frm_mt1d_desc;
 
 int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* desc, frm_mt1d** out);
@@ -297,6 +304,8 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U);
 int frm_mt1d_step(frm_mt1d* s, int32_t nsteps);
 int frm_mt1d_get_state(frm_mt1d* s, double* h_U);
 int frm_mt1d_tstep(const frm_mt1d* s, int32_t* tstep);
+/* the history block length in use (frm_mt1d_desc.hist_block) */
+int frm_mt1d_hist_block(const frm_mt1d* s, int32_t* len);
 
 /* Kind FRM_MT_AB: the Atangana-Baleanu test problem of solve_and_check
  * (MF_int_series.cpp:875-958, simple = 1) for each problem of the batch:

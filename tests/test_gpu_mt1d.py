@@ -175,6 +175,27 @@ def test_mt1d_f32c_within_1e6_of_fp64(ctx):
     b.close()
 
 
+def test_mt1d_hist_block_autotuned_and_forced(ctx):
+    # the c0 problem: the history block length is chosen by timing at
+    # creation (one of 16..512); a forced length gives the same field up to
+    # the order of the chunk sums, and both match the oracle
+    prob, u0 = W.mt1d_c0_case()
+    steps, tau = 600, W.C0_WORKLOAD["tau"]
+    o = _oracle_for(prob, tau, u0, 0, 1)
+    o.step(steps)
+    ref = o.state()
+    got = {}
+    for hb in (0, 16, 512):
+        b = S.MT1DBatch(ctx, [prob], tau, steps, [u0], hist_block=hb)
+        used = b.hist_block
+        assert used == hb if hb else used in (16, 32, 64, 128, 256, 512)
+        b.calc_step(steps)
+        got[hb] = b.state()[0]
+        b.close()
+        assert relerr(got[hb], ref) <= TOL, (hb, used)
+    assert relerr(got[16], got[512]) <= 1e-13
+
+
 def test_mt1d_capacity_error(ctx):
     prob, u0 = W.mt1d_c0_case()
     b = S.MT1DBatch(ctx, [prob], 1e-3, 5, [u0])

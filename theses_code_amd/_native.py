@@ -59,7 +59,8 @@ class MT1DProblem(ct.Structure):
 class MT1DDesc(ct.Structure):
     _fields_ = [("nprob", ct.c_int32), ("kind", ct.c_int32), ("capacity", ct.c_int32),
                 ("precision", ct.c_int32), ("picard", ct.c_int32), ("tau", _D),
-                ("probs", ct.POINTER(MT1DProblem)), ("graph", ct.c_int32)]
+                ("probs", ct.POINTER(MT1DProblem)), ("graph", ct.c_int32),
+                ("hist_block", ct.c_int32)]
 
 
 class TimesumDesc(ct.Structure):
@@ -130,6 +131,7 @@ SIGNATURES = {
     "frm_mt1d_step": (_I, [_V, ct.c_int32]),
     "frm_mt1d_get_state": (_I, [_V, _P]),
     "frm_mt1d_tstep": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_mt1d_hist_block": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_mt1d_ab_error": (_I, [_V, _P]),
     "frm_mt1d_set_observations": (_I, [_V, _I, _P, _P, _P]),
     "frm_mt1d_fit_error": (_I, [_V, _P]),

@@ -52,21 +52,29 @@ struct MtProbDev {
     int64_t ser_off, ser_coff;
 };
 
-// sum_{c < nch} part[c ldh + q] for nch >= 1: the partials of 8 chunks are
-// loaded together (clamped chunk index, so every load is in bounds), then
-// added in increasing c, as a plain loop adds them.
-__device__ __forceinline__ double chunk_sum(const double* __restrict__ part, int64_t ldh, int nch,
-                                            int64_t q) {
-    double h = 0.0;
-    for (int c0 = 0; c0 < nch; c0 += 8) {
-        double x[8];
+// The memory term's time-chunk partials added per node, in chunk order (the
+// consumers of a chunked launch), for the CM nodes q0 + e0 + k (k < CM,
+// e0 + k < n) of a lane: the loads of 32 / CM chunks of every node are
+// issued together (clamped indices, so every load is in bounds).
+template <int CM>
+__device__ __forceinline__ void chunk_sums(const double* __restrict__ part, int64_t ldh, int nch,
+                                           int64_t q0, int e0, int n, double (&hs)[CM]) {
+    constexpr int B = CM >= 32 ? 1 : 32 / CM;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = part[(int64_t)min(c0 + u, nch - 1) * ldh + q];
+    for (int k = 0; k < CM; ++k) hs[k] = 0.0;
+    for (int c0 = 0; c0 < nch; c0 += B) {
+        double x[CM][B];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (c0 + u < nch) h += x[u];
+        for (int k = 0; k < CM; ++k)
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                x[k][u] = part[(int64_t)min(c0 + u, nch - 1) * ldh + q0 + min(e0 + k, n - 1)];
+#pragma unroll
+        for (int k = 0; k < CM; ++k)
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (c0 + u < nch && e0 + k < n) hs[k] += x[k][u];
     }
-    return h;
 }
 
 // ---------------------------------------------------------------- physics
@@ -508,18 +516,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CM == 16 ? 
     const MtProbDev P = a.pr[p];
     const int N = P.N, n = N - 1, e0 = lane * CM;
     double* Ug = a.U + P.uoff;
-    for (int q = lane; q < N + 2; q += 64) xs[q] = Ug[q];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    {
+        // N + 2 <= 64 CM + 3 nodes: CM + 1 loads per lane, all in flight
+        double ux[CM + 1];
+#pragma unroll
+        for (int r = 0; r <= CM; ++r) ux[r] = Ug[min(lane + 64 * r, N + 1)];
+#pragma unroll
+        for (int r = 0; r <= CM; ++r)
+            if (lane + 64 * r < N + 2) xs[lane + 64 * r] = ux[r];
+    }
     double u0[CM], hs[CM];
+    chunk_sums<CM>(a.part, a.ldh, a.nch, P.hoff, e0, n, hs);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
     for (int k = 0; k < CM; ++k) {
         const int e = e0 + k;
         u0[k] = 0.0;
-        hs[k] = 0.0;
-        if (e < n) {
-            u0[k] = xs[e + 1];
-            hs[k] = a.nch > 0 ? chunk_sum(a.part, a.ldh, a.nch, P.hoff + e) : 0.0;
-        }
+        if (e < n) u0[k] = xs[e + 1];
     }
     const double d0 = a.d0[p];
     for (int it = 0; it < a.picard; ++it) {
@@ -712,7 +725,8 @@ __global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
     const double Ii1a = a.d0[p];
     const double A = -(1.0 / (h * h)) - (1.0 / (2.0 * h));
     double* C = a.U + P.uoff;
-    double ca[CM], cf[CM], cal[CM], v[CM], c0[CM];
+    double ca[CM], cf[CM], cal[CM], v[CM], c0[CM], hs[CM];
+    chunk_sums<CM>(a.part, a.ldh, (a.k >= 2 && !P.ser) ? a.nch : 0, P.hoff, e0, n, hs);
 #pragma unroll
     for (int k = 0; k < CM; ++k) {
         const int e = e0 + k;
@@ -722,12 +736,10 @@ __global__ __launch_bounds__(256) void k_ab1d_sweep(AbSweepArgs a) {
             c0[k] = C[j];
             double rp = Ii1a * c0[k] + ab_f(P.src, j * h, t, al, E);
             if (a.k >= 2) {
-                if (P.ser) {  // MF_intF_choice's value, already over (1 - a)
+                if (P.ser)  // MF_intF_choice's value, already over (1 - a)
                     rp -= a.part[P.hoff + e];
-                } else {
-                    const double hs = a.nch > 0 ? chunk_sum(a.part, a.ldh, a.nch, P.hoff + e) : 0.0;
-                    rp -= hs / (1.0 - al);
-                }
+                else
+                    rp -= hs[k] / (1.0 - al);
             }
             v[k] = rp;
             ca[k] = A;
@@ -1093,6 +1105,7 @@ struct frm_mt1d {
     MtProbDev* probs = nullptr;
     int* gprob = nullptr;
     int* gvalid = nullptr;    // real nodes per group (0 for padding groups)
+    int min_chunk = kMinChunk;      // history block length: shortest time chunk of the memory term
     int graph = 0;                  // steps as HIP graphs, built once per launch sequence
     std::vector<frm::StepGraph> graphs;  // one instantiated graph per launch sequence
     std::vector<frm::KRec>* rec = nullptr;  // while set, launches are recorded, not issued
@@ -1153,6 +1166,126 @@ static hipError_t mt_launch(frm_mt1d* s, void (*kern)(KArgs...), dim3 grid, dim3
     void* p[16];
     r.ptrs(p);
     return hipLaunchKernel(r.func, grid, block, p, 0, s->ctx->stream);
+}
+
+// Time chunks of the memory term at `depth` history slots: one, unless the
+// nodes alone give fewer than kMemtermWaves wavefronts; chunks are at least
+// `min_chunk` slots (the history block length) and at most kMaxChunks.
+static int mt_chunks(const frm_mt1d* s, int depth, int min_chunk) {
+    const int64_t waves = s->ldh / (2 * kWave);
+    int nch = 1;
+    if (waves < kMemtermWaves) nch = (int)((kMemtermWaves + waves - 1) / waves);
+    return std::max(1, std::min<int>({nch, (depth + min_chunk - 1) / min_chunk, kMaxChunks}));
+}
+
+// The memory term of `depth` history slots in `nch` time chunks.
+static hipError_t mt_launch_memterm(frm_mt1d* s, int depth, int nch) {
+    const int chunk = (depth + nch - 1) / nch;
+    const dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
+    if (s->precision == FRM_PREC_F32C)
+        return mt_launch(s, k_mt_memterm_f32c, grid, dim3(kMemtermThreads), (const float*)s->hist,
+                         s->ldh, depth, s->w32, s->ldw, s->gprob, s->gvalid, chunk, s->part);
+    return mt_launch(s, k_mt_memterm_f64, grid, dim3(kMemtermThreads), (const double*)s->hist,
+                     s->ldh, depth, s->w, s->ldw, s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid,
+                     chunk, s->part);
+}
+
+// The sweep of step k over the memory term's `nch` chunks; the step's
+// increment goes to history slot `depth` when `push`.
+static hipError_t mt_launch_sweep(frm_mt1d* s, int k, int depth, int nch, bool push) {
+    const dim3 g((unsigned)((s->nprob + 3) / 4));
+    const int cm = mt_pick_cm(s->maxN);
+    const size_t esz = s->precision == FRM_PREC_F32C ? sizeof(float) : sizeof(double);
+    if (s->kind == FRM_MT_AB) {
+        AbSweepArgs b;
+        b.pr = s->probs;
+        b.nprob = s->nprob;
+        b.U = s->U;
+        b.part = s->part;
+        b.nch = nch;
+        b.ldh = s->ldh;
+        b.d0 = s->d0;
+        b.cf = s->ab_cf;
+        b.cal = s->ab_cal;
+        b.ml = s->ab_ml;
+        b.ldml = s->capacity;
+        b.k = k;
+        b.tau = s->tau;
+        b.slot = (double*)s->hist + (size_t)depth * s->ldh;
+        b.err = s->ab_err;
+        switch (cm) {
+            case 4: return mt_launch(s, k_ab1d_sweep<4>, g, dim3(256), b);
+            case 8: return mt_launch(s, k_ab1d_sweep<8>, g, dim3(256), b);
+            default: return mt_launch(s, k_ab1d_sweep<16>, g, dim3(256), b);
+        }
+    }
+    MtSweepArgs a;
+    a.pr = s->probs;
+    a.nprob = s->nprob;
+    a.U = s->U;
+    a.part = s->part;
+    a.nch = nch;
+    a.ldh = s->ldh;
+    a.d0 = s->d0;
+    a.slot = push ? (void*)((char*)s->hist + esz * (size_t)depth * s->ldh) : nullptr;
+    a.slot_f32 = s->precision == FRM_PREC_F32C;
+    a.picard = s->picard;
+    switch (cm) {
+        case 4: return mt_launch(s, k_mt_sweep<4>, g, dim3(256), a);
+        case 8: return mt_launch(s, k_mt_sweep<8>, g, dim3(256), a);
+        default: return mt_launch(s, k_mt_sweep<16>, g, dim3(256), a);
+    }
+}
+
+// Autotuner of the history block length, the shortest time chunk of the
+// memory term.  With few nodes the history is split into time chunks so that
+// enough wavefronts stream it, and the sweep then adds the chunks' partial
+// sums: shorter chunks give the memory term more wavefronts and the sweep
+// more partials to add.  Each candidate length's memory term and sweep are
+// timed at the run's mean depth (capacity / 2 history slots) on the solver's
+// own buffers, before any state is set (the state is zeroed first; the
+// sweeps of kinds L1 and psi write no history slot, the AB sweep writes slot
+// D, which a step overwrites before it is read), and the fastest is kept.
+// A batch whose nodes alone fill the GPU takes one chunk whatever the length
+// and is not timed.
+static int mt_tune_block(frm_ctx* ctx, frm_mt1d* s, int force) {
+    if (force > 0) {
+        s->min_chunk = force;
+        return FRM_OK;
+    }
+    s->min_chunk = kMinChunk;
+    const int D = std::min(s->slots - 1, s->capacity / 2);
+    if (s->ser_count == s->nprob || s->ldh / (2 * kWave) >= kMemtermWaves || D < 32) return FRM_OK;
+    static const int kCand[] = {16, 32, 64, 128, 256, 512};
+    constexpr int kReps = 20;
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipMemsetAsync(s->U, 0, sizeof(double) * s->nU, ctx->stream);
+    float best = 0.f;
+    int last_nch = -1;
+    for (int L : kCand) {
+        if (e != hipSuccess) break;
+        const int nch = mt_chunks(s, D, L);
+        if (nch == last_nch) continue;  // the same launches as the shorter length
+        last_nch = nch;
+        for (int r = -2; r < kReps && e == hipSuccess; ++r) {
+            if (r == 0) e = hipEventRecord(a, ctx->stream);
+            if (e == hipSuccess) e = mt_launch_memterm(s, D, nch);
+            if (e == hipSuccess) e = mt_launch_sweep(s, D + 3, D, nch, false);
+        }
+        if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess && (best == 0.f || ms < best)) {
+            best = ms;
+            s->min_chunk = L;
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    return e == hipSuccess ? FRM_OK : check_hip(ctx, e, "frm_mt1d_create: history block tuning");
 }
 
 static void mt_free(frm_mt1d* s) {
@@ -1346,6 +1479,7 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
                 "frm_mt1d_create: kind AB takes picard 1 and FRM_PREC_F64");
     FRM_REQUIRE(ctx, d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_F32C,
                 "frm_mt1d_create: precision %d", d->precision);
+    FRM_REQUIRE(ctx, d->hist_block >= 0, "frm_mt1d_create: hist_block %d", d->hist_block);
     FRM_HIP(ctx, hipSetDevice(ctx->device));
     frm_mt1d* s = new frm_mt1d();
     s->ctx = ctx;
@@ -1478,6 +1612,10 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
             return rc;
         }
     }
+    if (const int rc = mt_tune_block(ctx, s, d->hist_block)) {
+        mt_free(s);
+        return rc;
+    }
     *out = s;
     return FRM_OK;
 }
@@ -1526,8 +1664,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
     if (s->tstep - 1 + nsteps > s->capacity)
         return set_error(ctx, FRM_E_RANGE, "frm_mt1d_step: %d steps exceed capacity %d",
                          s->tstep - 1 + nsteps, s->capacity);
-    const bool f32 = s->precision == FRM_PREC_F32C;
-    const size_t esz = f32 ? sizeof(float) : sizeof(double);
+    const size_t esz = s->precision == FRM_PREC_F32C ? sizeof(float) : sizeof(double);
     // one step: the launches of step s->tstep, queued on the context's stream
     auto step_once = [&]() -> int {
         const int k = s->tstep;
@@ -1577,24 +1714,11 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         }
         int nch = 0;
         if (depth > 0 && s->ser_count < s->nprob) {
-            const int64_t waves = s->ldh / (2 * kWave);
-            nch = 1;
-            if (waves < kMemtermWaves) nch = (int)((kMemtermWaves + waves - 1) / waves);
-            nch = std::max(1, std::min<int>({nch, (depth + kMinChunk - 1) / kMinChunk, kMaxChunks}));
-            const int chunk = (depth + nch - 1) / nch;
-            dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
+            nch = mt_chunks(s, depth, s->min_chunk);
             // algorithmic bytes: the problems' own nodes (padding lanes load nothing)
             LaunchTimer t(ctx, FRM_K_MEMTERM,
                           (double)depth * (s->gprob_mt ? s->hnodes_mt : s->hnodes) * esz);
-            if (f32)
-                FRM_HIP(ctx, mt_launch(s, k_mt_memterm_f32c, grid, dim3(kMemtermThreads),
-                                       (const float*)s->hist, s->ldh, depth, s->w32, s->ldw,
-                                       s->gprob, s->gvalid, chunk, s->part));
-            else
-                FRM_HIP(ctx, mt_launch(s, k_mt_memterm_f64, grid, dim3(kMemtermThreads),
-                                       (const double*)s->hist, s->ldh, depth, s->w, s->ldw,
-                                       s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid, chunk,
-                                       s->part));
+            FRM_HIP(ctx, mt_launch_memterm(s, depth, nch));
         }
         if (s->kind == FRM_MT_AB && s->ser_count > 0 && k >= 2) {
             AbSeriesArgs c;
@@ -1615,59 +1739,18 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
                                    c));
         }
         if (s->kind == FRM_MT_AB) {
-            AbSweepArgs b;
-            b.pr = s->probs;
-            b.nprob = s->nprob;
-            b.U = s->U;
-            b.part = s->part;
-            b.nch = nch;
-            b.ldh = s->ldh;
-            b.d0 = s->d0;
-            b.cf = s->ab_cf;
-            b.cal = s->ab_cal;
-            b.ml = s->ab_ml;
-            b.ldml = s->capacity;
-            b.k = k;
-            b.tau = s->tau;
-            b.slot = (double*)s->hist + (size_t)depth * s->ldh;
-            b.err = s->ab_err;
             {
                 LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
-                const dim3 g((unsigned)((s->nprob + 3) / 4));
-                hipError_t e;
-                switch (mt_pick_cm(s->maxN)) {
-                    case 4: e = mt_launch(s, k_ab1d_sweep<4>, g, dim3(256), b); break;
-                    case 8: e = mt_launch(s, k_ab1d_sweep<8>, g, dim3(256), b); break;
-                    default: e = mt_launch(s, k_ab1d_sweep<16>, g, dim3(256), b);
-                }
-                FRM_HIP(ctx, e);
+                FRM_HIP(ctx, mt_launch_sweep(s, k, depth, nch, true));
             }
             s->depth = depth + 1;
             s->tstep = k + 1;
             if (int rc = fit_after()) return rc;
             return FRM_OK;
         }
-        MtSweepArgs a;
-        a.pr = s->probs;
-        a.nprob = s->nprob;
-        a.U = s->U;
-        a.part = s->part;
-        a.nch = nch;
-        a.ldh = s->ldh;
-        a.d0 = s->d0;
-        a.slot = (k >= 3) ? (void*)((char*)s->hist + esz * (size_t)depth * s->ldh) : nullptr;
-        a.slot_f32 = f32;
-        a.picard = s->picard;
         {
             LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
-            const dim3 g((unsigned)((s->nprob + 3) / 4));
-            hipError_t e;
-            switch (mt_pick_cm(s->maxN)) {
-                case 4: e = mt_launch(s, k_mt_sweep<4>, g, dim3(256), a); break;
-                case 8: e = mt_launch(s, k_mt_sweep<8>, g, dim3(256), a); break;
-                default: e = mt_launch(s, k_mt_sweep<16>, g, dim3(256), a);
-            }
-            FRM_HIP(ctx, e);
+            FRM_HIP(ctx, mt_launch_sweep(s, k, depth, nch, k >= 3));
         }
         if (k >= 3) s->depth = depth + 1;
         s->tstep = k + 1;
@@ -1855,6 +1938,12 @@ int frm_mt1d_fit_error(frm_mt1d* s, double* out) {
                                 ctx->stream));
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int p = 0; p < s->nprob; ++p) out[p] = std::isfinite(e[p]) ? e[p] : 1e300;
+    return FRM_OK;
+}
+
+int frm_mt1d_hist_block(const frm_mt1d* s, int32_t* len) {
+    if (!s || !len) return set_error(nullptr, FRM_E_INVALID, "frm_mt1d_hist_block: null");
+    *len = s->min_chunk;
     return FRM_OK;
 }
 

@@ -134,7 +134,7 @@ class MT1DBatch:
 
     def __init__(self, ctx: N.Context, problems, tau: float, capacity: int, U0,
                  kind: int = N.MT_L1, precision: int = N.PREC_F64, picard: int = 1,
-                 graph: bool = False):
+                 graph: bool = False, hist_block: int = 0):
         self.ctx = ctx
         defaults = dict(form=N.FORM_CONST, top=N.TOP_AUTO, bottom_cond=1, L=1.0, beta=1.0, D=0.0,
                         k=0.0, k0=0.0, b=0.0, top_val=0.0, Ev=0.0, Hbottom=0.0, src=0,
@@ -147,7 +147,8 @@ class MT1DBatch:
         self.sizes = [int(pr["N"]) + 2 for pr in problems]
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(int)
         self._arr = arr
-        d = N.MT1DDesc(len(problems), kind, capacity, precision, picard, tau, arr, int(graph))
+        d = N.MT1DDesc(len(problems), kind, capacity, precision, picard, tau, arr, int(graph),
+                       int(hist_block))
         h = N._V()
         N._check(N.lib().frm_mt1d_create(ctx.h, ct.byref(d), ct.byref(h)), ctx.h)
         self.h = h
@@ -169,6 +170,13 @@ class MT1DBatch:
     def tstep(self) -> int:
         t = ct.c_int32()
         N._check(N.lib().frm_mt1d_tstep(self.h, ct.byref(t)), self.ctx.h)
+        return t.value
+
+    @property
+    def hist_block(self) -> int:
+        """The history block length in use (autotuned unless given)."""
+        t = ct.c_int32()
+        N._check(N.lib().frm_mt1d_hist_block(self.h, ct.byref(t)), self.ctx.h)
         return t.value
 
     def state(self):
