@@ -181,11 +181,11 @@ def run_mt1d(args, cfg_name):
         one_run.hist_block = hb
         return dt, U, kt
 
-    # c0's steps are tens of microseconds, so per-launch timing events would
-    # show in its wall clock: it is timed without them and the kernel times
-    # come from a second, event-timed run.  c3's 3 ms steps are timed with
-    # the events on (their cost is below 1%).
-    two_pass = cfg_name == "c0"
+    # The wall clock comes from a run without per-launch timing events (c0's
+    # steps are tens of microseconds, and with the events on the psi weights
+    # are no longer formed ahead on their own stream); the kernel times come
+    # from a second, event-timed run.
+    two_pass = True
     for prec in ([N.PREC_F64, N.PREC_F32C] if cfg_name == "c3" else [N.PREC_F64]):
         dt, U, kt = one_run(prec, not two_pass)
         if two_pass:

@@ -163,6 +163,34 @@ def test_psi_quadrature_fallback_batch_matches_oracle(ctx, graph):
         assert relerr(g, o.state()) <= TOL_STRESS, p
 
 
+@pytest.mark.parametrize("prec", [N.PREC_F64, N.PREC_F32C])
+def test_psi_weights_formed_ahead_match_in_order(ctx, prec):
+    # the psi weights of step k + 1 are formed on a second stream during
+    # step k; with per-launch timing on they are formed in order on the
+    # context's stream.  Both give the same bits, across calls of uneven
+    # length, a state read and a restart.
+    probs, U0 = _uneven_batch(N.MT_PSI, 411)
+    tau, steps = 2e-3, 50
+
+    def run(timed):
+        b = S.MT1DBatch(ctx, probs, tau, steps, U0, kind=N.MT_PSI, precision=prec, picard=2)
+        if timed:
+            ctx.timing_enable(1 << N.K_OTHER)
+        out = []
+        for chunk in (1, 9, 15):
+            b.calc_step(chunk)
+            out.append(np.concatenate(b.state()))
+        b.set_state(U0)
+        b.calc_step(steps)
+        out.append(np.concatenate(b.state()))
+        ctx.timing_enable(0)
+        b.close()
+        return out
+
+    for a, b in zip(run(False), run(True)):
+        assert np.array_equal(a, b)
+
+
 def test_mt1d_f32c_within_1e6_of_fp64(ctx):
     prob, u0 = W.mt1d_c0_case()
     steps, tau = 2000, W.C0_WORKLOAD["tau"]

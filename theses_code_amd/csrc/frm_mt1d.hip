@@ -1109,6 +1109,16 @@ struct frm_mt1d {
     int graph = 0;                  // steps as HIP graphs, built once per launch sequence
     std::vector<frm::StepGraph> graphs;  // one instantiated graph per launch sequence
     std::vector<frm::KRec>* rec = nullptr;  // while set, launches are recorded, not issued
+    hipStream_t lstream = nullptr;          // while set, launches go to this stream
+    // kind psi: the weights of step k + 1 are formed on a second stream while
+    // step k's memory term and sweep run on the context's stream (the weights
+    // do not depend on the state); w, w32 and d0 then hold two buffers, step
+    // k using buffer k & 1.  wready[b]: buffer b's weights are formed;
+    // wfree[b]: the last step that read buffer b is done.
+    hipStream_t wstream = nullptr;
+    hipEvent_t wready[2] = {nullptr, nullptr}, wfree[2] = {nullptr, nullptr};
+    int wahead = 0;                         // step whose weights were launched ahead (0: none)
+    size_t wsz = 0;                         // elements of one w buffer (0: one buffer)
     int64_t hnodes = 0, hnodes_mt = 0;  // real history nodes (all; full-history problems)
     double* U = nullptr;
     void* hist = nullptr;     // slots * ldh (FP64 or FP32)
@@ -1165,7 +1175,45 @@ static hipError_t mt_launch(frm_mt1d* s, void (*kern)(KArgs...), dim3 grid, dim3
     }
     void* p[16];
     r.ptrs(p);
-    return hipLaunchKernel(r.func, grid, block, p, 0, s->ctx->stream);
+    return hipLaunchKernel(r.func, grid, block, p, 0, s->lstream ? s->lstream : s->ctx->stream);
+}
+
+// The weight buffers of step k (kind psi: two, by step parity).
+static double* mt_w(const frm_mt1d* s, int k) {
+    return s->w + (s->wsz ? (size_t)(k & 1) * s->wsz : 0);
+}
+static float* mt_w32(const frm_mt1d* s, int k) {
+    return s->w32 ? s->w32 + (s->wsz ? (size_t)(k & 1) * s->wsz : 0) : nullptr;
+}
+static double* mt_d0(const frm_mt1d* s, int k) {
+    return s->d0 + (s->wsz ? (size_t)(k & 1) * s->nprob : 0);
+}
+
+// Kind psi: the weights of step k at `depth` history slots, then the
+// quadrature fallback of the intervals whose series was not finite.
+static hipError_t mt_launch_psi_weights(frm_mt1d* s, int k, int depth) {
+    const int threads = 256;
+    const dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
+    hipError_t e = mt_launch(s, k_psi_weights, grid, dim3(threads), s->probs, s->nprob, k, depth,
+                             s->tau, mt_w(s, k), s->ldw, mt_w32(s, k), mt_d0(s, k), s->bad,
+                             s->psi_queue, s->psi_qcap, s->psi_qcount);
+    // a fixed grid: threads beyond the queue's length exit at once
+    if (e == hipSuccess)
+        e = mt_launch(s, k_psi_quad, dim3(64), dim3(64), s->probs, k, depth, s->tau, mt_w(s, k),
+                      s->ldw, mt_w32(s, k), mt_d0(s, k), s->bad, s->psi_queue, s->psi_qcap,
+                      s->psi_qcount);
+    return e;
+}
+
+// The same on the weight stream, once the last reader of step k's buffers
+// is done; wready of the buffer marks the end.
+static hipError_t mt_launch_psi_weights_ahead(frm_mt1d* s, int k, int depth) {
+    hipError_t e = hipStreamWaitEvent(s->wstream, s->wfree[k & 1], 0);
+    s->lstream = s->wstream;
+    if (e == hipSuccess) e = mt_launch_psi_weights(s, k, depth);
+    s->lstream = nullptr;
+    if (e == hipSuccess) e = hipEventRecord(s->wready[k & 1], s->wstream);
+    return e;
 }
 
 // Time chunks of the memory term at `depth` history slots: one, unless the
@@ -1178,16 +1226,16 @@ static int mt_chunks(const frm_mt1d* s, int depth, int min_chunk) {
     return std::max(1, std::min<int>({nch, (depth + min_chunk - 1) / min_chunk, kMaxChunks}));
 }
 
-// The memory term of `depth` history slots in `nch` time chunks.
-static hipError_t mt_launch_memterm(frm_mt1d* s, int depth, int nch) {
+// The memory term of step k: `depth` history slots in `nch` time chunks.
+static hipError_t mt_launch_memterm(frm_mt1d* s, int k, int depth, int nch) {
     const int chunk = (depth + nch - 1) / nch;
     const dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
     if (s->precision == FRM_PREC_F32C)
         return mt_launch(s, k_mt_memterm_f32c, grid, dim3(kMemtermThreads), (const float*)s->hist,
-                         s->ldh, depth, s->w32, s->ldw, s->gprob, s->gvalid, chunk, s->part);
+                         s->ldh, depth, mt_w32(s, k), s->ldw, s->gprob, s->gvalid, chunk, s->part);
     return mt_launch(s, k_mt_memterm_f64, grid, dim3(kMemtermThreads), (const double*)s->hist,
-                     s->ldh, depth, s->w, s->ldw, s->gprob_mt ? s->gprob_mt : s->gprob, s->gvalid,
-                     chunk, s->part);
+                     s->ldh, depth, mt_w(s, k), s->ldw, s->gprob_mt ? s->gprob_mt : s->gprob,
+                     s->gvalid, chunk, s->part);
 }
 
 // The sweep of step k over the memory term's `nch` chunks; the step's
@@ -1204,7 +1252,7 @@ static hipError_t mt_launch_sweep(frm_mt1d* s, int k, int depth, int nch, bool p
         b.part = s->part;
         b.nch = nch;
         b.ldh = s->ldh;
-        b.d0 = s->d0;
+        b.d0 = mt_d0(s, k);
         b.cf = s->ab_cf;
         b.cal = s->ab_cal;
         b.ml = s->ab_ml;
@@ -1226,7 +1274,7 @@ static hipError_t mt_launch_sweep(frm_mt1d* s, int k, int depth, int nch, bool p
     a.part = s->part;
     a.nch = nch;
     a.ldh = s->ldh;
-    a.d0 = s->d0;
+    a.d0 = mt_d0(s, k);
     a.slot = push ? (void*)((char*)s->hist + esz * (size_t)depth * s->ldh) : nullptr;
     a.slot_f32 = s->precision == FRM_PREC_F32C;
     a.picard = s->picard;
@@ -1271,8 +1319,8 @@ static int mt_tune_block(frm_ctx* ctx, frm_mt1d* s, int force) {
         last_nch = nch;
         for (int r = -2; r < kReps && e == hipSuccess; ++r) {
             if (r == 0) e = hipEventRecord(a, ctx->stream);
-            if (e == hipSuccess) e = mt_launch_memterm(s, D, nch);
-            if (e == hipSuccess) e = mt_launch_sweep(s, D + 3, D, nch, false);
+            if (e == hipSuccess) e = mt_launch_memterm(s, 0, D, nch);
+            if (e == hipSuccess) e = mt_launch_sweep(s, 0, D, nch, false);
         }
         if (e == hipSuccess) e = hipEventRecord(b, ctx->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -1300,6 +1348,12 @@ static void mt_free(frm_mt1d* s) {
     (void)hipFree(s->U);
     (void)hipFree(s->hist);
     (void)hipFree(s->part);
+    if (s->wstream) (void)hipStreamSynchronize(s->wstream);
+    for (int b = 0; b < 2; ++b) {
+        if (s->wready[b]) (void)hipEventDestroy(s->wready[b]);
+        if (s->wfree[b]) (void)hipEventDestroy(s->wfree[b]);
+    }
+    if (s->wstream) (void)hipStreamDestroy(s->wstream);
     (void)hipFree(s->w);
     (void)hipFree(s->w32);
     (void)hipFree(s->d0);
@@ -1562,10 +1616,20 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
     if (e == hipSuccess) e = hipMalloc(&s->U, sizeof(double) * s->nU);
     if (e == hipSuccess) e = hipMalloc(&s->hist, esz * (size_t)s->slots * s->ldh);
     if (e == hipSuccess) e = hipMalloc(&s->part, sizeof(double) * (size_t)kMaxChunks * s->ldh);
-    if (e == hipSuccess) e = hipMalloc(&s->w, sizeof(double) * (size_t)d->nprob * s->ldw);
+    // kind psi: two buffers of weights, the next step's formed ahead
+    const size_t nwb = d->kind == FRM_MT_PSI ? 2 : 1;
+    if (d->kind == FRM_MT_PSI) s->wsz = (size_t)d->nprob * s->ldw;
+    if (e == hipSuccess) e = hipMalloc(&s->w, sizeof(double) * nwb * (size_t)d->nprob * s->ldw);
     if (e == hipSuccess && d->precision == FRM_PREC_F32C)
-        e = hipMalloc(&s->w32, sizeof(float) * (size_t)d->nprob * s->ldw);
-    if (e == hipSuccess) e = hipMalloc(&s->d0, sizeof(double) * d->nprob);
+        e = hipMalloc(&s->w32, sizeof(float) * nwb * (size_t)d->nprob * s->ldw);
+    if (e == hipSuccess) e = hipMalloc(&s->d0, sizeof(double) * nwb * d->nprob);
+    if (e == hipSuccess && d->kind == FRM_MT_PSI) {
+        e = hipStreamCreateWithFlags(&s->wstream, hipStreamNonBlocking);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipEventCreateWithFlags(&s->wready[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&s->wfree[b], hipEventDisableTiming);
+        }
+    }
     if (e == hipSuccess) e = hipMalloc(&s->bad, sizeof(int));
     if (e == hipSuccess && d->kind == FRM_MT_PSI) {
         // one entry per weight of a step, up to 2^20 (more set bad bit 4)
@@ -1634,6 +1698,7 @@ int frm_mt1d_set_state(frm_mt1d* s, const double* h_U) {
     FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     s->tstep = 1;
     s->depth = 0;
+    s->wahead = 0;  // weights formed ahead were for the old run (done: the stream waited on them)
     if (s->psi_qcount) {
         FRM_HIP(ctx, hipMemsetAsync(s->psi_qcount, 0, 2 * sizeof(int), ctx->stream));
         FRM_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1700,17 +1765,26 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             s->tt += s->tau;
             return FRM_OK;
         };
+        // kind psi: the weights overlap the previous step unless steps are
+        // recorded for a graph or launches are timed one by one
+        const bool ahead = s->kind == FRM_MT_PSI && s->wstream && !s->rec && !ctx->timing_mask;
         if (s->kind == FRM_MT_PSI) {
-            const int threads = 256;
-            dim3 grid((unsigned)((depth + 1 + threads - 1) / threads), (unsigned)s->nprob);
-            LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
-            FRM_HIP(ctx, mt_launch(s, k_psi_weights, grid, dim3(threads), s->probs, s->nprob, k,
-                                   depth, s->tau, s->w, s->ldw, s->w32, s->d0, s->bad,
-                                   s->psi_queue, s->psi_qcap, s->psi_qcount));
-            // a fixed grid: threads beyond the queue's length exit at once
-            FRM_HIP(ctx, mt_launch(s, k_psi_quad, dim3(64), dim3(64), s->probs, k, depth, s->tau,
-                                   s->w, s->ldw, s->w32, s->d0, s->bad, s->psi_queue,
-                                   s->psi_qcap, s->psi_qcount));
+            if (s->wahead == k) {
+                FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[k & 1], 0));
+            } else if (ahead) {
+                FRM_HIP(ctx, mt_launch_psi_weights_ahead(s, k, depth));
+                FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[k & 1], 0));
+            } else {
+                LaunchTimer t(ctx, FRM_K_OTHER, 0.0);
+                FRM_HIP(ctx, mt_launch_psi_weights(s, k, depth));
+            }
+            s->wahead = 0;
+            // the next step's weights (depth + 1 slots from step 3 on), formed
+            // while this step's memory term and sweep run
+            if (ahead && k + 1 <= s->capacity) {
+                FRM_HIP(ctx, mt_launch_psi_weights_ahead(s, k + 1, k >= 3 ? depth + 1 : depth));
+                s->wahead = k + 1;
+            }
         }
         int nch = 0;
         if (depth > 0 && s->ser_count < s->nprob) {
@@ -1718,7 +1792,7 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             // algorithmic bytes: the problems' own nodes (padding lanes load nothing)
             LaunchTimer t(ctx, FRM_K_MEMTERM,
                           (double)depth * (s->gprob_mt ? s->hnodes_mt : s->hnodes) * esz);
-            FRM_HIP(ctx, mt_launch_memterm(s, depth, nch));
+            FRM_HIP(ctx, mt_launch_memterm(s, k, depth, nch));
         }
         if (s->kind == FRM_MT_AB && s->ser_count > 0 && k >= 2) {
             AbSeriesArgs c;
@@ -1752,6 +1826,8 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             LaunchTimer t(ctx, FRM_K_SWEEP1D, 0.0);
             FRM_HIP(ctx, mt_launch_sweep(s, k, depth, nch, k >= 3));
         }
+        // the buffers of step k are free for step k + 2's weights
+        if (s->wstream && !s->rec) FRM_HIP(ctx, hipEventRecord(s->wfree[k & 1], ctx->stream));
         if (k >= 3) s->depth = depth + 1;
         s->tstep = k + 1;
         if (int rc = fit_after()) return rc;
@@ -1766,6 +1842,9 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
     if (!s->graph || ctx->timing_mask != 0) {
         for (int it = 0; it < nsteps; ++it)
             if (int rc = step_once()) return rc;
+        // a synchronisation of the context's stream also covers the weights
+        // formed ahead
+        if (s->wahead) FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[s->wahead & 1], 0));
         return FRM_OK;
     }
     std::vector<KRec> recs;
