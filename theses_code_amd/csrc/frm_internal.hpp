@@ -32,6 +32,9 @@ struct frm_ctx {
     double total_ms[FRM_K_COUNT] = {};
     double total_bytes[FRM_K_COUNT] = {};
     double* static_scratch = nullptr;  // this device's copy of the static partial-sum array
+    // second stream (default priority) for work that overlaps the context's
+    // stream, ordered against it by events; created on first use
+    hipStream_t aux_stream = nullptr;
 };
 
 namespace frm {

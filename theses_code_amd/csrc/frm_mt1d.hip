@@ -1110,11 +1110,11 @@ struct frm_mt1d {
     std::vector<frm::StepGraph> graphs;  // one instantiated graph per launch sequence
     std::vector<frm::KRec>* rec = nullptr;  // while set, launches are recorded, not issued
     hipStream_t lstream = nullptr;          // while set, launches go to this stream
-    // kind psi: the weights of step k + 1 are formed on a second stream while
-    // step k's memory term and sweep run on the context's stream (the weights
-    // do not depend on the state); w, w32 and d0 then hold two buffers, step
-    // k using buffer k & 1.  wready[b]: buffer b's weights are formed;
-    // wfree[b]: the last step that read buffer b is done.
+    // kind psi: the weights of step k + 1 are formed on the context's second
+    // stream while step k's memory term and sweep run on its stream (the
+    // weights do not depend on the state); w, w32 and d0 then hold two
+    // buffers, step k using buffer k & 1.  wready[b]: buffer b's weights are
+    // formed; wfree[b]: the last step that read buffer b is done.
     hipStream_t wstream = nullptr;
     hipEvent_t wready[2] = {nullptr, nullptr}, wfree[2] = {nullptr, nullptr};
     int wahead = 0;                         // step whose weights were launched ahead (0: none)
@@ -1348,12 +1348,11 @@ static void mt_free(frm_mt1d* s) {
     (void)hipFree(s->U);
     (void)hipFree(s->hist);
     (void)hipFree(s->part);
-    if (s->wstream) (void)hipStreamSynchronize(s->wstream);
+    if (s->wstream) (void)hipStreamSynchronize(s->wstream);  // the context's, shared
     for (int b = 0; b < 2; ++b) {
         if (s->wready[b]) (void)hipEventDestroy(s->wready[b]);
         if (s->wfree[b]) (void)hipEventDestroy(s->wfree[b]);
     }
-    if (s->wstream) (void)hipStreamDestroy(s->wstream);
     (void)hipFree(s->w);
     (void)hipFree(s->w32);
     (void)hipFree(s->d0);
@@ -1624,7 +1623,8 @@ int frm_mt1d_create(frm_ctx* ctx, const frm_mt1d_desc* d, frm_mt1d** out) {
         e = hipMalloc(&s->w32, sizeof(float) * nwb * (size_t)d->nprob * s->ldw);
     if (e == hipSuccess) e = hipMalloc(&s->d0, sizeof(double) * nwb * d->nprob);
     if (e == hipSuccess && d->kind == FRM_MT_PSI) {
-        e = hipStreamCreateWithFlags(&s->wstream, hipStreamNonBlocking);
+        if (!ctx->aux_stream) e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+        s->wstream = ctx->aux_stream;
         for (int b = 0; b < 2 && e == hipSuccess; ++b) {
             e = hipEventCreateWithFlags(&s->wready[b], hipEventDisableTiming);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&s->wfree[b], hipEventDisableTiming);

@@ -125,8 +125,10 @@ int frm_ctx_destroy(frm_ctx* ctx) {
     if (!ctx) return FRM_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux_stream) (void)hipStreamSynchronize(ctx->aux_stream);
     frm::harvest(ctx);
     for (auto ev : ctx->event_pool) (void)hipEventDestroy(ev);
+    if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return FRM_OK;
