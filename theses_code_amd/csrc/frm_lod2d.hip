@@ -86,7 +86,7 @@ struct frm_lod2d {
     int atab_chunks = 0;
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
     float* F = nullptr;        // [2][128][ld]: far parts of this block and the next
-    double* G = nullptr;       // [16][ld] mid part of the current 8-step sub-block
+    float* G = nullptr;        // [16][ld] mid part of the current 8-step sub-block
     // g(t) = t^p (g_func 3): the weights and the sweeps' Thomas coefficients
     // are rebuilt every step
     int g_func = 0;
@@ -1134,7 +1134,7 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
         // two far buffers: the current block's and the next one's, formed in slices
         if (e == hipSuccess) e = hipMalloc(&s->F, sizeof(float) * 2 * 2 * kBlockSteps * s->hist->ld);
-        if (e == hipSuccess) e = hipMalloc(&s->G, sizeof(double) * 2 * kSubSteps * s->hist->ld);
+        if (e == hipSuccess) e = hipMalloc(&s->G, sizeof(float) * 2 * kSubSteps * s->hist->ld);
         if (e == hipSuccess)
             e = hipMemcpy(s->atab, tab.data(), tab.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
@@ -1293,17 +1293,17 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                         return check_hip(ctx, hipGetLastError(), "block_far");
                 }
             }
-            // the slots from P0 up to the sub-block start, every 8 steps, in FP64
+            // the slots from P0 up to the sub-block start, every 8 steps, summed in FP64
             const int N1 = n / kSubSteps * kSubSteps;
             if (N1 > P0 && n == N1) {
-                LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * ((N1 - P0) * 4.0 + 2 * kSubSteps * 8));
+                LaunchTimer t(ctx, FRM_K_MEMTERM, (double)ld * ((N1 - P0) * 4.0 + 2 * kSubSteps * 4));
                 if (block_mid(ctx->stream, h32, ld, P0, N1, s->w, s->capacity, s->G))
                     return check_hip(ctx, hipGetLastError(), "block_mid");
             }
             nch = 1;
             const bool useF = P0 > 0, useG = N1 > P0;
             LaunchTimer t(ctx, FRM_K_MEMTERM,
-                          (double)ld * ((n - N1) * 4.0 + (useF ? 2 * 4 : 0) + (useG ? 2 * 8 : 0) +
+                          (double)ld * ((n - N1) * 4.0 + (useF ? 2 * 4 : 0) + (useG ? 2 * 4 : 0) +
                                         2 * 8));
             if (block_near(ctx->stream, h32, ld, N0, N1, n, s->w, s->capacity, Fcur, useF, s->G, useG,
                            s->hist->part))

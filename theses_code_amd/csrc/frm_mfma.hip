@@ -187,14 +187,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // sub-block starting at N1 and the first slot P0 the far part leaves out
 // (P0 < N1 <= P0 + 120),
 //   G[t*8 + j][i] = sum_{P0 <= s < N1} w_t(N1 + j - s) Delta_s[i],  j < 8,
-// in FP64 (two nodes per thread, 8-byte loads of the FP32 slots; the
-// weights are wavefront-uniform).  N1 - P0 is a multiple of 8: the slots go
+// summed in FP64 and stored in FP32 like the far part F (two nodes per
+// thread, 8-byte loads of the FP32 slots; the weights are
+// wavefront-uniform).  FP32 storage rounds G at 2^-24, below the far part's
+// split-BF16 error; the FP64 store cost 64 more bytes per node and sub-block
+// and 8 more per node and step on the reading side.  N1 - P0 is a multiple of 8: the slots go
 // in batches of 8 whose loads are all issued before the first product, and
 // a batch's 8 x 8 products touch only the 15 lags N1 - s0 - 7 .. N1 - s0 + 7.
 // Terms are added in increasing s, as a plain loop over s would add them.
 __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ hist, int64_t ld,
                                                    int N0, int N1, const double* __restrict__ w,
-                                                   int ldw, double* __restrict__ G) {
+                                                   int ldw, float* __restrict__ G) {
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
     if (i >= ld) return;
     double g[2][kSubSteps][2];
@@ -237,8 +240,8 @@ __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ his
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int j = 0; j < kSubSteps; ++j)
-            *reinterpret_cast<double2*>(G + (int64_t)(t * kSubSteps + j) * ld + i) =
-                make_double2(g[t][j][0], g[t][j][1]);
+            *reinterpret_cast<float2*>(G + (int64_t)(t * kSubSteps + j) * ld + i) =
+                make_float2((float)g[t][j][0], (float)g[t][j][1]);
 }
 
 // part[t*ld + i] = F[t*64 + n - N0][i] + G[t*8 + n - N1][i]
@@ -250,7 +253,7 @@ template <int CNT>
 __device__ __forceinline__ void near_part(const float* __restrict__ hist, int64_t ld, int64_t i,
                                           int N0, int N1, int n, const double* __restrict__ w,
                                           int ldw, const float* __restrict__ F, int use_F,
-                                          const double* __restrict__ G, int use_G,
+                                          const float* __restrict__ G, int use_G,
                                           double* __restrict__ part) {
     float4 x[CNT > 0 ? CNT : 1];
 #pragma unroll
@@ -266,11 +269,10 @@ __device__ __forceinline__ void near_part(const float* __restrict__ hist, int64_
     }
     if (use_G) {
         const int j = n - N1;
-        const double2* g0 = reinterpret_cast<const double2*>(G + (int64_t)j * ld + i);
-        const double2* g1 = reinterpret_cast<const double2*>(G + (int64_t)(kSubSteps + j) * ld + i);
-        const double2 a0 = g0[0], a1 = g0[1], b0 = g1[0], b1 = g1[1];
-        s0[0] += a0.x, s0[1] += a0.y, s0[2] += a1.x, s0[3] += a1.y;
-        s1[0] += b0.x, s1[1] += b0.y, s1[2] += b1.x, s1[3] += b1.y;
+        const float4 g0 = *reinterpret_cast<const float4*>(G + (int64_t)j * ld + i);
+        const float4 g1 = *reinterpret_cast<const float4*>(G + (int64_t)(kSubSteps + j) * ld + i);
+        s0[0] += g0.x, s0[1] += g0.y, s0[2] += g0.z, s0[3] += g0.w;
+        s1[0] += g1.x, s1[1] += g1.y, s1[2] += g1.z, s1[3] += g1.w;
     }
 #pragma unroll
     for (int u = 0; u < CNT; ++u) {
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(256) void k_block_near(const float* __restrict__ hi
                                                     int N0, int N1, int n,
                                                     const double* __restrict__ w, int ldw,
                                                     const float* __restrict__ F, int use_F,
-                                                    const double* __restrict__ G, int use_G,
+                                                    const float* __restrict__ G, int use_G,
                                                     double* __restrict__ part) {
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (i >= ld) return;
@@ -351,7 +353,7 @@ std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks) {
 }
 
 int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, const double* w,
-              int ldw, double* G) {
+              int ldw, float* G) {
     if (ld % 2 || N0 < 0 || N1 <= N0 || N1 - N0 >= 2 * kBlockSteps || (N1 - N0) % kSubSteps) return -1;
     const int64_t threads = ld / 2;
     const dim3 grid((unsigned)((threads + 255) / 256)), block(256);
@@ -360,7 +362,7 @@ int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, con
 }
 
 int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, int n,
-               const double* w, int ldw, const float* F, int use_F, const double* G, int use_G,
+               const double* w, int ldw, const float* F, int use_F, const float* G, int use_G,
                double* part) {
     if (ld % 4 || n < N1 || N1 < N0 || n - N1 >= kSubSteps || n - N0 >= kBlockSteps) return -1;
     const int64_t threads = ld / 4;

@@ -24,14 +24,14 @@ int block_far_groups(int64_t ld);
 // Host: weight-fragment table for chunks d = 32 .. 32*chunks from the BF16
 // parts of the weights, wparts[(p*2 + t)*ldw + lag - 1], p < 3.
 std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks);
-// G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i], j < 8, FP64;
-// N1 - N0 a positive multiple of 8 below 128.
+// G[t*8 + j][i] = sum_{N0 <= s < N1} w_t(N1 + j - s) Delta_s[i], j < 8, summed
+// in FP64, stored in FP32; N1 - N0 a positive multiple of 8 below 128.
 int block_mid(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, const double* w,
-              int ldw, double* G);
+              int ldw, float* G);
 // part[t*ld + i] = F[t*64 + n - N0][i] (if use_F) + G[t*8 + n - N1][i] (if use_G)
 //                  + sum_{N1 <= s < n} w_t(n - s) Delta_s[i]
 int block_near(hipStream_t st, const float* hist, int64_t ld, int N0, int N1, int n,
-               const double* w, int ldw, const float* F, int use_F, const double* G, int use_G,
+               const double* w, int ldw, const float* F, int use_F, const float* G, int use_G,
                double* part);
 
 }  // namespace frm
