@@ -10,8 +10,8 @@ whole 4000 steps from t = 0 on the same inputs; their fields at step 4000
 give the max / RMS relative error against FP64.  The K = --steps timed
 steps (default 64) are a window at history depth T/2 = 2000 inside those
 runs, starting on a 64-step block boundary (1984); the MFMA form forms the
-next block's far product in 8 node slices, one every 8 steps, so a window
-of K steps pays for ceil(K/8) slices.  The W = --warmup steps just before it
+next block's far product in 8 node slices at this size, one every 8 steps,
+so a window of K steps pays for ceil(K/8) slices.  The W = --warmup steps just before it
 are the last untimed ones.  `value` is the window rate of the fastest form
 within 1e-6 of FP64; the whole-run rates stand beside it.
 
@@ -670,7 +670,8 @@ def run_lod_forms(args, cfg_name):
                                   "note": "the K timed steps lie at history depth T/2 of "
                                           "whole runs from t = 0, starting on a 64-step block "
                                           "boundary; the MFMA form's far product is formed "
-                                          "in 8 node slices, one every 8 steps"},
+                                          "in 8 node slices at these sizes (at least 1024 "
+                                          "workgroups each), one every 8 steps"},
                        "parallelism": "1 GPU"},
             "chosen_form": dict(chosen, name=best),
             "forms": out_forms,

@@ -85,6 +85,7 @@ struct frm_lod2d {
     void* atab = nullptr;      // weight-operand fragments (atab_build)
     int atab_chunks = 0;
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
+    int far_slices = 8;        // node slices of a block's far product (1, 2, 4 or 8)
     float* F = nullptr;        // [2][128][ld]: far parts of this block and the next
     float* G = nullptr;        // [16][ld] mid part of the current 8-step sub-block
     // g(t) = t^p (g_func 3): the weights and the sweeps' Thomas coefficients
@@ -1130,6 +1131,12 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         }
         // d = 32 .. N0 + 32 for the largest block start N0 below the capacity
         s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk) + 1;
+        // each slice keeps at least 1024 workgroups (above the 768 the chip
+        // holds at once), so a smaller grid takes fewer, larger slices
+        {
+            const int groups = block_far_groups(s->hist->ld);
+            s->far_slices = groups >= 8 * 1024 ? 8 : groups >= 4 * 1024 ? 4 : groups >= 2 * 1024 ? 2 : 1;
+        }
         const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
         // two far buffers: the current block's and the next one's, formed in slices
@@ -1271,8 +1278,8 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
         } else if (mf) {
             // block b = n / 64 uses the far part of the slots before P0 =
             // N0 - 64 (F of parity b), formed during block b - 1; this
-            // block's steps N0 + 8q form, slice q of 8, the next block's far
-            // part (slots before N0, all settled) into F of parity b + 1
+            // block's steps N0 + q 64/S form, slice q of S, the next block's
+            // far part (slots before N0, all settled) into F of parity b + 1
             const int64_t ld = s->hist->ld;
             const int N0 = n / kBlockSteps * kBlockSteps;
             const int P0 = std::max(0, N0 - kBlockSteps);
@@ -1280,10 +1287,11 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
             const size_t fsz = (size_t)2 * kBlockSteps * ld;
             float* Fcur = s->F + (size_t)((N0 / kBlockSteps) & 1) * fsz;
             float* Fnext = s->F + (size_t)(((N0 / kBlockSteps) + 1) & 1) * fsz;
-            const int q = (n - N0) / kSubSteps;
-            if (N0 > 0 && N0 + kBlockSteps < s->capacity && (n - N0) % kSubSteps == 0) {
+            const int every = kBlockSteps / s->far_slices;
+            const int q = (n - N0) / every;
+            if (N0 > 0 && N0 + kBlockSteps < s->capacity && (n - N0) % every == 0) {
                 const int groups = block_far_groups(ld);
-                const int per = (groups + kBlockSteps / kSubSteps - 1) / (kBlockSteps / kSubSteps);
+                const int per = (groups + s->far_slices - 1) / s->far_slices;
                 const int wg0 = q * per, nwg = std::min(per, groups - wg0);
                 if (nwg > 0) {
                     const double nodes = (double)s->hist->n_nodes * nwg / groups;

@@ -8,8 +8,9 @@
 // dense product  F[t*64 + j][i] = sum_{s < P0} W[t*64 + j][s] Delta_s[i]  with
 // M = 128, K = P0, N = nodes -- read once per 64 steps instead of every step.
 // Its slots are settled one block before it is needed, so it is formed
-// during the previous block in 8 node slices, one every 8 steps: the cost is
-// spread evenly over the steps instead of falling on the block's first.
+// during the previous block in up to 8 node slices of at least 1024
+// workgroups, one every 64 / slices steps: the cost is spread evenly over
+// the steps instead of falling on the block's first.
 // The reference forms every step's sum from scratch (fr_d_t_g_2d.cpp:
 // 3340-3437).  W and Delta are split into two or three BF16 parts whose
 // cross products accumulate in FP32 on v_mfma_f32_16x16x32_bf16.  The rest
