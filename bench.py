@@ -92,7 +92,7 @@ def parse():
     ap.add_argument("--no-patterns", action="store_true",
                     help="c0/c3: skip the launch-pattern runs (fixed history block length 64, "
                          "steps replayed as HIP graphs)")
-    ap.add_argument("--forms", default="soe,mfma",
+    ap.add_argument("--forms", default="f32c,soe,mfma",
                     help="comma-separated forms timed against plain FP64 on the same workload "
                          "(soe, mfma; binomial = the reference's own series history, c1/c2; "
                          "empty for none)")
@@ -488,7 +488,9 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, T, precision, window=None, warmup=
     return dt, out, kt, fit, win
 
 
-FORMS = {"soe": "sum-of-exponentials history (FP64 mode states unless stated)",
+FORMS = {"f32c": "FP32 increments and weights, Kahan-compensated FP32 history sums, FP64 "
+                 "sweeps",
+         "soe": "sum-of-exponentials history (FP64 mode states unless stated)",
          "mfma": "blocked Toeplitz on MFMA: FP32 increments, split hi+lo BF16, FP32 acc, "
                  "64-step blocks, FP64 near part",
          "binomial": "the reference's single-series history (TG_solver dt_alg 3, 25 terms, "
@@ -547,6 +549,7 @@ def run_lod_forms(args, cfg_name):
     fkw = {"soe": dict(precision=N.PREC_SOE, soe_terms=cfg["soe_terms"], soe_f32=soe_f32,
                        sweep_w=args.sweep_w),
            "mfma": dict(precision=N.PREC_MFMA_BF16, sweep_w=args.sweep_w),
+           "f32c": dict(precision=N.PREC_F32C, sweep_w=args.sweep_w),
            "binomial": dict(precision=N.PREC_BINOMIAL, series_terms=BINOMIAL_TERMS,
                             sweep_w=args.sweep_w)}
     if cfg_name == "c4" and "binomial" in forms:
@@ -555,7 +558,8 @@ def run_lod_forms(args, cfg_name):
         forms.remove("binomial")
     # events in the window on the dominant kernel of each form only: the
     # far product (one launch per block) and the plain memory term (ms long)
-    fmask = {"mfma": 1 << N.K_BLOCKFAR, "soe": 1 << N.K_MEMTERM, "binomial": 0}
+    fmask = {"mfma": 1 << N.K_BLOCKFAR, "soe": 1 << N.K_MEMTERM, "f32c": 1 << N.K_MEMTERM,
+             "binomial": 0}
     tot_nodes, t_plain, tw_plain, kt_plain = 0, 0.0, 0.0, {}
     plain_win = {"launches": 0, "ms": 0.0, "bytes": 0.0}
     res = {f: dict(seconds=0.0, window=0.0, errs=[], rms=[], fits=[], kernels_ms={},
@@ -596,13 +600,13 @@ def run_lod_forms(args, cfg_name):
                 q["far"]["ms"] += ms_
                 q["far"]["bytes"] += by_
                 q["far"]["flops"] += 64.0 * (by_ - 512.0 * nodes * n_ / 8)
-            if f == "soe":
+            if f in ("soe", "f32c"):
                 n_, ms_, by_ = w0["kernels"]["memterm"]
                 q["mem"]["launches"] += n_
                 q["mem"]["ms"] += ms_
                 q["mem"]["bytes"] += by_
             # per-kernel breakdown from a second, event-timed run
-            if f != "soe" or args.breakdown_soe:
+            if f not in ("soe", "f32c") or args.breakdown_soe:
                 for k, v in _lod_run(*common, timing=True, **kw, **fkw[f])[2].items():
                     q["kernels_ms"][k] = q["kernels_ms"].get(k, 0.0) + v
             d = us - up
@@ -661,6 +665,7 @@ def run_lod_forms(args, cfg_name):
             "scaling": "weak", "vs_baseline": None,
             "dtype": {"mfma": "bf16x2 split MFMA (fp32 acc) far history, fp64 near/sweeps",
                       "soe": "fp64 sweeps; " + ("fp32" if soe_f32 else "fp64") + " SOE states",
+                      "f32c": "f32 compensated history, fp64 sweeps",
                       "binomial": "f64", "plain_fp64": "f64"}[best],
             "data": "synthetic",
             "config": {"workload": cfg["name"], "form": best, "sets": len(runs),
@@ -1065,6 +1070,7 @@ def main():
         s.close()
         for f in args.forms.split(","):
             fk = {"mfma": dict(precision=N.PREC_MFMA_BF16), "soe": dict(precision=N.PREC_SOE),
+                  "f32c": dict(precision=N.PREC_F32C),
                   "binomial": dict(precision=N.PREC_BINOMIAL, series_terms=BINOMIAL_TERMS)}[f]
             sf = TGSolver(ctx, sN, M, cfg["alpha"], cfg["tau"], cfg["sigma"], coefs, C0,
                           capacity=cap, C1=0.0, S=0.0, **fk)

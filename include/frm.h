@@ -167,7 +167,9 @@ enum {
 typedef struct frm_lod2d_desc {
     int32_t sN, M;
     int32_t capacity;  /* max number of steps (history depth) */
-    int32_t precision; /* FRM_PREC_* for the memory term */
+    int32_t precision; /* FRM_PREC_* for the memory term (FRM_PREC_F32C: FP32
+                          increments and weights, both targets' sums
+                          Kahan-compensated in FP32, 4 B per node and slot) */
     double alpha, tau, sigma, C1, S;
     const double* rowA; /* host arrays of (sN+2)*(M+2) values */
     const double* rowB;

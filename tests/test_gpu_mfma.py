@@ -83,6 +83,31 @@ def test_mfma_caputo_vs_full_history(ctx, alpha):
     mf.close()
 
 
+# The compensated FP32 history (FP32 increments and weights, Kahan sums)
+# against the FP64 history on the same fields, Caputo and AB.
+@pytest.mark.parametrize("alpha,kernel", [(0.6, N.KERNEL_CAPUTO), (0.9, N.KERNEL_CAPUTO),
+                                          (0.4, N.KERNEL_AB)])
+def test_f32c_lod_vs_full_history(ctx, alpha, kernel):
+    steps = 300
+    if kernel == N.KERNEL_CAPUTO:
+        sN, M, coefs, C0 = W.lod2d_case(63, 5, D=5e-3)
+        C0 = C0.reshape(-1)
+    else:
+        coefs, C0 = _case(6, 50, 40, 5e-2)
+        sN, M = 50, 40
+    kw = dict(kernel=kernel, ab_exact=kernel == N.KERNEL_AB)
+    full = S.TGSolver(ctx, sN, M, alpha, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0, **kw)
+    f32 = S.TGSolver(ctx, sN, M, alpha, 1e-3, 1.0, coefs, C0, capacity=steps, C1=0.0,
+                     precision=N.PREC_F32C, **kw)
+    full.calc_step(steps)
+    f32.calc_step(steps)
+    mx, rms = errs(f32.b_C, full.b_C)
+    print(f"F32C alpha {alpha} kernel {kernel}: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-6 and rms <= 1e-6
+    full.close()
+    f32.close()
+
+
 def test_mfma_ab_vs_full_history(ctx):
     steps = 200
     full, mf = _pair(ctx, 50, 40, 0.6, steps, seed=6, D=5e-2, kernel=N.KERNEL_AB)

@@ -13,7 +13,8 @@ quote them, against the plain FP64 GPU path on the same seeded inputs:
 * the largest Atangana-Baleanu parameter set, 2048 x 2048, 2000 steps: the
   64-term exponential sum with FP32 mode states (1 GiB of planes, streamed)
   and the three-part MFMA form;
-* the 512 x 512 workload, alpha 0.7, over its whole 5000 steps: both forms.
+* the 512 x 512 workload, alpha 0.7, over its whole 5000 steps: the MFMA
+  and SOE forms and the compensated FP32 history.
 
 Bar: max-norm and RMS relative error <= 1e-6 against FP64 for the chosen
 forms; <= 1e-12 for FP64 against the oracle."""
@@ -143,7 +144,8 @@ def test_c1_chosen_forms_full_depth(ctx):
     print(f"c1: 512^2, alpha {alpha}, {K} steps, plain FP64", flush=True)
     ref, _ = _run(ctx, case, alpha, cfg, K)
     for name, kw in (("MFMA", dict(precision=N.PREC_MFMA_BF16)),
-                     ("SOE", dict(precision=N.PREC_SOE, soe_terms=32))):
+                     ("SOE", dict(precision=N.PREC_SOE, soe_terms=32)),
+                     ("F32C", dict(precision=N.PREC_F32C))):
         got, _ = _run(ctx, case, alpha, cfg, K, **kw)
         mx, rms = errs(got, ref)
         print(f"c1 {name}: max {mx:.3e} rms {rms:.3e}", flush=True)

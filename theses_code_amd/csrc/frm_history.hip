@@ -129,19 +129,22 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
     s = t;
 }
 
-template <int NPL>
+// NT targets (weights w[t*ldw + lag - 1]); part[(c*NT + t)*ld + i].
+template <int NPL, int NT>
 __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f32c(
     const float* __restrict__ hist, int64_t ld, int depth, int first, const float* __restrict__ w,
-    int chunk, double* __restrict__ part) {
+    int ldw, int chunk, double* __restrict__ part) {
     typedef float vt __attribute__((ext_vector_type(NPL)));
     const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * NPL;
     if (i >= ld) return;
     const int c = blockIdx.y;
     const int s0 = first + c * chunk;
     const int s1 = min(depth, s0 + chunk);
-    float sm[NPL], cm[NPL];
+    float sm[NT][NPL], cm[NT][NPL];
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) sm[q] = cm[q] = 0.f;
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) sm[t][q] = cm[t][q] = 0.f;
     const int64_t ldv = ld / NPL;
     const vt* p = reinterpret_cast<const vt*>(hist + (int64_t)s0 * ld + i);
     const float* wp = w + (depth - 1 - s0);
@@ -152,24 +155,33 @@ __global__ __launch_bounds__(kMemtermThreads) void k_memterm_f32c(
         for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(p + (int64_t)u * ldv);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const float wt = wp[-u];
 #pragma unroll
-            for (int q = 0; q < NPL; ++q) kahan_add(sm[q], cm[q], wt * x[u][q]);
+            for (int t = 0; t < NT; ++t) {
+                const float wt = wp[t * ldw - u];
+#pragma unroll
+                for (int q = 0; q < NPL; ++q) kahan_add(sm[t][q], cm[t][q], wt * x[u][q]);
+            }
         }
         p += 8 * ldv;
         wp -= 8;
     }
     for (; s < s1; ++s) {
         const vt x = __builtin_nontemporal_load(p);
-        const float wt = wp[0];
 #pragma unroll
-        for (int q = 0; q < NPL; ++q) kahan_add(sm[q], cm[q], wt * x[q]);
+        for (int t = 0; t < NT; ++t) {
+            const float wt = wp[t * ldw];
+#pragma unroll
+            for (int q = 0; q < NPL; ++q) kahan_add(sm[t][q], cm[t][q], wt * x[q]);
+        }
         p += ldv;
         wp -= 1;
     }
-    double* o = part + (int64_t)c * ld + i;
 #pragma unroll
-    for (int q = 0; q < NPL; ++q) o[q] = (double)sm[q] - (double)cm[q];
+    for (int t = 0; t < NT; ++t) {
+        double* o = part + ((int64_t)c * NT + t) * ld + i;
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) o[q] = (double)sm[t][q] - (double)cm[t][q];
+    }
 }
 
 // out[t*n + i] = scale * sum_c part[(c*NT+t)*ld + i] (+ out)
@@ -268,18 +280,28 @@ int memterm_partials(frm_ctx* ctx, frm_hist* h, int ntargets, const double* d_w,
     const int threads = kMemtermThreads;
     dim3 grid((unsigned)(h->ld / 2 / threads), (unsigned)nchunks);
     if (h->precision == FRM_PREC_F32C) {
-        FRM_REQUIRE(ctx, ntargets == 1 && d_w32, "memterm_partials: FP32 store takes one target");
+        FRM_REQUIRE(ctx, (ntargets == 1 || ntargets == 2) && d_w32,
+                    "memterm_partials: FP32 store takes one or two targets and FP32 weights");
         LaunchTimer timer(tctx, FRM_K_MEMTERM,
                           (double)span * h->n_nodes * sizeof(float) +
-                              (double)h->n_nodes * sizeof(double));
+                              (double)ntargets * h->n_nodes * sizeof(double));
+        const float* hf = (const float*)h->data;
         if (h->ld >= kF32WideLd) {
             // ld is a multiple of 4 * 256: the last workgroup may be half idle
             dim3 g4((unsigned)((h->ld / 4 + threads - 1) / threads), (unsigned)nchunks);
-            hipLaunchKernelGGL(k_memterm_f32c<4>, g4, dim3(threads), 0, ctx->stream,
-                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, part);
+            if (ntargets == 1)
+                hipLaunchKernelGGL((k_memterm_f32c<4, 1>), g4, dim3(threads), 0, ctx->stream, hf,
+                                   h->ld, depth, first, d_w32, ldw, chunk, part);
+            else
+                hipLaunchKernelGGL((k_memterm_f32c<4, 2>), g4, dim3(threads), 0, ctx->stream, hf,
+                                   h->ld, depth, first, d_w32, ldw, chunk, part);
         } else {
-            hipLaunchKernelGGL(k_memterm_f32c<2>, grid, dim3(threads), 0, ctx->stream,
-                               (const float*)h->data, h->ld, depth, first, d_w32, chunk, part);
+            if (ntargets == 1)
+                hipLaunchKernelGGL((k_memterm_f32c<2, 1>), grid, dim3(threads), 0, ctx->stream, hf,
+                                   h->ld, depth, first, d_w32, ldw, chunk, part);
+            else
+                hipLaunchKernelGGL((k_memterm_f32c<2, 2>), grid, dim3(threads), 0, ctx->stream, hf,
+                                   h->ld, depth, first, d_w32, ldw, chunk, part);
         }
         FRM_HIP(ctx, hipGetLastError());
         return FRM_OK;

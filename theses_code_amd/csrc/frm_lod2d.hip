@@ -56,6 +56,7 @@ struct frm_lod2d {
     double* C = nullptr;       // state, reference layout
     double* Cprev = nullptr;   // C_n at interior nodes, compact node order
     double* w = nullptr;       // [2][capacity] lag-indexed history weights
+    float* w32 = nullptr;      // FRM_PREC_F32C: the same in FP32
     double* rcoef = nullptr;   // rows:    a, f, al interleaved per line (load_coefs)
     double* ccoef = nullptr;   // columns: the same
     frm::SweepShape rsh, csh;  // launch shapes of the row / column sweeps
@@ -668,14 +669,14 @@ static RowArgs row_args(frm_lod2d* s, int nch) {
 
 static ColArgs col_args(frm_lod2d* s, int nch, int n) {
     const bool soe = s->precision == FRM_PREC_SOE;
-    const bool mf = s->precision == FRM_PREC_MFMA_BF16;
+    const bool f32 = s->precision == FRM_PREC_MFMA_BF16 || s->precision == FRM_PREC_F32C;
     ColArgs a;
     a.C = s->C;
     a.Cprev = s->Cprev;
     const bool ring = soe || s->precision == FRM_PREC_BINOMIAL || s->precision == FRM_PREC_IAIBK;
     a.slot = (double*)s->hist->data +
              (int64_t)(ring ? (n % kSoeLag0) : s->hist->depth) * s->hist->ld;
-    a.slotf = mf ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
+    a.slotf = f32 ? (float*)s->hist->data + (int64_t)s->hist->depth * s->hist->ld : nullptr;
     a.coef = s->ccoef;
     a.part = s->hist->part;
     a.ldh = s->hist->ld;
@@ -803,6 +804,7 @@ static void lod_free(frm_lod2d* s) {
     (void)hipFree(s->C);
     (void)hipFree(s->Cprev);
     (void)hipFree(s->w);
+    (void)hipFree(s->w32);
     (void)hipFree(s->rcoef);
     (void)hipFree(s->ccoef);
     (void)hipFree(s->soe_H);
@@ -843,9 +845,9 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     FRM_REQUIRE(ctx, d->rowA && d->rowB && d->colA && d->colB,
                 "frm_lod2d_create: coefficient arrays missing");
     FRM_REQUIRE(ctx,
-                d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_SOE ||
-                    d->precision == FRM_PREC_MFMA_BF16 || d->precision == FRM_PREC_BINOMIAL ||
-                    d->precision == FRM_PREC_IAIBK,
+                d->precision == FRM_PREC_F64 || d->precision == FRM_PREC_F32C ||
+                    d->precision == FRM_PREC_SOE || d->precision == FRM_PREC_MFMA_BF16 ||
+                    d->precision == FRM_PREC_BINOMIAL || d->precision == FRM_PREC_IAIBK,
                 "frm_lod2d_create: precision %d not built", d->precision);
     FRM_REQUIRE(ctx,
                 d->trunc_eps == 0 ||
@@ -937,12 +939,13 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     const size_t nn = (size_t)s->ld * (d->M + 2);
     s->precision = d->precision;
     // the SOE and binomial forms keep only a two-slot ring of increments,
-    // the blocked MFMA form FP32 increments
+    // the blocked MFMA and compensated FP32 forms FP32 increments
     const bool mf = d->precision == FRM_PREC_MFMA_BF16;
     const bool ring = d->precision == FRM_PREC_SOE || d->precision == FRM_PREC_BINOMIAL ||
                       d->precision == FRM_PREC_IAIBK;
     int rc = hist_alloc(ctx, nodes, ring ? kSoeLag0 : d->capacity,
-                        mf ? FRM_PREC_MFMA_BF16 : FRM_PREC_F64, &s->hist);
+                        mf || d->precision == FRM_PREC_F32C ? d->precision : FRM_PREC_F64,
+                        &s->hist);
     if (rc) {
         lod_free(s);
         return rc;
@@ -1150,6 +1153,12 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         }
     }
     e = hipMemcpy(s->w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d->precision == FRM_PREC_F32C) {
+        std::vector<float> w32(w.begin(), w.end());
+        e = hipMalloc(&s->w32, w32.size() * sizeof(float));
+        if (e == hipSuccess)
+            e = hipMemcpy(s->w32, w32.data(), w32.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMemset(s->C, 0, nn * sizeof(double));
     if (e != hipSuccess) {
         lod_free(s);
@@ -1162,7 +1171,9 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
     }
     // leave no trace of the tuning runs: zero state, Cprev and history slots
     {
-        const size_t hb = (s->precision == FRM_PREC_MFMA_BF16 ? sizeof(float) : sizeof(double)) *
+        const size_t hb = (s->precision == FRM_PREC_MFMA_BF16 || s->precision == FRM_PREC_F32C
+                               ? sizeof(float)
+                               : sizeof(double)) *
                           (size_t)std::min(s->hist->capacity, kSoeLag0) * s->hist->ld;
         e = hipMemset(s->C, 0, nn * sizeof(double));
         if (e == hipSuccess) e = hipMemset(s->Cprev, 0, nodes * sizeof(double));
@@ -1318,7 +1329,7 @@ int frm_lod2d_step(frm_lod2d* s, int32_t nsteps) {
                 return check_hip(ctx, hipGetLastError(), "block_near");
         } else {
             int rc = memterm_partials(ctx, s->hist, 2, s->w, s->capacity, &nch,
-                                      s->hist->depth - s->trunc_L);
+                                      s->hist->depth - s->trunc_L, s->w32);
             if (rc) return rc;
         }
         {
