@@ -246,6 +246,12 @@ int frm_lod2d_sweep_shape(const frm_lod2d* s, int32_t* out);
  *   Om = -d0 U^n + (KK(i)-KK(i-1))/dL + Da * history   (mt_minimal.cpp:731-743, 860-940)
  * with d0 = Da (kind L1: Da = tau^-alpha/Gamma(2-alpha)) or
  * Da (1-alpha) kb_time[tstep] (kind PSI: Da = tau^-1/Gamma(2-alpha)).
+ * Kind PSI forms kb_time every step on the GPU by calc_int's series
+ * (mt_psi_caputo.cpp:720-740), and by its _kb quadrature fallback where a
+ * series is not finite; frm_mt1d_get_state returns FRM_E_STATE if a weight
+ * is still not finite.  The weights of step k + 1 are formed on the
+ * context's second stream while step k runs (in order while per-launch
+ * timing is on or in graph mode; the results are the same).
  * The history sum keeps the reference's index convention: the first two
  * snapshot increments never enter (mt_minimal.cpp:917-921).  Coefficient
  * forms: CONST (KK = D, C = 1), CV2 (KK = k, 1/C piecewise in U as cv_form 2,
