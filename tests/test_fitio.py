@@ -16,7 +16,10 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.fixture(scope="module")
 def checker(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("fitio") / "fitio_check")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"),
+    # with the address and undefined-behaviour sanitizers: the reader parses
+    # files of unknown length, malformed ones included
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=all", "-I" + os.path.join(REPO, "include"),
                     os.path.join(REPO, "tests", "cpp", "fitio_check.cpp"), "-o", out], check=True)
     return out
 

@@ -19,7 +19,10 @@ CSRC = os.path.join(REPO, "theses_code_amd", "csrc")
 @pytest.fixture(scope="module")
 def checker(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("iaibk") / "iaibk_check")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I" + CSRC,
+    # with the address and undefined-behaviour sanitizers: the planner indexes
+    # its slot lists and value tables by hand
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-I" + CSRC,
                     os.path.join(REPO, "tests", "cpp", "iaibk_check.cpp"),
                     os.path.join(CSRC, "frm_iaibk.cpp"), "-o", out], check=True)
     return out
