@@ -706,16 +706,19 @@ static hipError_t upload_coefs(frm_lod2d* s, bool rows, SweepShape sh) {
 // Autotuner of the sweep launch shape: for every wavefronts-per-line count
 // W in {1, 2, 4} that fits the line (or only `force`), lay the coefficients
 // out for it and time the row and the column sweep on the solver's own
-// buffers (an empty history, zero state: a step's full memory traffic and
-// arithmetic), then keep the fastest shape of each.  Runs at creation,
-// before the initial state is set; the state and the history slot it
-// writes are overwritten by frm_lod2d_set_state and the first step.
+// buffers (zero state, one chunk of memory-term partials as every step
+// after the first has: a step's full memory traffic and arithmetic), then
+// keep the fastest shape of each.  Runs at creation, before the initial
+// state is set; the state and the history slot it writes are overwritten
+// by frm_lod2d_set_state and the first step.
 static int tune_sweeps(frm_lod2d* s, int force) {
     frm_ctx* ctx = s->ctx;
+    // the one chunk of partials the timed sweeps read: zeros
+    FRM_HIP(ctx, hipMemsetAsync(s->hist->part, 0, sizeof(double) * 2 * s->hist->ld, ctx->stream));
     hipEvent_t e0, e1;
     FRM_HIP(ctx, hipEventCreate(&e0));
     FRM_HIP(ctx, hipEventCreate(&e1));
-    constexpr int kWarm = 2, kReps = 5;
+    constexpr int kWarm = 3, kReps = 20;
     int rc = FRM_OK;
     for (int pass = 0; pass < 2 && rc == FRM_OK; ++pass) {
         const bool rows = pass == 0;
@@ -732,8 +735,8 @@ static int tune_sweeps(frm_lod2d* s, int force) {
             for (int it = 0; it < kWarm + kReps && e == hipSuccess; ++it) {
                 if (it == kWarm) e = hipEventRecord(e0, ctx->stream);
                 if (e != hipSuccess) break;
-                e = rows ? launch_rows(sh, ctx->stream, row_args(s, 0))
-                         : launch_cols(sh, ctx->stream, col_args(s, 0, 0));
+                e = rows ? launch_rows(sh, ctx->stream, row_args(s, 1))
+                         : launch_cols(sh, ctx->stream, col_args(s, 1, 0));
             }
             if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
