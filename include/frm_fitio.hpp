@@ -25,7 +25,18 @@
 // checking run does (mt_minimal.cpp:1631-1635, 1697-1702): N+1 rows
 // "t z U wetness " with z = i dL, %g formatting.  The soil model behind
 // `wetness` is the caller's.
+//
+// CheckError accumulates the checking run's error measure (1655-1684): a
+// model value v at a check point is clamped to v <= 0, err += (F - v)^2,
+// and the relative term is 1 where F == 0 and v != 0, min(|(F - v)/F|, 1)
+// where F != 0 (nothing where both are 0).  write_log_row writes one row of
+// the checking log (log_*.txt, 1678-1704): "T", the "Z F v" triplets of the
+// row, then "rlw .. twc .. sum_err .. avg_err .. avg_rel_err ..", %g
+// formatting; rlw and twc (root-layer wetness and total water content) come
+// from the caller's soil model, as do the model values (the depth and time
+// interpolation of 1644-1668 is the caller's, as in frm_mt1d_fit_error).
 #pragma once
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -116,6 +127,32 @@ inline void write_res_rows(FILE* f, double t, double dL, const double* U, const 
     for (int i = 0; i < N + 1; i++)
         std::fprintf(f, "%g %g %g %g \n", t, (double)i * dL, U[i], wetness[i]);
     std::fflush(f);
+}
+
+struct CheckError {
+    double err = 0.0;   // sum of (F - v)^2
+    double srel = 0.0;  // sum of the relative terms
+    int n = 0;          // check values counted
+    // adds one check value F against the model value v; returns v as counted
+    // (clamped to v <= 0)
+    double add(double F, double v) {
+        if (v > 0.0) v = 0.0;
+        err += (F - v) * (F - v);
+        if ((F == 0) && (v != 0.0)) srel += 1.0;
+        if (F != 0) srel += std::fabs((F - v) / F) < 1 ? std::fabs((F - v) / F) : 1.0;
+        n++;
+        return v;
+    }
+};
+
+// One log row: T, then n triplets (Z[i], F[i], v[i]), then the soil-model
+// figures and the accumulated error measure.
+inline void write_log_row(FILE* f, double T, const double* Z, const double* F, const double* v,
+                          int n, double rlw, double twc, const CheckError& e) {
+    std::fprintf(f, "%g ", T);
+    for (int i = 0; i < n; i++) std::fprintf(f, "%g %g %g ", Z[i], F[i], v[i]);
+    std::fprintf(f, "rlw %g twc %g sum_err %g avg_err %g avg_rel_err %g\n", rlw, twc, e.err,
+                 (e.n ? e.err / e.n : 0), (e.n ? e.srel / e.n : 0));
 }
 
 }  // namespace frm

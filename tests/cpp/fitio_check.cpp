@@ -45,5 +45,14 @@ int main(int argc, char** argv) {
     // one result time level, as the checking run writes it
     std::vector<double> U = {-1.5, -0.25, 0.0}, w = {0.3, 0.35, 0.4};
     frm::write_res_rows(stdout, 1000.0, 0.05, U.data(), w.data(), 2);
+    // one log row over the check values: model values are the check values
+    // shifted by a fixed pattern (one positive, clamped to 0)
+    frm::CheckError e;
+    std::vector<double> v(in.ncheck);
+    for (int i = 0; i < in.ncheck; ++i)
+        v[i] = e.add(in.check_F[i], in.check_F[i] * 0.9 + (i % 3 == 0 ? 0.5 : -0.25));
+    if (in.ncheck > 0)
+        frm::write_log_row(stdout, in.check_T[0], in.check_Z.data(), in.check_F.data(), v.data(),
+                           in.ncheck, 0.31, 12.5, e);
     return 0;
 }

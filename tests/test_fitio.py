@@ -1,4 +1,4 @@
-"""frm::read_fit_input and frm::write_res_rows (include/frm_fitio.hpp): the
+"""frm::read_fit_input, frm::write_res_rows and frm::write_log_row (include/frm_fitio.hpp): the
 fitting input format of mt_minimal.cpp:1955-2043 and the result rows of
 1631-1635.  The reference ships no sample input, so the files here are
 written from the format; the expected values come from splitting the text
@@ -149,9 +149,36 @@ def test_result_rows_format(checker, tmp_path):
     p = tmp_path / "fit_input.txt"
     p.write_text(text)
     _, lines = run(checker, p, 1.0)
-    rows = lines[-3:]
+    rows = lines[-4:-1]  # the last line is the log row
     U, w = [-1.5, -0.25, 0.0], [0.3, 0.35, 0.4]
     assert rows == ["%g %g %g %g " % (1000.0, i * 0.05, U[i], w[i]) for i in range(3)]
+
+
+def test_log_row_format_and_error_measure(checker, tmp_path):
+    # check values with F = 0 (model value nonzero: relative term 1), a
+    # positive model value (clamped to 0), a relative error above 1 (capped)
+    # and an ordinary one; the checker's model values are 0.9 F + 0.5 for
+    # the 1st and 4th, 0.9 F - 0.25 otherwise (mt_minimal.cpp:1655-1704)
+    F = [0.0, -2.0, -0.1, -1.0]
+    text = f"1 1 1 {len(F)} 0 0 1 1 0.25\n30 0.7 1.4 1.6 1e-6 200\n1.0 0.5 -1.0\n0.5 -1.0\n"
+    text += "".join(f"{100.0 + i} {0.1 * (i + 1)} {f}\n" for i, f in enumerate(F))
+    text += "0.0 1.0 0.5 \n0.3\n"
+    p = tmp_path / "fit_input.txt"
+    p.write_text(text)
+    _, lines = run(checker, p, 1.0)
+    v = [0.9 * f + (0.5 if i % 3 == 0 else -0.25) for i, f in enumerate(F)]
+    v = [min(x, 0.0) for x in v]  # the model value counted is clamped to <= 0
+    assert v == [0.0, -2.05, -0.34, -0.4]
+    err = sum((f - x) ** 2 for f, x in zip(F, v))
+    rel = [0.0,                   # F = 0 and v = 0: no term
+           abs((-2.0 + 2.05) / -2.0), min(abs((-0.1 + 0.34) / -0.1), 1.0),
+           abs((-1.0 + 0.4) / -1.0)]
+    assert rel[2] == 1.0
+    want = "%g " % 100.0 + "".join("%g %g %g " % (0.1 * (i + 1), f, x)
+                                   for i, (f, x) in enumerate(zip(F, v)))
+    want += "rlw %g twc %g sum_err %g avg_err %g avg_rel_err %g" % (0.31, 12.5, err, err / 4,
+                                                                     sum(rel) / 4)
+    assert lines[-1] == want
 
 
 # ---- against the reference's own reader (tests/golden/fitio_*.npz: the
