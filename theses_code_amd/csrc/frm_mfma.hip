@@ -192,10 +192,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 // thread, 8-byte loads of the FP32 slots; the weights are
 // wavefront-uniform).  FP32 storage rounds G at 2^-24, below the far part's
 // split-BF16 error; the FP64 store cost 64 more bytes per node and sub-block
-// and 8 more per node and step on the reading side.  N1 - P0 is a multiple of 8: the slots go
-// in batches of 8 whose loads are all issued before the first product, and
-// a batch's 8 x 8 products touch only the 15 lags N1 - s0 - 7 .. N1 - s0 + 7.
-// Terms are added in increasing s, as a plain loop over s would add them.
+// and 8 more per node and step on the reading side.  N1 - P0 is a multiple
+// of 8: the slots go in batches of 8 whose loads are all issued before the
+// first product, and a batch's 8 x 8 products touch only the 15 lags
+// N1 - s0 - 7 .. N1 - s0 + 7.  Terms are added in increasing s, as a plain
+// loop over s would add them.
 __global__ __launch_bounds__(256) void k_block_mid(const float* __restrict__ hist, int64_t ld,
                                                    int N0, int N1, const double* __restrict__ w,
                                                    int ldw, float* __restrict__ G) {
