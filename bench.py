@@ -330,7 +330,19 @@ def run_mt1d(args, cfg_name):
             "rms_rel_err_vs_fp64": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(U64 * U64))),
             "kernels_ms": {k: v[1] for k, v in kt32.items()}}
         line["batched_fp64"] = {"value": nodes * K / dt, "seconds": dt,
-                                "kernels_ms": line["kernels_ms"]}
+                                "kernels_ms": line["kernels_ms"], "roofline": line["roofline"]}
+        # the value's form: the compensated FP32 memory term (4 B per slot and
+        # node of the problems' own nodes; four nodes per lane at this size)
+        ms32, nb32 = kt32["memterm"][1], kt32["memterm"][2]
+        a32 = nb32 / (ms32 * 1e-3) / 1e9 if ms32 else 0.0
+        line["roofline"] = {"bound": "hbm", "achieved": a32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": a32 / HBM_PEAK_GBS, "traffic": None,
+                            "traffic_source": "not profiled (no counter pass over this command)",
+                            "kernel": "k_mt_memterm_f32c",
+                            "launches": kt32["memterm"][0],
+                            "avg_launch_ms": ms32 / kt32["memterm"][0] if kt32["memterm"][0] else None,
+                            "bytes_model": "4 B x depth x the problems' own nodes (padding lanes "
+                                           "load nothing)"}
         line["value"] = nodes * K / dt32
         line["ms_per_step"] = dt32 * 1e3 / K
         line["dtype"] = "f64 weights/Thomas/state, f32 compensated history"

@@ -146,8 +146,14 @@ __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f64(
 }
 
 // FP32 increments, FP32 weights, Kahan-compensated FP32 sums; the chunk's
-// compensated sum is returned in FP64.
+// compensated sum is returned in FP64.  NPL nodes per lane: 2 (8-byte
+// loads, a wavefront is one 128-node group) or 4 (16-byte loads, each half
+// wavefront one group: the weights of both halves' problems are read
+// through the scalar cache and each lane picks its half's).  Large batches
+// take 4 (c3: 5.8 -> 6.x TB/s); small ones keep 2, whose twice as many
+// wavefronts fill the GPU with fewer time chunks.
 typedef float mt_fv2 __attribute__((ext_vector_type(2)));
+typedef float mt_fv4 __attribute__((ext_vector_type(4)));
 
 __device__ inline void kahan_add(float& s, float& c, float x) {
     const float y = x - c;
@@ -156,47 +162,66 @@ __device__ inline void kahan_add(float& s, float& c, float x) {
     s = t;
 }
 
+template <int NPL>
 __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f32c(
     const float* __restrict__ hist, int64_t ldh, int depth, const float* __restrict__ w, int ldw,
     const int* __restrict__ gprob, const int* __restrict__ gvalid, int chunk,
     double* __restrict__ part) {
+    static_assert(NPL == 2 || NPL == 4, "two or four nodes per lane");
+    typedef float vt __attribute__((ext_vector_type(NPL)));
+    constexpr int SL = NPL == 2 ? 8 : 4;  // slots in flight per lane: 64 bytes
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t j = t * 2;
-    if (j >= ldh) return;
-    const int p = gprob[t >> 6];
-    const bool live = p >= 0 && (int)(j & (kGroup - 1)) < gvalid[t >> 6];
+    const int64_t j = t * NPL;
+    const bool in = j < ldh;
+    const int64_t grp = in ? j / kGroup : 0;
+    const int p = in ? gprob[grp] : -1;
+    const bool live = in && p >= 0 && (int)(j & (kGroup - 1)) < gvalid[grp];
+    // the problems of the wavefront's groups, wavefront-uniform values (lanes
+    // past ldh carry p = -1; a dead group's weight pointer is problem 0's,
+    // in bounds and never used)
+    const int pa = __builtin_amdgcn_readfirstlane(p);
+    const int pb = NPL == 4 ? __builtin_amdgcn_readlane(p, 32) : pa;
+    const bool upper = NPL == 4 && (threadIdx.x & 63) >= 32;
     const int c = blockIdx.y;
     const int s0 = c * chunk;
     const int s1 = min(depth, s0 + chunk);
-    float s0x = 0.f, c0x = 0.f, s1x = 0.f, c1x = 0.f;
+    float sm[NPL], cm[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) sm[k] = cm[k] = 0.f;
     if (live) {
-        const int64_t ldv = ldh / 2;
-        const mt_fv2* q = reinterpret_cast<const mt_fv2*>(hist + (int64_t)s0 * ldh + j);
-        const float* wp = w + (int64_t)p * ldw + (depth - 1 - s0);
-        // 8 slots in flight per lane: the same 64 bytes as the FP64 kernel's 4
+        const int64_t ldv = ldh / NPL;
+        const vt* q = reinterpret_cast<const vt*>(hist + (int64_t)s0 * ldh + j);
+        const float* wa = w + (int64_t)max(pa, 0) * ldw + (depth - 1 - s0);
+        const float* wb = w + (int64_t)max(pb, 0) * ldw + (depth - 1 - s0);
         int s = s0;
-        for (; s + 8 <= s1; s += 8) {
-            mt_fv2 x[8];
+        for (; s + SL <= s1; s += SL) {
+            vt x[SL];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
+            for (int u = 0; u < SL; ++u) x[u] = __builtin_nontemporal_load(q + (int64_t)u * ldv);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                kahan_add(s0x, c0x, wp[-u] * x[u].x);
-                kahan_add(s1x, c1x, wp[-u] * x[u].y);
+            for (int u = 0; u < SL; ++u) {
+                const float wu = upper ? wb[-u] : wa[-u];
+#pragma unroll
+                for (int k = 0; k < NPL; ++k) kahan_add(sm[k], cm[k], wu * x[u][k]);
             }
-            q += 8 * ldv;
-            wp -= 8;
+            q += SL * ldv;
+            wa -= SL;
+            wb -= SL;
         }
         for (; s < s1; ++s) {
-            const mt_fv2 x = __builtin_nontemporal_load(q);
-            kahan_add(s0x, c0x, wp[0] * x.x);
-            kahan_add(s1x, c1x, wp[0] * x.y);
+            const vt x = __builtin_nontemporal_load(q);
+            const float wu = upper ? wb[0] : wa[0];
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) kahan_add(sm[k], cm[k], wu * x[k]);
             q += ldv;
-            wp -= 1;
+            wa -= 1;
+            wb -= 1;
         }
     }
-    part[(int64_t)c * ldh + j] = (double)s0x - (double)c0x;
-    part[(int64_t)c * ldh + j + 1] = (double)s1x - (double)c1x;
+    if (in) {
+#pragma unroll
+        for (int k = 0; k < NPL; ++k) part[(int64_t)c * ldh + j + k] = (double)sm[k] - (double)cm[k];
+    }
 }
 
 // ---------------------------------------------------------------- psi weights
@@ -1230,9 +1255,17 @@ static int mt_chunks(const frm_mt1d* s, int depth, int min_chunk) {
 static hipError_t mt_launch_memterm(frm_mt1d* s, int k, int depth, int nch) {
     const int chunk = (depth + nch - 1) / nch;
     const dim3 grid((unsigned)(s->ldh / 2 / kMemtermThreads), (unsigned)nch);
-    if (s->precision == FRM_PREC_F32C)
-        return mt_launch(s, k_mt_memterm_f32c, grid, dim3(kMemtermThreads), (const float*)s->hist,
+    if (s->precision == FRM_PREC_F32C) {
+        // four nodes per lane where the batch fills the GPU with half the wavefronts
+        if (nch == 1 && s->ldh / (4 * kWave) >= kMemtermWaves) {
+            const dim3 g4((unsigned)((s->ldh / 4 + kMemtermThreads - 1) / kMemtermThreads), 1);
+            return mt_launch(s, k_mt_memterm_f32c<4>, g4, dim3(kMemtermThreads),
+                             (const float*)s->hist, s->ldh, depth, mt_w32(s, k), s->ldw, s->gprob,
+                             s->gvalid, chunk, s->part);
+        }
+        return mt_launch(s, k_mt_memterm_f32c<2>, grid, dim3(kMemtermThreads), (const float*)s->hist,
                          s->ldh, depth, mt_w32(s, k), s->ldw, s->gprob, s->gvalid, chunk, s->part);
+    }
     return mt_launch(s, k_mt_memterm_f64, grid, dim3(kMemtermThreads), (const double*)s->hist,
                      s->ldh, depth, mt_w(s, k), s->ldw, s->gprob_mt ? s->gprob_mt : s->gprob,
                      s->gvalid, chunk, s->part);
