@@ -284,3 +284,30 @@ def test_mt1d_graph_long_call_matches_eager(ctx):
     assert np.array_equal(e.state()[0], g.state()[0])
     e.close()
     g.close()
+
+
+def test_mt1d_f32c_large_batch_four_nodes_per_lane(ctx):
+    # a batch whose groups give at least 2048 wavefronts at four nodes per lane
+    # (here 640 problems of 700..1000 nodes, 4569 128-node groups): the
+    # compensated FP32 memory term reads 16 bytes per lane, half a wavefront
+    # per group, each half's problem weights picked per lane.  Neighbouring
+    # groups of different problems share wavefronts.  Against the FP64 batch
+    # (1e-6), and graph replay against eager launches (bit for bit).
+    probs, U0 = W.mt1d_c3_case(nprob=640, Nmin=700, Nmax=1000)
+    steps = 40
+    tau = W.C3_WORKLOAD["tau"]
+
+    def run(prec, graph=False):
+        b = S.MT1DBatch(ctx, probs, tau, steps, U0, kind=N.MT_PSI, precision=prec, picard=2,
+                        graph=graph)
+        b.calc_step(steps)
+        U = np.concatenate(b.state())
+        b.close()
+        return U
+
+    u64 = run(N.PREC_F64)
+    u32 = run(N.PREC_F32C)
+    u32g = run(N.PREC_F32C, graph=True)
+    assert np.all(np.isfinite(u32))
+    assert relerr(u32, u64) <= 1e-6
+    assert np.array_equal(u32g, u32)
