@@ -290,12 +290,13 @@ typedef struct frm_mt1d_desc {
     int32_t picard;    /* Picard iterations per step, >= 1 */
     double tau;
     const frm_mt1d_problem* probs; /* host array of nprob */
-    int32_t graph;     /* 1: every step runs as a HIP graph: one graph per
-                          distinct sequence of kernels (a handful per run), built
-                          and instantiated the first time, then replayed with
-                          this step's arguments and grids set on its nodes
-                          (hipGraphExecKernelNodeSetParams); not while per-launch
-                          timing events are on */
+    int32_t graph;     /* 1: steps run as HIP graphs of up to 32 steps each:
+                          one graph per distinct sequence of kernels (a handful
+                          per run), built and instantiated the first time, then
+                          replayed with these steps' arguments and grids set on
+                          its nodes (hipGraphExecKernelNodeSetParams); kind psi
+                          forms the next step's weights on a branch beside the
+                          step.  Not while per-launch timing events are on */
     int32_t hist_block; /* history block length: the shortest time chunk, in
                            slots, into which the memory term splits the history
                            when the nodes alone give too few wavefronts (the

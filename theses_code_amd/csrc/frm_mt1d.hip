@@ -1090,6 +1090,8 @@ namespace frm {
 struct KRec {
     const void* func = nullptr;
     dim3 grid, block;
+    int step = 0;         // step of the recorded run it belongs to
+    bool branch = false;  // weights formed ahead (kind psi): a branch of the graph
     alignas(16) unsigned char buf[448];
     size_t used = 0;
     size_t off[16];
@@ -1106,11 +1108,14 @@ struct KRec {
         for (int i = 0; i < n; ++i) p[i] = buf + off[i];
     }
 };
-// One step's launches as an instantiated graph (a chain of kernel nodes),
-// reused for every step that issues the same kernels in the same order: only
-// the nodes' arguments and grids change between steps.
+// The launches of up to kGraphSteps consecutive steps as an instantiated
+// graph (a chain of kernel nodes), reused for every run of steps that issues
+// the same kernels in the same order: only the nodes' arguments and grids
+// change between launches.
+constexpr int kGraphSteps = 32;
 struct StepGraph {
     std::vector<const void*> funcs;
+    std::vector<int> tags;  // step * 2 + branch of every node
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipGraphNode_t> nodes;
@@ -1134,6 +1139,7 @@ struct frm_mt1d {
     int graph = 0;                  // steps as HIP graphs, built once per launch sequence
     std::vector<frm::StepGraph> graphs;  // one instantiated graph per launch sequence
     std::vector<frm::KRec>* rec = nullptr;  // while set, launches are recorded, not issued
+    int rec_step = 0;                        // step of the recorded run being recorded
     hipStream_t lstream = nullptr;          // while set, launches go to this stream
     // kind psi: the weights of step k + 1 are formed on the context's second
     // stream while step k's memory term and sweep run on its stream (the
@@ -1195,6 +1201,8 @@ static hipError_t mt_launch(frm_mt1d* s, void (*kern)(KArgs...), dim3 grid, dim3
     r.block = block;
     (r.arg(static_cast<std::decay_t<KArgs>>(args)), ...);
     if (s->rec) {
+        r.step = s->rec_step;
+        r.branch = s->lstream != nullptr;
         s->rec->push_back(r);
         return hipSuccess;
     }
@@ -1231,13 +1239,14 @@ static hipError_t mt_launch_psi_weights(frm_mt1d* s, int k, int depth) {
 }
 
 // The same on the weight stream, once the last reader of step k's buffers
-// is done; wready of the buffer marks the end.
+// is done; wready of the buffer marks the end.  While a graph is recorded
+// the launches become a branch of it, ordered by the graph's edges instead.
 static hipError_t mt_launch_psi_weights_ahead(frm_mt1d* s, int k, int depth) {
-    hipError_t e = hipStreamWaitEvent(s->wstream, s->wfree[k & 1], 0);
+    hipError_t e = s->rec ? hipSuccess : hipStreamWaitEvent(s->wstream, s->wfree[k & 1], 0);
     s->lstream = s->wstream;
     if (e == hipSuccess) e = mt_launch_psi_weights(s, k, depth);
     s->lstream = nullptr;
-    if (e == hipSuccess) e = hipEventRecord(s->wready[k & 1], s->wstream);
+    if (e == hipSuccess && !s->rec) e = hipEventRecord(s->wready[k & 1], s->wstream);
     return e;
 }
 
@@ -1798,13 +1807,14 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
             s->tt += s->tau;
             return FRM_OK;
         };
-        // kind psi: the weights overlap the previous step unless steps are
-        // recorded for a graph or launches are timed one by one
-        const bool ahead = s->kind == FRM_MT_PSI && s->wstream && !s->rec && !ctx->timing_mask;
+        // kind psi: the weights overlap the previous step unless launches are
+        // timed one by one (recorded for a graph: a branch of it, which the
+        // next step's first launch depends on)
+        const bool ahead = s->kind == FRM_MT_PSI && s->wstream && !ctx->timing_mask;
         if (s->kind == FRM_MT_PSI) {
             if (s->wahead == k) {
-                FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[k & 1], 0));
-            } else if (ahead) {
+                if (!s->rec) FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[k & 1], 0));
+            } else if (ahead && !s->rec) {
                 FRM_HIP(ctx, mt_launch_psi_weights_ahead(s, k, depth));
                 FRM_HIP(ctx, hipStreamWaitEvent(ctx->stream, s->wready[k & 1], 0));
             } else {
@@ -1866,12 +1876,13 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         if (int rc = fit_after()) return rc;
         return FRM_OK;
     };
-    // Graph mode: each step's launches are recorded, matched with the graph
-    // built for the same kernel sequence (a chain of kernel nodes,
-    // instantiated once), the nodes get this step's arguments and grids
-    // (hipGraphExecKernelNodeSetParams) and the graph is launched.  Steps
-    // differ only in arguments and grids (depth, time-chunk count, slots), so
-    // a run uses a handful of graphs.  Not while per-launch timing is on.
+    // Graph mode: the launches of up to kGraphSteps steps are recorded,
+    // matched with the graph built for the same kernel sequence (a chain of
+    // kernel nodes, instantiated once), the nodes get these steps' arguments
+    // and grids (hipGraphExecKernelNodeSetParams) and the graph is launched
+    // once.  Runs of steps differ only in arguments and grids (depth,
+    // time-chunk count, slots), so a run uses a handful of graphs.  Not while
+    // per-launch timing is on.
     if (!s->graph || ctx->timing_mask != 0) {
         for (int it = 0; it < nsteps; ++it)
             if (int rc = step_once()) return rc;
@@ -1881,73 +1892,139 @@ int frm_mt1d_step(frm_mt1d* s, int32_t nsteps) {
         return FRM_OK;
     }
     std::vector<KRec> recs;
-    for (int it = 0; it < nsteps; ++it) {
+    std::vector<void*> ptrs;
+    for (int it = 0; it < nsteps;) {
         recs.clear();
         const int tstep0 = s->tstep, depth0 = s->depth;
         const double tt0 = s->tt;
+        // record steps until the run is full; a step that fails is taken
+        // back (the solver stays where it was before it) and ends the run
+        int done = 0, rc = FRM_OK;
+        const int wahead0 = s->wahead;
         s->rec = &recs;
-        const int rc = step_once();
-        s->rec = nullptr;
-        // a step that is not launched leaves the solver where it was
-        auto unstep = [&](int code) {
-            s->tstep = tstep0;
-            s->depth = depth0;
-            s->tt = tt0;
-            return code;
-        };
-        if (rc) return unstep(rc);
-        StepGraph* g = nullptr;
-        for (auto& c : s->graphs) {
-            if (c.funcs.size() != recs.size()) continue;
-            bool same = true;
-            for (size_t i = 0; i < recs.size() && same; ++i) same = c.funcs[i] == recs[i].func;
-            if (same) {
-                g = &c;
+        for (; done < kGraphSteps && it + done < nsteps; ++done) {
+            const size_t mark = recs.size();
+            const int ts = s->tstep, dp = s->depth, wa = s->wahead;
+            const double tt = s->tt;
+            s->rec_step = done;
+            rc = step_once();
+            if (rc) {
+                recs.resize(mark);
+                s->tstep = ts;
+                s->depth = dp;
+                s->tt = tt;
+                s->wahead = wa;
                 break;
             }
         }
-        void* ptrs[16];
-        auto params = [&](KRec& r) {
-            r.ptrs(ptrs);
-            hipKernelNodeParams p{};
-            p.blockDim = r.block;
-            p.extra = nullptr;
-            p.func = const_cast<void*>(r.func);
-            p.gridDim = r.grid;
-            p.kernelParams = ptrs;
-            p.sharedMemBytes = 0;
-            return p;
+        s->rec = nullptr;
+        // none of the run's steps is launched: the solver goes back to its start
+        auto unrun = [&](int code) {
+            s->tstep = tstep0;
+            s->depth = depth0;
+            s->tt = tt0;
+            s->wahead = wahead0;
+            return code;
         };
-        if (!g) {
-            StepGraph ng;
-            FRM_HIP(ctx, hipGraphCreate(&ng.graph, 0));
-            hipError_t e = hipSuccess;
-            for (size_t i = 0; i < recs.size() && e == hipSuccess; ++i) {
-                hipKernelNodeParams p = params(recs[i]);
-                hipGraphNode_t node = nullptr;
-                e = hipGraphAddKernelNode(&node, ng.graph, i ? &ng.nodes.back() : nullptr, i ? 1 : 0,
-                                          &p);
-                ng.nodes.push_back(node);
-                ng.funcs.push_back(recs[i].func);
+        if (!recs.empty()) {
+            StepGraph* g = nullptr;
+            for (auto& c : s->graphs) {
+                if (c.funcs.size() != recs.size()) continue;
+                bool same = true;
+                for (size_t i = 0; i < recs.size() && same; ++i)
+                    same = c.funcs[i] == recs[i].func && c.tags[i] == recs[i].step * 2 + recs[i].branch;
+                if (same) {
+                    g = &c;
+                    break;
+                }
             }
-            if (e == hipSuccess) e = hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) {
-                if (ng.exec) (void)hipGraphExecDestroy(ng.exec);
-                (void)hipGraphDestroy(ng.graph);
-                return unstep(check_hip(ctx, e, "frm_mt1d_step: step graph"));
+            ptrs.resize(16 * recs.size());
+            auto params = [&](size_t i) {
+                void** pp = ptrs.data() + 16 * i;
+                recs[i].ptrs(pp);
+                hipKernelNodeParams q{};
+                q.blockDim = recs[i].block;
+                q.extra = nullptr;
+                q.func = const_cast<void*>(recs[i].func);
+                q.gridDim = recs[i].grid;
+                q.kernelParams = pp;
+                q.sharedMemBytes = 0;
+                return q;
+            };
+            if (!g) {
+                // Edges: each step's own launches in order.  The weights
+                // formed ahead in step j (for step j + 1, into the buffers
+                // step j - 1 read) start after step j - 1's last launch (or
+                // after step j's weights when step j formed them itself) and
+                // after the weights formed ahead in step j - 1 (the weight
+                // kernels share the fallback queue, so they stay in order);
+                // step j + 1's first launch waits for them.  Without weights
+                // formed ahead the graph is a chain.
+                StepGraph ng;
+                FRM_HIP(ctx, hipGraphCreate(&ng.graph, 0));
+                hipError_t e = hipSuccess;
+                hipGraphNode_t main_last = nullptr, br_last = nullptr;
+                std::vector<hipGraphNode_t> step_last(done + 1, nullptr), step_br(done + 1, nullptr);
+                int cur = -1;
+                for (size_t i = 0; i < recs.size() && e == hipSuccess; ++i) {
+                    const KRec& r = recs[i];
+                    hipGraphNode_t dep[2];
+                    size_t nd = 0;
+                    if (r.branch) {
+                        if (step_br[r.step]) {
+                            dep[nd++] = step_br[r.step];
+                        } else {
+                            if (cur == r.step && main_last) dep[nd++] = main_last;
+                            else if (r.step > 0 && step_last[r.step - 1]) dep[nd++] = step_last[r.step - 1];
+                            if (br_last) dep[nd++] = br_last;
+                        }
+                    } else {
+                        if (main_last) dep[nd++] = main_last;
+                        if (r.step != cur && r.step > 0 && step_br[r.step - 1]) dep[nd++] = step_br[r.step - 1];
+                    }
+                    hipKernelNodeParams q = params(i);
+                    hipGraphNode_t node = nullptr;
+                    e = hipGraphAddKernelNode(&node, ng.graph, nd ? dep : nullptr, nd, &q);
+                    if (r.branch) {
+                        br_last = node;
+                        step_br[r.step] = node;
+                    } else {
+                        main_last = node;
+                        step_last[r.step] = node;
+                        cur = r.step;
+                    }
+                    ng.nodes.push_back(node);
+                    ng.funcs.push_back(r.func);
+                    ng.tags.push_back(r.step * 2 + r.branch);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&ng.exec, ng.graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    if (ng.exec) (void)hipGraphExecDestroy(ng.exec);
+                    (void)hipGraphDestroy(ng.graph);
+                    return unrun(check_hip(ctx, e, "frm_mt1d_step: step graph"));
+                }
+                s->graphs.push_back(ng);
+                g = &s->graphs.back();
+            } else {
+                for (size_t i = 0; i < recs.size(); ++i) {
+                    hipKernelNodeParams q = params(i);
+                    const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &q);
+                    if (e != hipSuccess) return unrun(check_hip(ctx, e, "frm_mt1d_step: node update"));
+                }
             }
-            s->graphs.push_back(ng);
-            g = &s->graphs.back();
-        } else {
-            for (size_t i = 0; i < recs.size(); ++i) {
-                hipKernelNodeParams p = params(recs[i]);
-                const hipError_t e = hipGraphExecKernelNodeSetParams(g->exec, g->nodes[i], &p);
-                if (e != hipSuccess) return unstep(check_hip(ctx, e, "frm_mt1d_step: node update"));
-            }
+            const hipError_t e = hipGraphLaunch(g->exec, ctx->stream);
+            if (e != hipSuccess) return unrun(check_hip(ctx, e, "frm_mt1d_step: graph launch"));
         }
-        const hipError_t e = hipGraphLaunch(g->exec, ctx->stream);
-        if (e != hipSuccess) return unstep(check_hip(ctx, e, "frm_mt1d_step: graph launch"));
+        if (rc) return rc;
+        it += done;
     }
+    // the weight buffers' events, for steps launched eagerly later: every
+    // graph so far is ahead of them on the context's stream
+    if (s->wstream)
+        for (int b = 0; b < 2; ++b) {
+            FRM_HIP(ctx, hipEventRecord(s->wfree[b], ctx->stream));
+            FRM_HIP(ctx, hipEventRecord(s->wready[b], ctx->stream));
+        }
     return FRM_OK;
 }
 
