@@ -10,8 +10,9 @@ whole 4000 steps from t = 0 on the same inputs; their fields at step 4000
 give the max / RMS relative error against FP64.  The K = --steps timed
 steps (default 64) are a window at history depth T/2 = 2000 inside those
 runs, starting on a 64-step block boundary (1984); the MFMA form forms the
-next block's far product in 8 node slices at this size, one every 8 steps,
-so a window of K steps pays for ceil(K/8) slices.  The W = --warmup steps just before it
+next block's far product in node slices of about one round of workgroups
+(16 at this size on a 256-CU MI355X, one every 4 steps), so a window of K
+steps pays for the slices that start in it.  The W = --warmup steps just before it
 are the last untimed ones.  `value` is the window rate of the fastest form
 within 1e-6 of FP64; the whole-run rates stand beside it.
 
@@ -496,6 +497,7 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, T, precision, window=None, warmup=
     out = s.b_C
     fit = s.fit_error
     win["sweep_shape"] = s.sweep_shape
+    win["far_slices"] = s.far_slices
     s.close()
     return dt, out, kt, fit, win
 
@@ -528,7 +530,7 @@ def run_lod_forms(args, cfg_name):
     configured length T from t = 0; the K = --steps timed steps are a window
     at history depth T/2 (the run average, since per-step cost is linear in
     depth), started on a 64-step block boundary (the MFMA form's far
-    product is formed in 8 slices, one every 8 steps).  The line's value is the
+    product is formed in node slices, one every 64 / slices steps).  The line's value is the
     window rate of the fastest form within 1e-6 of FP64 (error taken at step
     T); the whole-run rates are reported beside it."""
     from theses_code_amd import _native as N
@@ -575,7 +577,7 @@ def run_lod_forms(args, cfg_name):
     tot_nodes, t_plain, tw_plain, kt_plain = 0, 0.0, 0.0, {}
     plain_win = {"launches": 0, "ms": 0.0, "bytes": 0.0}
     res = {f: dict(seconds=0.0, window=0.0, errs=[], rms=[], fits=[], kernels_ms={},
-                   far={"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0},
+                   far={"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0, "slices": []},
                    mem={"launches": 0, "ms": 0.0, "bytes": 0.0}, shapes=[]) for f in forms}
     for r in runs:
         sN, M = r["case"][0], r["case"][1]
@@ -604,14 +606,16 @@ def run_lod_forms(args, cfg_name):
             if f == "mfma":
                 # the far slices in the window: algorithmic bytes from the
                 # library (every settled slot read once, 4 B per node; the
-                # 128 F rows written once, 4 B per node and row); each slice
-                # covers 1/8 of the nodes, so the useful flops (2 x 128 per
-                # slot and node) are 64 x (bytes - 512 B x nodes / 8)
+                # 128 F rows written once, 4 B per node and row); a slice
+                # covers 1/slices of the nodes, so the useful flops (2 x 128
+                # per slot and node) are 64 x (bytes - 512 B x nodes / slices)
                 n_, ms_, by_ = w0["kernels"]["blockfar"]
+                sl = max(1, w0["far_slices"])
                 q["far"]["launches"] += n_
                 q["far"]["ms"] += ms_
                 q["far"]["bytes"] += by_
-                q["far"]["flops"] += 64.0 * (by_ - 512.0 * nodes * n_ / 8)
+                q["far"]["flops"] += 64.0 * (by_ - 512.0 * nodes * n_ / sl)
+                q["far"]["slices"].append(sl)
             if f in ("soe", "f32c"):
                 n_, ms_, by_ = w0["kernels"]["memterm"]
                 q["mem"]["launches"] += n_
@@ -687,8 +691,10 @@ def run_lod_forms(args, cfg_name):
                                   "note": "the K timed steps lie at history depth T/2 of "
                                           "whole runs from t = 0, starting on a 64-step block "
                                           "boundary; the MFMA form's far product is formed "
-                                          "in 8 node slices at these sizes (at least 1024 "
-                                          "workgroups each), one every 8 steps"},
+                                          "in node slices of about one round of workgroups "
+                                          "(two per CU), one every 64 / slices steps",
+                                  "mfma_far_slices": res["mfma"]["far"]["slices"]
+                                  if "mfma" in res else None},
                        "parallelism": "1 GPU"},
             "chosen_form": dict(chosen, name=best),
             "forms": out_forms,
@@ -707,8 +713,9 @@ def run_lod_forms(args, cfg_name):
             "frac_of_measured_read": a / read_gbs if read_gbs else None,
             "traffic": ratio * far["bytes"] / far["launches"] if ratio else None,
             "traffic_source": f"stored PMC ratio ({src}) x algorithmic bytes" if ratio else None,
-            "kernel": f"k_block_far<{nprod}> (far product of the MFMA form, one node slice of 8 "
-                      "per launch, every 8 steps)",
+            "kernel": f"k_block_far<{nprod}> (far product of the MFMA form, one node slice per "
+                      f"launch: {sorted(set(far['slices']))} slices per block, one every 64 / "
+                      "slices steps)",
             "launches": far["launches"], "avg_launch_ms": far["ms"] / far["launches"],
             "bytes_per_launch": far["bytes"] / far["launches"],
             "bytes_model": "4 B x settled slots x slice nodes read + 4 B x 128 rows x slice "

@@ -236,6 +236,11 @@ int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out);
 /* Launch shapes of the sweeps in use: out[0..3] = elements per lane and
  * wavefronts per line of the row sweep, then of the column sweep. */
 int frm_lod2d_sweep_shape(const frm_lod2d* s, int32_t* out);
+/* Node slices in which the blocked Toeplitz form (FRM_PREC_MFMA_BF16)
+ * forms a 64-step block's far product during the block before it, one
+ * every 64 / slices steps; each slice is about one round of the product's
+ * workgroups on the device (0 for the other forms). */
+int frm_lod2d_far_slices(const frm_lod2d* s, int32_t* slices);
 
 /* ------------------------- batched 1D H_solver (mt_minimal / mt_psi_caputo)
  * A batch of independent 1D problems stepped together, each one

@@ -124,6 +124,7 @@ SIGNATURES = {
     "frm_diag_psi_kb_quad": (_I, [_V, ct.c_int32, _P, _P]),
     "frm_lod2d_fit_error": (_I, [_V, _P]),
     "frm_lod2d_sweep_shape": (_I, [_V, ct.POINTER(ct.c_int32)]),
+    "frm_lod2d_far_slices": (_I, [_V, ct.POINTER(ct.c_int32)]),
     "frm_lod2d_weights": (_I, [_V, ct.c_int32, _P]),
     "frm_mt1d_create": (_I, [_V, ct.POINTER(MT1DDesc), ct.POINTER(_V)]),
     "frm_mt1d_destroy": (_I, [_V]),

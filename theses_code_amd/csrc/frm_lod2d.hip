@@ -86,7 +86,7 @@ struct frm_lod2d {
     void* atab = nullptr;      // weight-operand fragments (atab_build)
     int atab_chunks = 0;
     int mfma_nprod = 6;        // 3 (two parts) or 6 (three parts)
-    int far_slices = 8;        // node slices of a block's far product (1, 2, 4 or 8)
+    int far_slices = 8;        // node slices of a block's far product (1 to 64)
     float* F = nullptr;        // [2][128][ld]: far parts of this block and the next
     float* G = nullptr;        // [16][ld] mid part of the current 8-step sub-block
     // g(t) = t^p (g_func 3): the weights and the sweeps' Thomas coefficients
@@ -1137,11 +1137,18 @@ int frm_lod2d_create(frm_ctx* ctx, const frm_lod2d_desc* d, frm_lod2d** out) {
         }
         // d = 32 .. N0 + 32 for the largest block start N0 below the capacity
         s->atab_chunks = std::max(1, (d->capacity - 1) / kBlockSteps * kBlockSteps / kFarChunk) + 1;
-        // each slice keeps at least 1024 workgroups (above the 768 the chip
-        // holds at once), so a smaller grid takes fewer, larger slices
+        // slices of about one round of far-product workgroups (two per CU:
+        // a launch of whole rounds keeps every CU busy to its end), at most
+        // one per step
         {
             const int groups = block_far_groups(s->hist->ld);
-            s->far_slices = groups >= 8 * 1024 ? 8 : groups >= 4 * 1024 ? 4 : groups >= 2 * 1024 ? 2 : 1;
+            const int round = block_far_round(ctx->device);
+            if (round <= 0) {
+                lod_free(s);
+                return set_error(ctx, FRM_E_HIP, "frm_lod2d_create: CU count of device %d",
+                                 ctx->device);
+            }
+            s->far_slices = std::max(1, std::min(kBlockSteps, (groups + round - 1) / round));
         }
         const std::vector<uint16_t> tab = atab_build(parts.data(), d->capacity, s->atab_chunks);
         e = hipMalloc(&s->atab, tab.size() * sizeof(uint16_t));
@@ -1398,6 +1405,12 @@ int frm_lod2d_weights(const frm_lod2d* s, int32_t n, double* out) {
         out[3 + k] = s->hw[3 + k];
         out[3 + n + k] = s->hw[3 + s->capacity + k];
     }
+    return FRM_OK;
+}
+
+int frm_lod2d_far_slices(const frm_lod2d* s, int32_t* slices) {
+    if (!s || !slices) return set_error(nullptr, FRM_E_INVALID, "frm_lod2d_far_slices: null");
+    *slices = s->precision == FRM_PREC_MFMA_BF16 ? s->far_slices : 0;
     return FRM_OK;
 }
 

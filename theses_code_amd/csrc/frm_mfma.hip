@@ -8,9 +8,9 @@
 // dense product  F[t*64 + j][i] = sum_{s < P0} W[t*64 + j][s] Delta_s[i]  with
 // M = 128, K = P0, N = nodes -- read once per 64 steps instead of every step.
 // Its slots are settled one block before it is needed, so it is formed
-// during the previous block in up to 8 node slices of at least 1024
-// workgroups, one every 64 / slices steps: the cost is spread evenly over
-// the steps instead of falling on the block's first.
+// during the previous block in node slices of about one round of
+// workgroups each (two per CU), one every 64 / slices steps: the cost is
+// spread evenly over the steps instead of falling on the block's first.
 // The reference forms every step's sum from scratch (fr_d_t_g_2d.cpp:
 // 3340-3437).  W and Delta are split into two or three BF16 parts whose
 // cross products accumulate in FP32 on v_mfma_f32_16x16x32_bf16.  The rest
@@ -30,11 +30,21 @@ constexpr int kKc = 32;   // slots per MFMA k-chunk
 constexpr int kNt = 128;  // nodes per workgroup (8 column tiles)
 constexpr int kBpad = 40; // LDS row length of a node's 32 slots (bf16), padded
 static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block in pairs");
+constexpr int kFarWgPerCu = 2;  // k_block_far workgroups resident per CU
 
-// Workgroup: 4 wavefronts, 128 rows (2 targets x 64 steps) x 128 nodes.
-// Wavefront w owns row tiles 2w, 2w+1 (16 rows each) and all 8 column
-// tiles; lane l holds A[l%16][8(l/16)+q], B[8(l/16)+q][l%16] and
-// D[4(l/16)+r][l%16] of each 16x16x32 product.
+// Workgroup: 8 wavefronts for a tile of 128 rows (2 targets x 64 steps) x
+// 128 nodes.  Wavefronts 0-3 run the products: wavefront w owns row tiles
+// r0, r0 + 2 (16 rows each, r0 = 0, 1, 4, 5) and all 8 column tiles; lane l
+// holds A[l%16][8(l/16)+q], B[8(l/16)+q][l%16] and D[4(l/16)+r][l%16] of
+// each 16x16x32 product.  Wavefronts 4-7 only stream the history: they keep
+// two 32-slot chunks in flight in registers, split each into BF16 parts and
+// store it into one of two LDS buffers, which the product wavefronts read;
+// one barrier per chunk hands a buffer over.  With the stream and the
+// products in different wavefronts neither waits for the other except at
+// the barrier (measured on 2^20 nodes at depth 2048: 5.1-5.2 TB/s with 512
+// workgroups per launch, one round at two per CU, against 4.4-4.5 for the
+// earlier kernel, whose four wavefronts each loaded, split and multiplied;
+// F is bit-identical).
 //
 // Operands are split into BF16 parts x = x0 + x1 (+ x2): NP = 3 forms
 // x0 y0 + x0 y1 + x1 y0 (relative error ~2^-16); NP = 6 adds x0 y2 + x2 y0 +
@@ -43,58 +53,89 @@ static_assert(kBlockSteps == 2 * kKc, "k_block_far takes the chunks of a block i
 //
 // The history chunk (32 slots x 128 nodes, FP32) is loaded with
 // non-temporal loads (the stream is read once; the weight table stays in
-// L2): thread (g = tid % 64, q = tid / 64) reads slots 8q..8q+7 of nodes g
-// and g + 64 (four bytes per lane, 256 contiguous bytes per wavefront load),
-// splits them into BF16 parts and stores each node's 8 slots as one 16-byte
-// LDS write into node-major rows [node][slot].  With 80-byte rows, the 64
-// nodes of one store fall on all 8 groups of 4 banks: the earlier layout
-// (float4 loads, 4 nodes x 4 slots per thread, 8-byte stores 4 rows apart)
-// left 75% of the LDS cycles to bank conflicts and ran 10% slower.  Two chunks are
-// in flight: the loop is unrolled by two over register buffers x (even
-// chunks) and y (odd chunks), each refilled right after its chunk's LDS
-// store, with the weight loads issued first so that waiting for them never
-// waits for the history behind them.
+// L2): stream thread (g = t % 64, q = t / 64) reads slots 8q..8q+7 of nodes
+// g and g + 64 (four bytes per lane, 256 contiguous bytes per wavefront
+// load), splits them into BF16 parts and stores each node's 8 slots as one
+// 16-byte LDS write into node-major rows [node][slot].  With 80-byte rows,
+// the 64 nodes of one store fall on all 8 groups of 4 banks (a layout with
+// 8-byte stores 4 rows apart spent 75% of the LDS cycles in bank conflicts).
+// Loads are unguarded: K is a multiple of two chunks (block_far checks it),
+// and a guarded load compiles to a select between global and private
+// addresses and a flat load, which serialises the prefetch.
 //
 // The weight operand of a chunk depends only on d = N0 - s0 (Toeplitz): its
 // fragments come ready-made from a table built once per solver (atab_build).
 // Row tile rt + 2 of target t at lag base d equals row tile rt at d + 32
 // (rows j and j + 32, same k), i.e. the fragment rt held one chunk earlier;
-// so wavefront w owns row tiles r0 and r0 + 2 (r0 = 0, 1, 4, 5) and loads
-// one fragment per chunk, keeping the previous one.  Both forms are held to
-// 168 registers, three workgroups per CU (150 and 161 registers without
-// spills since the loader stores 16-byte pieces).
+// so a product wavefront loads one fragment per chunk, a chunk ahead, and
+// keeps the previous one.
 template <int NP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_block_far(
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_block_far(
     const float* __restrict__ hist, int64_t ld, int N0, int K, const bf16x8* __restrict__ atab,
     float* __restrict__ F, int wg0) {
     constexpr int NS = NP == 6 ? 3 : 2;    // BF16 parts per operand
     constexpr int CT = kNt / 16;           // column tiles per workgroup
-    static_assert(kNt == 128 && kKc == 32, "the loader map takes 2 x 64 nodes, 4 x 8 slots");
-    __shared__ __attribute__((aligned(16))) __bf16 sB[NS][kNt][kBpad];
+    static_assert(kNt == 128 && kKc == 32, "the stream map takes 2 x 64 nodes, 4 x 8 slots");
+    __shared__ __attribute__((aligned(16))) __bf16 sB[2][NS][kNt][kBpad];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t i0 = (int64_t)(blockIdx.x + wg0) * kNt;
+    const int nch = K / kKc;  // even
+    if (wave >= 4) {
+        // stream wavefronts: chunk c + 1 is split into buffer (c + 1) & 1
+        // while the products run on chunk c; chunk c + 3 is then loaded
+        const int t = tid - 256, ng = t & 63, sq = t >> 6;
+        float x[8][2], y[8][2];
+        auto load = [&](float (&d)[8][2], int c) {
+            const float* q = hist + (int64_t)(c * kKc + 8 * sq) * ld + i0 + ng;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                d[u][0] = __builtin_nontemporal_load(q + (int64_t)u * ld);
+                d[u][1] = __builtin_nontemporal_load(q + (int64_t)u * ld + 64);
+            }
+        };
+        auto split = [&](float (&d)[8][2], int b) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float r[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) r[u] = d[u][h];
+#pragma unroll
+                for (int p = 0; p < NS; ++p) {
+                    bf16x8 v;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        v[u] = (__bf16)r[u];
+                        r[u] -= (float)v[u];
+                    }
+                    *reinterpret_cast<bf16x8*>(&sB[b][p][ng + 64 * h][8 * sq]) = v;
+                }
+            }
+        };
+        load(x, 0);
+        load(y, 1);
+        split(x, 0);
+        if (2 < nch) load(x, 2);
+        __syncthreads();
+        // the barriers pair with the product wavefronts' (same count)
+        for (int c = 0; c < nch; c += 2) {
+            split(y, 1);  // chunk c + 1
+            if (c + 3 < nch) load(y, c + 3);
+            __syncthreads();
+            if (c + 2 < nch) {
+                split(x, 0);  // chunk c + 2
+                if (c + 4 < nch) load(x, c + 4);
+            }
+            __syncthreads();
+        }
+        return;
+    }
     const int kg = lane >> 4, lc = lane & 15;
     const int r0 = (wave & 1) + 4 * (wave >> 1);  // row tiles r0, r0 + 2
-    const int64_t i0 = (int64_t)(blockIdx.x + wg0) * kNt;
     f32x4 acc[2][CT];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < CT; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // loader map: slots 8*sq .. 8*sq+7 of the chunk for nodes ng and ng + 64.
-    // K is a multiple of two chunk lengths (block_far checks it), so every
-    // chunk is whole and the loads need no guard: a guarded load compiles to
-    // a select between global and private addresses and a flat load, which
-    // serialises the prefetch.
-    const int ng = tid & 63, sq = tid >> 6;
-    float x[8][2], y[8][2];
-    auto load = [&](float (&d)[8][2], int s0) {
-        const float* q = hist + (int64_t)(s0 + 8 * sq) * ld + i0 + ng;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            d[u][0] = __builtin_nontemporal_load(q + (int64_t)u * ld);
-            d[u][1] = __builtin_nontemporal_load(q + (int64_t)u * ld + 64);
-        }
-    };
     // an: fragment of row tile r0 for the chunk at s0 (d = N0 - s0); acar:
     // the one of the chunk before, which is row tile r0 + 2's for this chunk
     bf16x8 an[NS], acar[NS];
@@ -103,9 +144,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
         for (int p = 0; p < NS; ++p) dst[p] = at[p * 64];
     };
-    // one chunk: BF16 parts of d into LDS, then (after the barrier) the next
-    // loads, then the products
-    auto chunk = [&](float (&d)[8][2], int next_a, int next_h) {
+    afrag(acar, N0 + kKc);  // row tile r0 + 2 of the first chunk
+    afrag(an, N0);
+    auto chunk = [&](int c, int b) {
         bf16x8 ap[2][NS];
 #pragma unroll
         for (int p = 0; p < NS; ++p) {
@@ -113,63 +154,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             ap[1][p] = acar[p];
             acar[p] = an[p];
         }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float r[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) r[u] = d[u][h];
-#pragma unroll
-            for (int p = 0; p < NS; ++p) {
-                bf16x8 v;
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    v[u] = (__bf16)r[u];
-                    r[u] -= (float)v[u];
-                }
-                *reinterpret_cast<bf16x8*>(&sB[p][ng + 64 * h][8 * sq]) = v;
-            }
-        }
-        __syncthreads();
-        if (next_a >= 0) afrag(an, N0 - next_a);
-        __builtin_amdgcn_sched_barrier(0);
-        if (next_h >= 0) load(d, next_h);
-        __builtin_amdgcn_sched_barrier(0);
+        if (c + 1 < nch) afrag(an, N0 - (c + 1) * kKc);
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct) {
             bf16x8 bp[NS];
 #pragma unroll
             for (int p = 0; p < NS; ++p)
-                bp[p] = *reinterpret_cast<const bf16x8*>(&sB[p][ct * 16 + lc][kg * 8]);
+                bp[p] = *reinterpret_cast<const bf16x8*>(&sB[b][p][ct * 16 + lc][kg * 8]);
 #pragma unroll
             for (int rr = 0; rr < 2; ++rr) {
-                f32x4 c = acc[rr][ct];
+                f32x4 cc = acc[rr][ct];
                 if constexpr (NP == 6) {  // small terms first
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[1], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[2], c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][2], bp[0], c, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[1], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[2], cc, 0, 0, 0);
+                    cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][2], bp[0], cc, 0, 0, 0);
                 }
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[1], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[0], c, 0, 0, 0);
-                c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[0], c, 0, 0, 0);
-                acc[rr][ct] = c;
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[1], cc, 0, 0, 0);
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][1], bp[0], cc, 0, 0, 0);
+                cc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rr][0], bp[0], cc, 0, 0, 0);
+                acc[rr][ct] = cc;
             }
         }
-        __syncthreads();
     };
-    afrag(acar, N0 + kKc);  // row tile r0 + 2 of the first chunk
-    afrag(an, N0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(x, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(y, kKc);
-    __builtin_amdgcn_sched_barrier(0);
-    int s0 = 0;
-    for (; s0 + 2 * kKc < K; s0 += 2 * kKc) {
-        chunk(x, s0 + kKc, s0 + 2 * kKc);
-        chunk(y, s0 + 2 * kKc, s0 + 3 * kKc);
+    __syncthreads();
+    for (int c = 0; c < nch; c += 2) {
+        chunk(c, 0);
+        __syncthreads();
+        chunk(c + 1, 1);
+        __syncthreads();
     }
-    chunk(x, s0 + kKc, -1);  // the last pair loads no history
-    chunk(y, -1, -1);
 #pragma unroll
     for (int rr = 0; rr < 2; ++rr) {
         const int rt = r0 + 2 * rr;
@@ -317,7 +330,7 @@ int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, int K, cons
         N0 / kKc + 1 > atab_chunks || (nprod != 3 && nprod != 6) || wg0 < 0 || nwg <= 0 ||
         (int64_t)wg0 + nwg > ld / kNt)
         return -1;
-    const dim3 grid((unsigned)nwg), block(256);
+    const dim3 grid((unsigned)nwg), block(512);
     const bf16x8* a = (const bf16x8*)atab;
     if (nprod == 6)
         hipLaunchKernelGGL(k_block_far<6>, grid, block, 0, st, hist, ld, N0, K, a, F, wg0);
@@ -327,6 +340,14 @@ int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, int K, cons
 }
 
 int block_far_groups(int64_t ld) { return (int)(ld / kNt); }
+
+int block_far_round(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        cus <= 0)
+        return -1;
+    return kFarWgPerCu * cus;
+}
 
 // Host: the weight-operand fragments for d = 32, 64, ..., 32*chunks.  Entry
 // [(d/32 - 1)*8 + rt][p][lane] holds, for row tile rt (target t = rt/4, steps

@@ -21,6 +21,9 @@ constexpr int kFarChunk = 32;    // history slots per product k-chunk
 int block_far(hipStream_t st, const float* hist, int64_t ld, int N0, int K, const void* atab,
               int atab_chunks, int nprod, float* F, int wg0, int nwg);
 int block_far_groups(int64_t ld);
+// Workgroups of block_far resident on the whole of `device` at once (one
+// round); -1 if the device cannot be queried.
+int block_far_round(int device);
 // Host: weight-fragment table for chunks d = 32 .. 32*chunks from the BF16
 // parts of the weights, wparts[(p*2 + t)*ldw + lag - 1], p < 3.
 std::vector<uint16_t> atab_build(const float* wparts, int ldw, int chunks);

@@ -100,6 +100,13 @@ class TGSolver:
         return v.value
 
     @property
+    def far_slices(self) -> int:
+        """Node slices of the MFMA form's far product per 64-step block (0: other forms)."""
+        v = ct.c_int32()
+        N._check(N.lib().frm_lod2d_far_slices(self.h, ct.byref(v)), self.ctx.h)
+        return v.value
+
+    @property
     def sweep_shape(self):
         """(elements per lane, wavefronts per line) of the row and the column sweep."""
         out = (ct.c_int32 * 4)()
