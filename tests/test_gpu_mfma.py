@@ -154,3 +154,25 @@ def test_mfma_long_history(ctx):
     assert mx <= 1e-6 and rms <= 1e-6
     full.close()
     mf.close()
+
+
+# Node slices of the far product: about one round of workgroups each, so a
+# grid whose workgroup count is not a multiple of the round gets a slice
+# count that need not divide 64 and a last slice smaller than the others
+# (here 700 x 600 interior nodes: about 3290 workgroups of 128 nodes, 7
+# slices on a 256-CU device, one every 9 steps; random velocities at cell
+# Peclet number 0.03).  Over four blocks the form stays within the
+# chosen-form bar of the FP64 path.
+def test_mfma_uneven_far_slices(ctx):
+    steps = 300
+    full, mf = _pair(ctx, 701, 601, 0.7, steps, D=5e-3)
+    sl = mf.far_slices
+    print(f"far slices per block: {sl}")
+    assert 1 <= sl <= 64 and full.far_slices == 0
+    full.calc_step(steps)
+    mf.calc_step(steps)
+    mx, rms = errs(mf.b_C, full.b_C)
+    print(f"uneven slices: max {mx:.3e} rms {rms:.3e}")
+    assert mx <= 1e-6 and rms <= 1e-6
+    full.close()
+    mf.close()
