@@ -7,11 +7,11 @@
  * kernel that scales rocprof FETCH_SIZE / WRITE_SIZE counters.
  *
  * frm_diag_read streams `bytes` (a multiple of 8 MiB) of device memory as
- * rows of 8 MiB, each lane summing one 16-byte column over all rows with
- * non-temporal loads, four rows in flight, and writing its sum once (the
- * access pattern of the history sums): the measured read bandwidth, which
- * the read-dominated kernels are compared against.  It allocates its own
- * buffer of per-lane sums (8 MiB).
+ * rows of 8 MiB, each lane summing one 4-byte column over all rows with
+ * non-temporal loads, 16 rows in flight, and writing its sum once (the
+ * fastest read pattern measured on MI355X): the measured read bandwidth,
+ * which the read-dominated kernels are compared against.  It allocates its
+ * own buffer of per-lane sums (8 MiB).
  *
  * frm_diag_psi_kb_quad runs the device restatement of the psi-Caputo
  * weights' quadrature fallback, _kb(1, t0, t1, gtj, sing, alpha)
