@@ -95,3 +95,15 @@ def test_hist_capacity_error(ctx):
     assert N.lib().frm_hist_push(ctx.h, h, a.ptr, a.ptr) == N.FRM_E_RANGE
     N.lib().frm_hist_destroy(h)
     a.free()
+
+
+def test_diag_probes_run_and_check_sizes(ctx):
+    # the bandwidth probes behind the bench's measured figures: they run and
+    # return a positive rate; the read stream takes whole 8 MiB rows only
+    assert ctx.read_bandwidth(64 << 20, reps=2) > 0.0
+    assert ctx.copy_bandwidth(16 << 20, reps=2) > 0.0
+    src = ctx.malloc(12 << 20)
+    ms = N.ct.c_double()
+    rc = N.lib().frm_diag_read(ctx.h, src, 12 << 20, 1, N.ct.byref(ms))
+    ctx.free(src)
+    assert rc == N.FRM_E_INVALID

@@ -235,8 +235,9 @@ class Context:
         return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     def read_bandwidth(self, nbytes: int = 4 << 30, reps: int = 10) -> float:
-        """Measured device read bandwidth in GB/s (a non-temporal 16-byte
-        stream, four loads in flight per lane, one small write per lane)."""
+        """Measured device read bandwidth in GB/s (a non-temporal stream of
+        4-byte columns, 16 rows in flight per lane, one small write per lane);
+        nbytes a multiple of 8 MiB."""
         src = self.malloc(nbytes)
         ms = _D()
         try:
