@@ -148,12 +148,10 @@ __global__ __launch_bounds__(kMemtermThreads) void k_mt_memterm_f64(
 // FP32 increments, FP32 weights, Kahan-compensated FP32 sums; the chunk's
 // compensated sum is returned in FP64.  NPL nodes per lane: 2 (8-byte
 // loads, a wavefront is one 128-node group) or 4 (16-byte loads, each half
-// wavefront one group: the weights of both halves' problems are read
-// through the scalar cache and each lane picks its half's).  Large batches
-// take 4 (c3: 5.8 -> 6.x TB/s); small ones keep 2, whose twice as many
-// wavefronts fill the GPU with fewer time chunks.
-typedef float mt_fv2 __attribute__((ext_vector_type(2)));
-typedef float mt_fv4 __attribute__((ext_vector_type(4)));
+// wavefront one group, each lane taking its half's problem weights).  Large
+// batches take 4 (4096 uneven psi problems: 5.8 -> 6.4 TB/s); small ones
+// keep 2, whose twice as many wavefronts fill the GPU with fewer time
+// chunks.
 
 __device__ inline void kahan_add(float& s, float& c, float x) {
     const float y = x - c;
@@ -1109,7 +1107,8 @@ struct KRec {
     }
 };
 // The launches of up to kGraphSteps consecutive steps as an instantiated
-// graph (a chain of kernel nodes), reused for every run of steps that issues
+// graph (each step's kernel nodes in a chain; for kind psi the next step's
+// weights on a branch beside it), reused for every run of steps that issues
 // the same kernels in the same order: only the nodes' arguments and grids
 // change between launches.
 constexpr int kGraphSteps = 32;
