@@ -448,13 +448,15 @@ def run_ab1d(args, series=False):
 
 
 def _lod_run(ctx, N, case, alpha, tau, sigma, T, precision, window=None, warmup=0,
-             mask=0, timing=False, **kw):
+             mask=0, timing=False, stop_after_window=False, **kw):
     """One run of T steps from t = 0.
 
     `window` = (start, K): the steps start+1 .. start+K are timed on their
     own between two synchronisations, with HIP events on the launches of the
     kernel classes in `mask` (the rest of the run carries no events); the
     `warmup` steps just before the window are the last untimed ones.
+    `stop_after_window`: the run ends with the window (the whole-run time
+    and the field returned are then those of step start + K).
     `timing`: events on every launch of the run instead (a breakdown run;
     its wall time then includes the events).  Returns the whole run's wall
     time, the final field, the per-class event totals and the window's data."""
@@ -483,7 +485,8 @@ def _lod_run(ctx, N, case, alpha, tau, sigma, T, precision, window=None, warmup=
         win["seconds"] = time.perf_counter() - tw
         ctx.timing_enable(0)
         win["kernels"] = {name: ctx.timing(k) for name, k in classes if mask >> k & 1}
-        s.calc_step(T - start - K)
+        if not stop_after_window:
+            s.calc_step(T - start - K)
     else:
         s.calc_step(T)
     ctx.sync()
@@ -532,7 +535,11 @@ def run_lod_forms(args, cfg_name):
     depth), started on a 64-step block boundary (the MFMA form's far
     product is formed in node slices, one every 64 / slices steps).  The line's value is the
     window rate of the fastest form within 1e-6 of FP64 (error taken at step
-    T); the whole-run rates are reported beside it."""
+    T); the whole-run rates are reported beside it.  Each window is timed
+    twice on the same steps: in the whole run, with HIP events on the form's
+    dominant kernel (its launch durations give the roofline), and in a run
+    that stops after the window and records no events (the rate: two event
+    records per far-product launch cost about 3.5% of the window)."""
     from theses_code_amd import _native as N
     from theses_code_amd import workloads as Wl
 
@@ -574,9 +581,9 @@ def run_lod_forms(args, cfg_name):
     # far product (one launch per block) and the plain memory term (ms long)
     fmask = {"mfma": 1 << N.K_BLOCKFAR, "soe": 1 << N.K_MEMTERM, "f32c": 1 << N.K_MEMTERM,
              "binomial": 0}
-    tot_nodes, t_plain, tw_plain, kt_plain = 0, 0.0, 0.0, {}
+    tot_nodes, t_plain, tw_plain, tw_plain_ev, kt_plain = 0, 0.0, 0.0, 0.0, {}
     plain_win = {"launches": 0, "ms": 0.0, "bytes": 0.0}
-    res = {f: dict(seconds=0.0, window=0.0, errs=[], rms=[], fits=[], kernels_ms={},
+    res = {f: dict(seconds=0.0, window=0.0, window_events=0.0, errs=[], rms=[], fits=[], kernels_ms={},
                    far={"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0, "slices": []},
                    mem={"launches": 0, "ms": 0.0, "bytes": 0.0}, shapes=[]) for f in forms}
     for r in runs:
@@ -587,7 +594,9 @@ def run_lod_forms(args, cfg_name):
         dp, up, _, _, wp = _lod_run(*common, precision=N.PREC_F64, window=(start, K), warmup=W,
                                     mask=1 << N.K_MEMTERM, **kw)
         t_plain += dp
-        tw_plain += wp["seconds"]
+        tw_plain_ev += wp["seconds"]
+        tw_plain += _lod_run(*common, precision=N.PREC_F64, window=(start, K), warmup=W,
+                             stop_after_window=True, **kw)[4]["seconds"]
         n_, ms_, by_ = wp["kernels"]["memterm"]
         plain_win["launches"] += n_
         plain_win["ms"] += ms_
@@ -601,7 +610,14 @@ def run_lod_forms(args, cfg_name):
                                           **kw, **fkw[f])
             q = res[f]
             q["seconds"] += ds
-            q["window"] += w0["seconds"]
+            q["window_events"] += w0["seconds"]
+            if fmask[f]:
+                # the same window again, without events, for the rate
+                w0n = _lod_run(*common, window=(start, K), warmup=W, stop_after_window=True,
+                               **kw, **fkw[f])[4]
+                q["window"] += w0n["seconds"]
+            else:
+                q["window"] += w0["seconds"]
             q["shapes"].append(w0["sweep_shape"])
             if f == "mfma":
                 # the far slices in the window: algorithmic bytes from the
@@ -635,6 +651,7 @@ def run_lod_forms(args, cfg_name):
     for f in forms:
         q = res[f]
         o = {"form": FORMS[f], "value": tot_nodes * K / q["window"], "window_seconds": q["window"],
+             "window_seconds_with_events": q["window_events"],
              "value_whole_run": tot_nodes * T / q["seconds"], "seconds_whole_run": q["seconds"],
              "speedup_vs_plain_fp64": tw_plain / q["window"],
              "speedup_vs_plain_fp64_whole_run": t_plain / q["seconds"],
@@ -657,6 +674,7 @@ def run_lod_forms(args, cfg_name):
     ratio, src = _stored_ratio(f"memterm_lod2d_{'1024' if cfg_name == 'c2' else 'ab'}_traffic.json")
     plain = {"form": "plain FP64 full history (the drop-in)",
              "value": tot_nodes * K / tw_plain, "window_seconds": tw_plain,
+             "window_seconds_with_events": tw_plain_ev,
              "value_whole_run": tot_nodes * T / t_plain, "seconds_whole_run": t_plain,
              "kernels_ms_whole_run": kt_plain or None,
              "roofline": {"bound": "hbm", "achieved": pw_gbs, "peak": HBM_PEAK_GBS,
@@ -692,7 +710,10 @@ def run_lod_forms(args, cfg_name):
                                           "whole runs from t = 0, starting on a 64-step block "
                                           "boundary; the MFMA form's far product is formed "
                                           "in node slices of about one round of workgroups "
-                                          "(two per CU), one every 64 / slices steps",
+                                          "(two per CU), one every 64 / slices steps; the "
+                                          "rate is timed on a run without events, the "
+                                          "roofline's launch durations on the same window of "
+                                          "the whole run, with events",
                                   "mfma_far_slices": res["mfma"]["far"]["slices"]
                                   if "mfma" in res else None},
                        "parallelism": "1 GPU"},
