@@ -44,7 +44,10 @@ constexpr int kFarWgPerCu = 2;  // k_block_far workgroups resident per CU
 // the barrier (measured on 2^20 nodes at depth 2048: 5.1-5.2 TB/s with 512
 // workgroups per launch, one round at two per CU, against 4.4-4.5 for the
 // earlier kernel, whose four wavefronts each loaded, split and multiplied;
-// F is bit-identical).
+// F is bit-identical).  The history stream sets the pace: the stream
+// wavefronts' loads alone, without the split, the LDS stores or the
+// products, take the whole kernel's time, and neither a third chunk in
+// flight nor 16-byte loads change it.
 //
 // Operands are split into BF16 parts x = x0 + x1 (+ x2): NP = 3 forms
 // x0 y0 + x0 y1 + x1 y0 (relative error ~2^-16); NP = 6 adds x0 y2 + x2 y0 +
